@@ -1,0 +1,519 @@
+/*
+ * xs_oracle.c -- CPU oracle for the XaNSoNS scattering calculators.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product path): loaded by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ *
+ * Built with -O2 -fno-fast-math -ffp-contract=off so that every expression
+ * below is evaluated with IEEE rounding in the written order, as the
+ * reference's g++ -O3 x86-64 build does (no FMA in the baseline ISA).
+ * Function-by-function reference citations are in xs_oracle.h.
+ */
+#include "xs_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define XS_PI 3.14159265358979323846
+
+static int clamp_threads(int nthreads) { return nthreads < 1 ? 1 : nthreads; }
+
+static int thread_id(void) {
+#ifdef _OPENMP
+  return omp_get_thread_num();
+#else
+  return 0;
+#endif
+}
+
+unsigned or_pair_index(unsigned a, unsigned b, unsigned n_el) {
+  /* rows (0,0..n-1), (1,1..n-1), ...: row a starts after sum_{t<a}(n-t). */
+  return a * n_el - a * (a - 1) / 2 + (b - a);
+}
+
+/* The k-th row visited by team member t of T in the reference's zig-zag
+ * interleave (CalcFunctions.cpp:573-575): t, 2T-1-t, 2T+t, 4T-1-t, ... */
+static size_t zigzag_row(size_t k, int t, int T) {
+  size_t base = k * (size_t)T;
+  return base + ((k & 1) ? (size_t)(T - 1 - t) : (size_t)t);
+}
+
+unsigned or_hist_size(const double *xyz, size_t n, double bin) {
+  const double *x = xyz, *y = xyz + n, *z = xyz + 2 * n;
+  if (n == 0) return 1;
+  double lo[3] = {x[0], y[0], z[0]}, hi[3] = {x[0], y[0], z[0]};
+  for (size_t i = 0; i < n; i++) {
+    if (hi[0] < x[i]) hi[0] = x[i];
+    if (hi[1] < y[i]) hi[1] = y[i];
+    if (hi[2] < z[i]) hi[2] = z[i];
+    if (lo[0] > x[i]) lo[0] = x[i];
+    if (lo[1] > y[i]) lo[1] = y[i];
+    if (lo[2] > z[i]) lo[2] = z[i];
+  }
+  double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+  double diag = sqrt(dx * dx + dy * dy + dz * dz);
+  return (unsigned)(diag / bin) + 1;
+}
+
+double or_atomic_density(const double *xyz, size_t n) {
+  const double *x = xyz, *y = xyz + n, *z = xyz + 2 * n;
+  double c[3] = {0, 0, 0};
+  for (size_t i = 0; i < n; i++) {
+    c[0] += x[i];
+    c[1] += y[i];
+    c[2] += z[i];
+  }
+  c[0] /= (double)n;
+  c[1] /= (double)n;
+  c[2] /= (double)n;
+  double r2max = 0;
+  for (size_t i = 0; i < n; i++) {
+    double dx = c[0] - x[i], dy = c[1] - y[i], dz = c[2] - z[i];
+    double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 > r2max) r2max = d2;
+  }
+  double size = 2. * sqrt(r2max);
+  double rs = 0.33333 * size;
+  double rs2 = rs * rs;
+  unsigned count = 0;
+  for (size_t i = 0; i < n; i++) {
+    double dx = c[0] - x[i], dy = c[1] - y[i], dz = c[2] - z[i];
+    if (dx * dx + dy * dy + dz * dz < rs2) count++;
+  }
+  double R = sqrt(rs2);
+  return count / (4. / 3. * XS_PI * R * R * R);
+}
+
+/* Shared driver for both histogram precisions: each team member owns a
+ * private histogram copy and visits rows in the zig-zag order; the copies
+ * are summed afterwards (CalcFunctions.cpp:765-804). */
+#define HIST_BODY(T, SQRT)                                                     \
+  size_t ntot = 0;                                                             \
+  for (unsigned e = 0; e < n_el; e++) ntot += el_count[e];                     \
+  const unsigned ntypes = n_el * (n_el + 1) / 2;                               \
+  const size_t hsize = (size_t)ntypes * nhist;                                 \
+  const T *x = xyz, *y = xyz + ntot, *z = xyz + 2 * ntot;                      \
+  int nt = clamp_threads(nthreads);                                            \
+  uint64_t *priv = (uint64_t *)calloc((size_t)nt * (hsize + 1), sizeof(uint64_t)); \
+  if (!priv) return -1;                                                        \
+  _Pragma("omp parallel num_threads(nt)") {                                    \
+    int t = thread_id();                                                       \
+    uint64_t *h = priv + (size_t)t * (hsize + 1);                              \
+    uint64_t *ovf = h + hsize;                                                 \
+    size_t oa = 0;                                                             \
+    unsigned p = 0;                                                            \
+    for (unsigned a = 0; a < n_el; oa += el_count[a], a++) {                   \
+      size_t ob = oa;                                                          \
+      for (unsigned b = a; b < n_el; ob += el_count[b], b++, p++) {            \
+        uint64_t *hp = h + (size_t)p * nhist;                                  \
+        for (size_t k = 0;; k++) {                                             \
+          size_t i = zigzag_row(k, t, nt);                                     \
+          if (i >= el_count[a]) break;                                         \
+          T xi = x[oa + i], yi = y[oa + i], zi = z[oa + i];                    \
+          size_t j0 = (a == b) ? i + 1 : 0;                                    \
+          for (size_t j = j0; j < el_count[b]; j++) {                          \
+            T dx = xi - x[ob + j], dy = yi - y[ob + j], dz = zi - z[ob + j];   \
+            T s = dx * dx + dy * dy;                                           \
+            s = s + dz * dz;                                                   \
+            T r = SQRT(s);                                                     \
+            unsigned idx = (unsigned)(r / bin);                                \
+            if (idx < nhist)                                                   \
+              hp[idx] += 1;                                                    \
+            else                                                               \
+              *ovf += 1;                                                       \
+          }                                                                    \
+        }                                                                      \
+      }                                                                        \
+    }                                                                          \
+  }                                                                            \
+  memset(hist, 0, hsize * sizeof(uint64_t));                                   \
+  uint64_t ov = 0;                                                             \
+  for (int t = 0; t < nt; t++) {                                               \
+    const uint64_t *h = priv + (size_t)t * (hsize + 1);                        \
+    for (size_t i = 0; i < hsize; i++) hist[i] += h[i];                        \
+    ov += h[hsize];                                                            \
+  }                                                                            \
+  if (overflow) *overflow = ov;                                                \
+  free(priv);                                                                  \
+  return 0;
+
+int or_hist_f64(const double *xyz, const unsigned *el_count, unsigned n_el,
+                double bin, unsigned nhist, uint64_t *hist, uint64_t *overflow,
+                int nthreads) {
+  HIST_BODY(double, sqrt)
+}
+
+int or_hist_f32(const float *xyz, const unsigned *el_count, unsigned n_el,
+                float bin, unsigned nhist, uint64_t *hist, uint64_t *overflow,
+                int nthreads) {
+  HIST_BODY(float, sqrtf)
+}
+
+void or_debye_pairsum(const double *xyz, const unsigned *el_count, unsigned n_el,
+                      const double *q, unsigned nq, double *S, int nthreads) {
+  size_t ntot = 0;
+  for (unsigned e = 0; e < n_el; e++) ntot += el_count[e];
+  const double *x = xyz, *y = xyz + ntot, *z = xyz + 2 * ntot;
+  const unsigned ntypes = n_el * (n_el + 1) / 2;
+  int nt = clamp_threads(nthreads);
+  double *acc = (double *)calloc((size_t)nt * nq, sizeof(double));
+  memset(S, 0, (size_t)ntypes * nq * sizeof(double));
+  size_t oa = 0;
+  unsigned p = 0;
+  for (unsigned a = 0; a < n_el; oa += el_count[a], a++) {
+    size_t ob = oa;
+    for (unsigned b = a; b < n_el; ob += el_count[b], b++, p++) {
+#pragma omp parallel num_threads(nt)
+      {
+        int t = thread_id();
+        double *my = acc + (size_t)t * nq;
+        memset(my, 0, nq * sizeof(double));
+        for (size_t k = 0;; k++) {
+          size_t i = zigzag_row(k, t, nt);
+          if (i >= el_count[a]) break;
+          size_t j0 = (a == b) ? i + 1 : 0;
+          for (size_t j = j0; j < el_count[b]; j++) {
+            double dx = x[oa + i] - x[ob + j], dy = y[oa + i] - y[ob + j],
+                   dz = z[oa + i] - z[ob + j];
+            double r = sqrt(dx * dx + dy * dy + dz * dz);
+            for (unsigned iq = 0; iq < nq; iq++) {
+              double qr = r * q[iq];
+              my[iq] += sin(qr) / (qr + 0.00000001);
+            }
+          }
+        }
+      }
+      for (int t = 0; t < nt; t++)
+        for (unsigned iq = 0; iq < nq; iq++) S[(size_t)p * nq + iq] += acc[(size_t)t * nq + iq];
+    }
+  }
+  free(acc);
+}
+
+void or_partial_debye_pairsum(const double *xyz, const unsigned *el_count,
+                              unsigned n_el, const unsigned *blk_count,
+                              unsigned nblocks, const double *q, unsigned nq,
+                              double *S, int nthreads) {
+  size_t ntot = 0;
+  for (unsigned e = 0; e < n_el; e++) ntot += el_count[e];
+  const double *x = xyz, *y = xyz + ntot, *z = xyz + 2 * ntot;
+  const unsigned ntypes = n_el * (n_el + 1) / 2;
+  const unsigned nparts = nblocks * (nblocks + 1) / 2;
+  const size_t psize = (size_t)nparts * nq;
+  int nt = clamp_threads(nthreads);
+  double *acc = (double *)calloc((size_t)nt * psize, sizeof(double));
+  memset(S, 0, (size_t)ntypes * psize * sizeof(double));
+  /* block id of every atom, element by element (block-major inside each
+   * element, CalcFunctions.cpp:650-661) */
+  unsigned *blk = (unsigned *)malloc((ntot ? ntot : 1) * sizeof(unsigned));
+  {
+    size_t o = 0;
+    for (unsigned e = 0; e < n_el; e++)
+      for (unsigned b = 0; b < nblocks; b++)
+        for (unsigned k = 0; k < blk_count[e * nblocks + b]; k++) blk[o++] = b;
+  }
+  size_t oa = 0;
+  unsigned p = 0;
+  for (unsigned a = 0; a < n_el; oa += el_count[a], a++) {
+    size_t ob = oa;
+    for (unsigned b = a; b < n_el; ob += el_count[b], b++, p++) {
+#pragma omp parallel num_threads(nt)
+      {
+        int t = thread_id();
+        double *my = acc + (size_t)t * psize;
+        memset(my, 0, psize * sizeof(double));
+        for (size_t k = 0;; k++) {
+          size_t i = zigzag_row(k, t, nt);
+          if (i >= el_count[a]) break;
+          size_t j0 = (a == b) ? i + 1 : 0;
+          unsigned bi = blk[oa + i];
+          for (size_t j = j0; j < el_count[b]; j++) {
+            unsigned bj = blk[ob + j];
+            unsigned lo = bi < bj ? bi : bj, hi = bi < bj ? bj : bi;
+            unsigned part = nblocks * lo - (lo * (lo + 1)) / 2 + hi;
+            double dx = x[oa + i] - x[ob + j], dy = y[oa + i] - y[ob + j],
+                   dz = z[oa + i] - z[ob + j];
+            double r = sqrt(dx * dx + dy * dy + dz * dz);
+            double *dst = my + (size_t)part * nq;
+            for (unsigned iq = 0; iq < nq; iq++) {
+              double qr = r * q[iq];
+              dst[iq] += sin(qr) / (qr + 0.00000001);
+            }
+          }
+        }
+      }
+      for (int t = 0; t < nt; t++)
+        for (size_t i = 0; i < psize; i++) S[(size_t)p * psize + i] += acc[(size_t)t * psize + i];
+    }
+  }
+  free(blk);
+  free(acc);
+}
+
+void or_hist_pairsum(const uint64_t *hist, unsigned ntypes, unsigned nhist,
+                     double bin, const double *q, unsigned nq, double *S,
+                     int nthreads) {
+  int nt = clamp_threads(nthreads);
+  double *acc = (double *)calloc((size_t)nt * nq, sizeof(double));
+  for (unsigned p = 0; p < ntypes; p++) {
+    const uint64_t *h = hist + (size_t)p * nhist;
+#pragma omp parallel num_threads(nt)
+    {
+      int t = thread_id();
+      double *my = acc + (size_t)t * nq;
+      memset(my, 0, nq * sizeof(double));
+      for (unsigned k = (unsigned)t; k < nhist; k += (unsigned)nt) {
+        if (!h[k]) continue;
+        double r = ((double)k + 0.5) * bin;
+        for (unsigned iq = 0; iq < nq; iq++) {
+          double qr = r * q[iq];
+          my[iq] += h[k] * sin(qr) / (qr + 0.00000001);
+        }
+      }
+    }
+    for (unsigned iq = 0; iq < nq; iq++) {
+      double s = 0;
+      for (int t = 0; t < nt; t++) s += acc[(size_t)t * nq + iq];
+      S[(size_t)p * nq + iq] = s;
+    }
+  }
+  free(acc);
+}
+
+void or_polar_1d(double *I, const double *q, unsigned nq, double lambda) {
+  for (unsigned iq = 0; iq < nq; iq++) {
+    double st = q[iq] * (lambda * 0.25 / XS_PI);
+    double c2t = 1. - 2. * (st * st);
+    I[iq] *= 0.5 * (1. + c2t * c2t);
+  }
+}
+
+static double weight(const double *ff, const double *sl, unsigned el, unsigned iq,
+                     unsigned nq) {
+  return ff ? ff[(size_t)el * nq + iq] : sl[el];
+}
+
+void or_intensity_1d(const double *S, unsigned n_el, const unsigned *el_count,
+                     const double *ff, const double *sl, const double *q,
+                     unsigned nq, int polar, double lambda, double *I) {
+  unsigned ntot = 0;
+  for (unsigned e = 0; e < n_el; e++) ntot += el_count[e];
+  for (unsigned iq = 0; iq < nq; iq++) I[iq] = 0;
+  unsigned p = 0;
+  for (unsigned a = 0; a < n_el; a++)
+    for (unsigned b = a; b < n_el; b++, p++)
+      for (unsigned iq = 0; iq < nq; iq++)
+        I[iq] += S[(size_t)p * nq + iq] * weight(ff, sl, a, iq, nq) *
+                 weight(ff, sl, b, iq, nq) * 2.;
+  for (unsigned e = 0; e < n_el; e++)
+    for (unsigned iq = 0; iq < nq; iq++) {
+      double w = weight(ff, sl, e, iq, nq);
+      I[iq] += el_count[e] * (w * w);
+    }
+  for (unsigned iq = 0; iq < nq; iq++) I[iq] /= ntot;
+  if (polar) or_polar_1d(I, q, nq, lambda);
+}
+
+void or_partial_intensity_1d(const double *S, unsigned n_el,
+                             const unsigned *blk_count, unsigned nblocks,
+                             const double *ff, const double *sl,
+                             const double *q, unsigned nq, int polar,
+                             double lambda, unsigned ntot, double *I) {
+  const unsigned nparts = nblocks * (nblocks + 1) / 2;
+  const size_t psize = (size_t)nparts * nq;
+  for (size_t i = 0; i < psize + nq; i++) I[i] = 0;
+  unsigned p = 0;
+  for (unsigned a = 0; a < n_el; a++)
+    for (unsigned b = a; b < n_el; b++, p++)
+      for (unsigned part = 0; part < nparts; part++)
+        for (unsigned iq = 0; iq < nq; iq++)
+          I[(size_t)(part + 1) * nq + iq] +=
+              S[((size_t)p * nparts + part) * nq + iq] * weight(ff, sl, a, iq, nq) *
+              weight(ff, sl, b, iq, nq) * 2.;
+  for (unsigned e = 0; e < n_el; e++)
+    for (unsigned bk = 0; bk < nblocks; bk++) {
+      size_t start = (size_t)nq * (1 + (nblocks * bk - (bk * (bk + 1)) / 2 + bk));
+      for (unsigned iq = 0; iq < nq; iq++) {
+        double w = weight(ff, sl, e, iq, nq);
+        I[start + iq] += blk_count[e * nblocks + bk] * (w * w);
+      }
+    }
+  for (size_t i = nq; i < psize + nq; i++) I[i] /= ntot;
+  if (polar)
+    for (unsigned part = 1; part < nparts + 1; part++) or_polar_1d(I + (size_t)part * nq, q, nq, lambda);
+  for (unsigned part = 1; part < nparts + 1; part++)
+    for (unsigned iq = 0; iq < nq; iq++) I[iq] += I[(size_t)part * nq + iq];
+}
+
+void or_pdf(const uint64_t *hist, unsigned n_el, const unsigned *el_count,
+            unsigned nhist, double bin, int type, double rho0, const double *sl,
+            int cpu_radius_offset, double *pdf) {
+  unsigned ntot = 0;
+  for (unsigned e = 0; e < n_el; e++) ntot += el_count[e];
+  const unsigned ntypes = n_el * (n_el + 1) / 2;
+  for (size_t i = 0; i < (size_t)(1 + ntypes) * nhist; i++) pdf[i] = 0;
+  double fav2 = 0;
+  for (unsigned e = 0; e < n_el; e++) fav2 += sl[e] * el_count[e];
+  fav2 /= ntot;
+  fav2 *= fav2;
+  size_t start = 0;
+  for (unsigned a = 0; a < n_el; a++) {
+    for (unsigned b = a; b < n_el; b++, start += nhist) {
+      double wab = sl[a] * sl[b] / fav2;
+      double *part = pdf + nhist;
+      if (type == 0) {
+        double mult = 2. / (bin * ntot);
+        for (size_t i = start; i < start + nhist; i++) part[i] = hist[i] * mult;
+      } else if (type == 1) {
+        double mult = 0.5 / (XS_PI * bin * rho0 * ntot);
+        for (size_t i = start; i < start + nhist; i++) {
+          double r = ((cpu_radius_offset ? i : i - start) + 0.5) * bin;
+          part[i] = hist[i] * mult / (r * r);
+        }
+      } else {
+        double mult = 2. / (bin * ntot);
+        double sub = (b > a) ? 8. * XS_PI * rho0 * (double)el_count[a] *
+                                   (double)el_count[b] / ((double)ntot * (double)ntot)
+                             : 4. * XS_PI * rho0 * ((double)el_count[a] * (double)el_count[a]) /
+                                   ((double)ntot * (double)ntot);
+        for (size_t i = start; i < start + nhist; i++) {
+          double r = ((cpu_radius_offset ? i : i - start) + 0.5) * bin;
+          part[i] = hist[i] * mult / r - sub * r;
+        }
+      }
+      for (unsigned i = 0; i < nhist; i++) pdf[i] += part[start + i] * wab;
+    }
+  }
+}
+
+/* Elementary right-handed rotations about axis 0 (x), 1 (y), 2 (z). */
+static void elementary(int axis, double ang, double M[9]) {
+  double c = cos(ang), s = sin(ang);
+  for (int i = 0; i < 9; i++) M[i] = 0;
+  int u = (axis + 1) % 3, v = (axis + 2) % 3;
+  M[axis * 3 + axis] = 1;
+  M[u * 3 + u] = c;
+  M[u * 3 + v] = -s;
+  M[v * 3 + u] = s;
+  M[v * 3 + v] = c;
+}
+
+static void matmul3(const double A[9], const double B[9], double C[9]) {
+  double T[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      T[i * 3 + j] = A[i * 3 + 0] * B[0 * 3 + j] + A[i * 3 + 1] * B[1 * 3 + j] +
+                     A[i * 3 + 2] * B[2 * 3 + j];
+  memcpy(C, T, sizeof(T));
+}
+
+void or_rot_matrix(const double euler[3], int convention, double R[9]) {
+  /* axis sequences of the 12 conventions, numbered as typedefs.h:59-70 */
+  static const int seq[12][3] = {{0, 2, 0}, {0, 1, 0}, {1, 0, 1}, {1, 2, 1},
+                                 {2, 1, 2}, {2, 0, 2}, {0, 2, 1}, {0, 1, 2},
+                                 {1, 0, 2}, {1, 2, 0}, {2, 1, 0}, {2, 0, 1}};
+  double A[9], B[9], C[9];
+  elementary(seq[convention][0], euler[0], A);
+  elementary(seq[convention][1], euler[1], B);
+  elementary(seq[convention][2], euler[2], C);
+  matmul3(A, B, R);
+  matmul3(R, C, R);
+}
+
+void or_amp2d(const double *xyz, const unsigned *el_count, unsigned n_el,
+              const double *qvec, size_t nqp, const double *rot, unsigned nrot,
+              const double *ff, double *I, int nthreads) {
+  size_t ntot = 0;
+  for (unsigned e = 0; e < n_el; e++) ntot += el_count[e];
+  const double *x = xyz, *y = xyz + ntot, *z = xyz + 2 * ntot;
+  int nt = clamp_threads(nthreads);
+  for (size_t k = 0; k < nqp; k++) I[k] = 0;
+  for (unsigned r = 0; r < nrot; r++) {
+    const double *R = rot + (size_t)r * 9;
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (long long k = 0; k < (long long)nqp; k++) {
+      const double *qv = qvec + (size_t)k * 3;
+      /* q' = R^T q (CalcFunctions.cpp:1020-1022, 1033) */
+      double qx = qv[0] * R[0] + qv[1] * R[3] + qv[2] * R[6];
+      double qy = qv[0] * R[1] + qv[1] * R[4] + qv[2] * R[7];
+      double qz = qv[0] * R[2] + qv[1] * R[5] + qv[2] * R[8];
+      double Ar = 0, Ai = 0;
+      size_t o = 0;
+      for (unsigned e = 0; e < n_el; o += el_count[e], e++) {
+        double ar = 0, ai = 0;
+        for (size_t j = o; j < o + el_count[e]; j++) {
+          double ph = qx * x[j] + qy * y[j] + qz * z[j];
+          ar += cos(ph);
+          ai += sin(ph);
+        }
+        double w = ff[(size_t)e * nqp + k];
+        Ar += w * ar;
+        Ai += w * ai;
+      }
+      I[k] += Ar * Ar + Ai * Ai;
+    }
+  }
+}
+
+void or_calc_int2d(const double *xyz, const unsigned *el_count, unsigned n_el,
+                   const double *q, unsigned nq, unsigned nfi,
+                   const double euler_min[3], const double euler_max[3],
+                   const unsigned euler_n[3], int convention, double lambda,
+                   int polar, const double *ff, const double *sl, double *I2D,
+                   double *I1D, int nthreads) {
+  unsigned ntot = 0;
+  for (unsigned e = 0; e < n_el; e++) ntot += el_count[e];
+  const size_t nqp = (size_t)nq * nfi;
+  double *qvec = (double *)malloc(nqp * 3 * sizeof(double));
+  double *w = (double *)malloc((size_t)n_el * nqp * sizeof(double));
+  const double dfi = 2. * XS_PI / nfi;
+  for (unsigned iq = 0; iq < nq; iq++) {
+    double st = q[iq] * (lambda * 0.25 / XS_PI);
+    double ct = 1. - st * st; /* CalcFunctions.cpp:1001 */
+    for (unsigned ifi = 0; ifi < nfi; ifi++) {
+      size_t k = (size_t)iq * nfi + ifi;
+      qvec[3 * k + 0] = ct * cos(ifi * dfi) * q[iq];
+      qvec[3 * k + 1] = ct * sin(ifi * dfi) * q[iq];
+      qvec[3 * k + 2] = -st * q[iq];
+      for (unsigned e = 0; e < n_el; e++)
+        w[(size_t)e * nqp + k] = ff ? ff[(size_t)e * nq + iq] : sl[e];
+    }
+  }
+  double step[3];
+  for (int a = 0; a < 3; a++)
+    step[a] = euler_n[a] < 2 ? 0 : (euler_max[a] - euler_min[a]) / euler_n[a];
+  unsigned nrot = euler_n[0] * euler_n[1] * euler_n[2];
+  double *rot = (double *)malloc((size_t)(nrot ? nrot : 1) * 9 * sizeof(double));
+  unsigned r = 0;
+  for (unsigned ia = 0; ia < euler_n[0]; ia++)
+    for (unsigned ib = 0; ib < euler_n[1]; ib++)
+      for (unsigned ig = 0; ig < euler_n[2]; ig++, r++) {
+        double e3[3] = {euler_min[0] + (ia + 0.5) * step[0],
+                        euler_min[1] + (ib + 0.5) * step[1],
+                        euler_min[2] + (ig + 0.5) * step[2]};
+        or_rot_matrix(e3, convention, rot + (size_t)r * 9);
+      }
+  or_amp2d(xyz, el_count, n_el, qvec, nqp, rot, nrot, w, I2D, nthreads);
+  double norm = 1. / (ntot * euler_n[0] * euler_n[1] * euler_n[2]);
+  for (size_t k = 0; k < nqp; k++) I2D[k] *= norm;
+  if (polar) {
+    for (unsigned iq = 0; iq < nq; iq++) {
+      double st = q[iq] * (lambda * 0.25 / XS_PI);
+      double c2t = 1. - 2. * (st * st);
+      double f = 0.5 * (1. + c2t * c2t);
+      for (unsigned ifi = 0; ifi < nfi; ifi++) I2D[(size_t)iq * nfi + ifi] *= f;
+    }
+  }
+  for (unsigned iq = 0; iq < nq; iq++) {
+    double s = 0;
+    for (unsigned ifi = 0; ifi < nfi; ifi++) s += I2D[(size_t)iq * nfi + ifi];
+    I1D[iq] = s / nfi;
+  }
+  free(rot);
+  free(w);
+  free(qvec);
+}

@@ -1,0 +1,178 @@
+"""Seeded synthetic atomic ensembles and form factors.
+
+These stand in for the reference's ensemble builder (CalcFunctions.cpp:251-450,
+block::calcAtoms), whose random generator is clock-seeded
+(CalcFunctions.cpp:259) and so cannot reproduce a run.  Every builder here
+draws from numpy's PCG64 with an explicit seed, stores positions as float32
+(the precision the GPU kernels consume) and returns atoms grouped
+element-major, the order block::sortAtoms (CalcFunctions.cpp:159-196)
+produces.
+
+A structure is returned as ``Ensemble(xyz, el_count, names)`` with ``xyz`` a
+(3, N) float32 array (rows x, y, z).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Sequence
+
+import numpy as np
+
+FOUR_PI = 4.0 * np.pi
+
+
+@dataclass
+class Ensemble:
+    xyz: np.ndarray                      # (3, N) float32, element-major
+    el_count: List[int]                  # atoms per element
+    names: List[str] = field(default_factory=list)
+
+    @property
+    def n(self) -> int:
+        return int(self.xyz.shape[1])
+
+    @property
+    def n_el(self) -> int:
+        return len(self.el_count)
+
+    def xyz64(self) -> np.ndarray:
+        """Positions promoted exactly to float64, the oracle's input."""
+        return np.ascontiguousarray(self.xyz, dtype=np.float64)
+
+    def element(self, e: int) -> np.ndarray:
+        off = int(sum(self.el_count[:e]))
+        return self.xyz[:, off:off + self.el_count[e]]
+
+
+def _concat(parts: Sequence[np.ndarray], names: Sequence[str]) -> Ensemble:
+    counts = [int(p.shape[0]) for p in parts]
+    xyz = np.concatenate([p.reshape(-1, 3) for p in parts], axis=0) if parts else np.zeros((0, 3))
+    return Ensemble(np.ascontiguousarray(xyz.T.astype(np.float32)), counts, list(names))
+
+
+def q_mesh(qmin: float, qmax: float, n: int) -> np.ndarray:
+    """q_k = qmin + k*(qmax-qmin)/n, k < n: the endpoint-exclusive mesh of
+    ReadXML.cpp:220-222."""
+    dq = (qmax - qmin) / n
+    return np.array([qmin + k * dq for k in range(n)], dtype=np.float64)
+
+
+def fcc_sites(a: float, nmax: int) -> np.ndarray:
+    """fcc lattice sites with cell indices in [-nmax, nmax]^3."""
+    r = np.arange(-nmax, nmax + 1)
+    cx, cy, cz = np.meshgrid(r, r, r, indexing="ij")
+    cells = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], axis=1).astype(np.float64)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    return ((cells[:, None, :] + basis[None, :, :]).reshape(-1, 3)) * a
+
+
+def bcc_sites(a: float, nmax: int) -> np.ndarray:
+    r = np.arange(-nmax, nmax + 1)
+    cx, cy, cz = np.meshgrid(r, r, r, indexing="ij")
+    cells = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], axis=1).astype(np.float64)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0.5]])
+    return ((cells[:, None, :] + basis[None, :, :]).reshape(-1, 3)) * a
+
+
+def fcc_sphere(a: float = 4.078, radius: float = 20.0, sigma: float = 0.05,
+               seed: int = 1, name: str = "Au") -> Ensemble:
+    """Single-element fcc lattice cut to a sphere centred on a lattice site,
+    Gaussian displacement sigma per coordinate (workload: 2 nm-radius fcc
+    sphere, about 2,000 atoms)."""
+    nmax = int(np.ceil(radius / a)) + 1
+    s = fcc_sites(a, nmax)
+    s = s[np.einsum("ij,ij->i", s, s) <= radius * radius]
+    rng = np.random.Generator(np.random.PCG64(seed))
+    s = s + rng.normal(0.0, sigma, size=s.shape)
+    return _concat([s], [name])
+
+
+def rocksalt_cube(a: float = 5.64, ncell: int = 50, vacancy: float = 0.02,
+                  sigma: float = 0.08, seed_vac: int = 3, seed_jit: int = 4,
+                  names=("Na", "Cl")) -> Ensemble:
+    """Two-element rock-salt cube of ncell^3 conventional cells (8*ncell^3
+    sites), Bernoulli vacancies and Gaussian jitter (workload: 1,000,000-site
+    rock-salt cube, a=5.64 A, 2% vacancies, sigma=0.08 A)."""
+    r = np.arange(ncell, dtype=np.float64)
+    cx, cy, cz = np.meshgrid(r, r, r, indexing="ij")
+    cells = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], axis=1)
+    fcc = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    shift = np.array([0.5, 0.0, 0.0])
+    na = ((cells[:, None, :] + fcc[None]).reshape(-1, 3)) * a
+    cl = ((cells[:, None, :] + (fcc + shift)[None]).reshape(-1, 3)) * a
+    rv = np.random.Generator(np.random.PCG64(seed_vac))
+    keep_na = rv.random(na.shape[0]) >= vacancy
+    keep_cl = rv.random(cl.shape[0]) >= vacancy
+    na, cl = na[keep_na], cl[keep_cl]
+    rj = np.random.Generator(np.random.PCG64(seed_jit))
+    na = na + rj.normal(0.0, sigma, size=na.shape)
+    cl = cl + rj.normal(0.0, sigma, size=cl.shape)
+    return _concat([na, cl], list(names))
+
+
+def disordered_cubic(a: float = 2.5, n: int = 37, jitter: float = 0.6,
+                     fractions=(0.6, 0.3, 0.1), natoms: int | None = 50000,
+                     seed_jit: int = 6, seed_species: int = 7,
+                     names=("Fe", "Ni", "Cr")) -> Ensemble:
+    """Simple-cubic lattice with uniform jitter in [-j, j] per coordinate and
+    species drawn with the given fractions, centred on the origin (workload:
+    50,000-atom disordered solid, 3 elements 60/30/10%)."""
+    r = np.arange(n, dtype=np.float64)
+    cx, cy, cz = np.meshgrid(r, r, r, indexing="ij")
+    s = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], axis=1) * a
+    if natoms is not None:
+        s = s[:natoms]
+    rj = np.random.Generator(np.random.PCG64(seed_jit))
+    s = s + rj.uniform(-jitter, jitter, size=s.shape)
+    s = s - s.mean(axis=0)
+    rs = np.random.Generator(np.random.PCG64(seed_species))
+    u = rs.random(s.shape[0])
+    edges = np.cumsum(fractions)
+    species = np.searchsorted(edges, u, side="right")
+    species = np.minimum(species, len(fractions) - 1)
+    parts = [s[species == e] for e in range(len(fractions))]
+    return _concat(parts, list(names[:len(fractions)]))
+
+
+def gaussian_ff_coeffs(n_el: int, seed: int = 2) -> np.ndarray:
+    """(n_el, 9) rows a1..a4, b1..b4, c with a_k~U[0.5,20], b_k~U[0.1,100] A^2,
+    c~U[0,2] (a 4-Gaussian x-ray form factor)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = np.empty((n_el, 9))
+    for e in range(n_el):
+        out[e, 0:4] = rng.uniform(0.5, 20.0, 4)
+        out[e, 4:8] = rng.uniform(0.1, 100.0, 4)
+        out[e, 8] = rng.uniform(0.0, 2.0)
+    return out
+
+
+def gaussian_ff(q: np.ndarray, coeffs: np.ndarray) -> np.ndarray:
+    """f_el(q) = sum_k a_k exp(-b_k (q/4pi)^2) + c, evaluated in float64.
+    Returns (n_el, len(q))."""
+    q = np.asarray(q, dtype=np.float64)
+    s2 = (q / FOUR_PI) ** 2
+    coeffs = np.atleast_2d(coeffs)
+    out = np.empty((coeffs.shape[0], q.size))
+    for e, c in enumerate(coeffs):
+        out[e] = sum(c[k] * np.exp(-c[4 + k] * s2) for k in range(4)) + c[8]
+    return out
+
+
+def random_cluster(n_per_el: Sequence[int], box: float, min_dist: float,
+                   seed: int, names: Sequence[str] | None = None) -> Ensemble:
+    """Random atoms in a cube with a minimum separation (rejection sampling),
+    for small irregular test cases."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    total = int(sum(n_per_el))
+    pts: list = []
+    while len(pts) < total:
+        p = rng.uniform(-box / 2, box / 2, 3)
+        if all(np.sum((p - o) ** 2) >= min_dist ** 2 for o in pts):
+            pts.append(p)
+    pts = np.array(pts)
+    parts, o = [], 0
+    for c in n_per_el:
+        parts.append(pts[o:o + c])
+        o += c
+    names = list(names) if names else [f"E{e}" for e in range(len(n_per_el))]
+    return _concat(parts, names)
