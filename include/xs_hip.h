@@ -1,0 +1,157 @@
+/*
+ * xs_hip.h -- C ABI of the MI355X (gfx950) backend for the XaNSoNS
+ * scattering calculators.
+ *
+ * The reference picks its calculator backend at link time: main.cpp declares
+ * one set of prototypes (CPU main.cpp:65-68, CUDA main.cpp:46-52, OpenCL
+ * main.cpp:54-63) and calls one of them per scenario (main.cpp:151-233).
+ * This library is what a HIP build links instead.  The C++ drop-in
+ * (softx-d-16-00083_amd/dropin/CalcFunctionsHIP.cpp) keeps the CPU
+ * signatures of main.cpp:65-68 and calls the functions below; the Python
+ * harness binds the same symbols with ctypes.
+ *
+ * Conventions
+ *  - Every call returns XS_OK (0) or a negative XS_ERR_* code; the message
+ *    of the last failure on the calling thread is xs_last_error().
+ *  - Coordinates are structure-of-arrays: xyz[0..n) x, [n..2n) y, [2n..3n) z,
+ *    atoms grouped element-major (el_count[0] atoms of element 0 first, ...).
+ *  - el_count, and every scalar, is host memory.  The array arguments are
+ *    host memory (the call copies, computes and returns synchronously) or,
+ *    with XS_DEVICE_PTRS, device memory on the context's device (the call
+ *    enqueues on the context stream and returns without waiting).
+ *  - Pair types p = (a <= b) are ordered (0,0),(0,1),...,(0,n-1),(1,1),...
+ *    (CalcFunctions.cpp:777-778); ntypes = n_el*(n_el+1)/2.
+ *  - One context per host thread; calls on one context are serialised on
+ *    its stream.
+ */
+#ifndef XS_HIP_H
+#define XS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XS_OK 0
+#define XS_ERR_ARG (-1)     /* invalid argument */
+#define XS_ERR_HIP (-2)     /* a HIP runtime call failed */
+#define XS_ERR_NOMEM (-3)   /* device or pinned allocation failed */
+#define XS_ERR_DEVICE (-4)  /* no usable device / bad device id */
+#define XS_ERR_RANGE (-5)   /* a size exceeds what the kernels support */
+
+/* flags */
+#define XS_DEVICE_PTRS 0x1u     /* array arguments are device pointers */
+#define XS_F64_BINNING 0x2u     /* xs_hist: xyz is double, bin as the CPU path (double) */
+#define XS_PDF_CPU_RADIUS 0x4u  /* xs_pdf: radius offset of the CPU path for p >= 1 */
+
+/* PDF types, typedefs.h:54-56 */
+#define XS_PDF_RDF 0
+#define XS_PDF_PDF 1
+#define XS_PDF_RPDF 2
+
+typedef struct xs_ctx xs_ctx;
+
+/* Open a context on a device and create its stream.
+ * Replaces SetDeviceCuda (CalcFunctionsCUDA.cu:1485-1519), which falls back
+ * to the fastest device on a bad id; here a bad id is XS_ERR_DEVICE. */
+int xs_init(int device, xs_ctx **ctx);
+
+/* Release the context, its stream and workspace.
+ * Replaces delDataFromDevice (CalcFunctionsCUDA.cu:1557-1566). */
+int xs_free(xs_ctx *ctx);
+
+/* Use an existing HIP stream (hipStream_t) for all later calls; NULL
+ * restores the context's own stream.  No counterpart in the reference
+ * (its calls are synchronous on the null stream). */
+int xs_set_stream(xs_ctx *ctx, void *stream);
+
+/* Wait for all work enqueued on the context stream. */
+int xs_synchronize(xs_ctx *ctx);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *xs_last_error(void);
+
+/* Number of visible HIP devices. */
+int xs_device_count(int *count);
+
+/* Per-kernel timing with HIP events on the context stream.  enable=1 starts
+ * (and resets) collection; xs_profile_read waits for the recorded events and
+ * returns the summed duration and launch count of one kernel family:
+ * "hist", "hist_prep", "hist_pairsum", "pdf", "debye", "amp2d".
+ * Replaces the commented-out per-kernel cudaEvent timers
+ * (CalcFunctionsCUDA.cu:716-731, 899-909, 1016-1032). */
+int xs_profile(xs_ctx *ctx, int enable);
+int xs_profile_read(xs_ctx *ctx, const char *kernel, double *total_ms,
+                    unsigned long long *count);
+
+/* H1: per-pair-type histogram of interatomic distances (uint64 counts).
+ * Default: xyz is float and the bin index is (unsigned)(sqrtf((dx*dx+dy*dy)
+ * +dz*dz)/bin) in IEEE single precision, the expression of calcHistKernel
+ * (CalcFunctionsCUDA.cu:832-833) without fast-math.  With XS_F64_BINNING xyz
+ * is double and the index is computed in double exactly as calcHist does
+ * (CalcFunctions.cpp:784-788), so the counts equal the reference CPU path's.
+ * Same-element pairs are j > i, cross-element pairs all (i, j)
+ * (CalcFunctions.cpp:782).  hist (ntypes*nhist) is overwritten; pairs whose
+ * index is >= nhist go to *overflow (may be NULL; with XS_DEVICE_PTRS a
+ * device pointer).  Replaces calcHist (CalcFunctions.cpp:757-809) and
+ * calcHistCuda (CalcFunctionsCUDA.cu:855-920). */
+int xs_hist(xs_ctx *ctx, const void *xyz, size_t n_atoms,
+            const unsigned *el_count, unsigned n_el, double bin, unsigned nhist,
+            uint64_t *hist, uint64_t *overflow, unsigned flags);
+
+/* H3: histogram pair sums S[p*nq+iq] = sum_k h_p[k] sin(q r_k)/(q r_k + 1e-8),
+ * r_k = (k+1/2) bin, empty bins skipped, S = 0 where q = 0.  FP64 terms,
+ * sin advanced from bin to bin by a rotation, FP64 atomics across blocks.
+ * Replaces the bin loop of calcInt1DHist (CalcFunctions.cpp:844-871) and
+ * calcIntHistKernelXray/Neutron (CalcFunctionsCUDA.cu:923-985). */
+int xs_hist_pairsum(xs_ctx *ctx, const uint64_t *hist, unsigned ntypes,
+                    unsigned nhist, double bin, const double *q, unsigned nq,
+                    double *S, unsigned flags);
+
+/* H4: RDF / PDF / rPDF from the histogram in FP64, pdf = (1+ntypes)*nhist,
+ * total first.  Replaces CalcFunctions.cpp:919-957 and
+ * CalcFunctionsCUDA.cu:1092-1150.  XS_PDF_CPU_RADIUS reproduces the CPU
+ * loop's radius r = (p*nhist + k + 1/2) bin for pair types p >= 1
+ * (CalcFunctions.cpp:942, 950); without it r = (k + 1/2) bin as the CUDA
+ * and OpenCL kernels compute (CalcFunctionsCUDA.cu:1057, 1066). */
+int xs_pdf(xs_ctx *ctx, const uint64_t *hist, unsigned n_el,
+           const unsigned *el_count, unsigned nhist, double bin, int pdf_type,
+           double rho0, const double *sl, double *pdf, unsigned flags);
+
+/* H5: direct Debye pair sums S[p*nq+iq] = sum_{pairs in p} sin(q r)/(q r + 1e-8)
+ * with FP64 pair terms (xyz double).  On a uniform q mesh (the reference's,
+ * ReadXML.cpp:220-222) sin(q r) advances from q to q by a rotation; other
+ * meshes evaluate the series per q.  Replaces the pair loop of calcIntDebye
+ * (CalcFunctions.cpp:561-591) and calcIntDebyeKernel (CalcFunctionsCUDA.cu:1163-1273). */
+int xs_debye_pairsum(xs_ctx *ctx, const double *xyz, size_t n_atoms,
+                     const unsigned *el_count, unsigned n_el, const double *q,
+                     unsigned nq, double *S, unsigned flags);
+
+/* H6: block-pair partial Debye sums.  blk_count[el*nblocks+b] atoms of
+ * element el belong to block b (block-major inside each element).
+ * S[(p*nparts + part)*nq + iq], part(iB<=jB) = nblocks*iB - iB*(iB+1)/2 + jB.
+ * Replaces the pair loop of calcIntPartialDebye (CalcFunctions.cpp:662-703). */
+int xs_partial_debye_pairsum(xs_ctx *ctx, const double *xyz, size_t n_atoms,
+                             const unsigned *el_count, unsigned n_el,
+                             const unsigned *blk_count, unsigned nblocks,
+                             const double *q, unsigned nq, double *S,
+                             unsigned flags);
+
+/* H7: orientation-summed squared amplitude for an explicit q-vector list,
+ * I[k] = sum_r | sum_el ff[el*nqp+k] sum_{j in el} exp(i (R_r^T q_k).x_j) |^2,
+ * rot[r*9..] = R_r row-major (all double).  FP64 phase q'.x reduced to one
+ * turn, FP32 hardware sin/cos of the reduced phase, FP64 amplitude sums.
+ * Replaces calcInt2DKernelXray/Neutron + Sum2DKernel
+ * (CalcFunctionsCUDA.cu:514-650) and the inner loops of calcInt2D
+ * (CalcFunctions.cpp:1026-1062). */
+int xs_amp2d(xs_ctx *ctx, const double *xyz, size_t n_atoms,
+             const unsigned *el_count, unsigned n_el, const double *qvec,
+             size_t nqp, const double *rot, unsigned nrot, const double *ff,
+             double *I, unsigned flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XS_HIP_H */

@@ -1,0 +1,277 @@
+// H5/H6: direct Debye pair sums on gfx950.
+//
+// S_p(q) = sum_{pairs in p} sin(q r)/(q r + 1e-8)   (CalcFunctions.cpp:577-583)
+//
+// Work item = (I tile, J tile) of up to 32 x 32 atoms; tiles never cross an
+// element or a structural block, so each item feeds exactly one pair type and
+// one block pair (the partial sums of calcIntPartialDebye,
+// CalcFunctions.cpp:662-703; the plain sum is the one-block case).
+// A work group of 4 waves first computes the item's pair distances in FP64
+// into LDS (r/2pi, 1/r and, on a uniform q mesh, sin/cos of the phase step
+// dq r), then each lane owns 16 consecutive q and each wave a quarter of the
+// pairs: sin(q r) is seeded once per pair and lane from the series in
+// xs_sincos.h and advanced to the next q by the stored rotation (4 FMAs), so a
+// pair-q term costs about 7 FP64 instructions.  FP64 because the intensity at
+// low q is a strongly cancelling difference (see xs_iq.hip).  The 1e-8 of the
+// denominator is applied to first order through a second accumulator of
+// sin(q r)/r^2.  The four waves' partial sums are combined in LDS and added to
+// S with FP64 atomics.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "xs_internal.h"
+#include "xs_sincos.h"
+
+namespace xs {
+namespace debye {
+
+constexpr int kTT = 32;       // atoms per tile
+constexpr int kQPT = 16;      // q per lane
+constexpr int kWaves = 4;
+constexpr int kBlock = 64 * kWaves;
+constexpr int kQBlock = 64 * kQPT;  // q per work group
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+struct Tile {
+  uint32_t off, cnt, el, blk;
+};
+struct Item {
+  uint32_t ti, tj, p, part;
+};
+
+template <bool UNIFORM>
+__global__ __launch_bounds__(kBlock) void debye_kernel(const double *__restrict__ x,
+                                                       const double *__restrict__ y,
+                                                       const double *__restrict__ z,
+                                                       const Tile *__restrict__ tiles,
+                                                       const Item *__restrict__ items,
+                                                       const double *__restrict__ q, unsigned nq,
+                                                       double dq, unsigned nparts,
+                                                       double *__restrict__ S) {
+  __shared__ double4 pr[kTT * kTT];
+  const Item it = items[blockIdx.x];
+  const Tile A = tiles[it.ti], B = tiles[it.tj];
+  const bool diag = it.ti == it.tj;
+  for (unsigned m = threadIdx.x; m < kTT * kTT; m += kBlock) {
+    const unsigned il = m / kTT, jl = m % kTT;
+    double4 v = make_double4(0.0, 0.0, 1.0, 1.0);
+    if (il < A.cnt && jl < B.cnt && (!diag || jl > il)) {
+      const size_t i = A.off + il, j = B.off + jl;
+      const double dx = x[i] - x[j], dy = y[i] - y[j], dz = z[i] - z[j];
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      if (r > 0.0) {
+        v.x = r / kTwoPi;
+        v.y = 1.0 / r;
+        if (UNIFORM) {
+          double s, c;
+          sincos_2pi(dq * v.x, &s, &c);
+          v.z = s;
+          v.w = c;
+        }
+      }
+    }
+    pr[m] = v;
+  }
+  __syncthreads();
+
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned iq0 = blockIdx.y * kQBlock + lane * kQPT;
+  double acc[kQPT], corr[kQPT];
+#pragma unroll
+  for (int m = 0; m < kQPT; ++m) acc[m] = corr[m] = 0.0;
+  const double q0 = iq0 < nq ? q[iq0] : 0.0;
+  const unsigned pstart = wave * (kTT * kTT / kWaves), pend = pstart + kTT * kTT / kWaves;
+  for (unsigned m = pstart; m < pend; ++m) {
+    const double4 v = pr[m];
+    if (v.y == 0.0) continue;
+    const double ir = v.y, ir2 = v.y * v.y;
+    if (UNIFORM) {
+      double s, c;
+      sincos_2pi(q0 * v.x, &s, &c);
+#pragma unroll
+      for (int k = 0; k < kQPT; ++k) {
+        acc[k] = fma(s, ir, acc[k]);
+        corr[k] = fma(s, ir2, corr[k]);
+        const double sn = fma(s, v.w, c * v.z);
+        c = fma(c, v.w, -(s * v.z));
+        s = sn;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kQPT; ++k) {
+        const double qk = iq0 + k < nq ? q[iq0 + k] : 0.0;
+        double s, c;
+        sincos_2pi(qk * v.x, &s, &c);
+        acc[k] = fma(s, ir, acc[k]);
+        corr[k] = fma(s, ir2, corr[k]);
+      }
+    }
+  }
+  // sum over the waves: waves 1..3 park sin/(q r) - 1e-8 sin/(q r)^2 per q
+  double res[kQPT];
+#pragma unroll
+  for (int k = 0; k < kQPT; ++k) {
+    const double qk = iq0 + k < nq ? q[iq0 + k] : 0.0;
+    res[k] = qk == 0.0 ? 0.0 : (acc[k] - (1e-8 / qk) * corr[k]) / qk;
+  }
+  __syncthreads();
+  double *park = reinterpret_cast<double *>(pr);
+  if (wave > 0) {
+#pragma unroll
+    for (int k = 0; k < kQPT; ++k) park[((wave - 1) * kQPT + k) * 64 + lane] = res[k];
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int k = 0; k < kQPT; ++k) {
+      double t = res[k];
+      for (int w = 0; w < kWaves - 1; ++w) t += park[(w * kQPT + k) * 64 + lane];
+      if (iq0 + k < nq && t != 0.0)
+        atomicAdd(&S[((size_t)it.p * nparts + it.part) * nq + iq0 + k], t);
+    }
+  }
+}
+
+static bool uniform_mesh(const std::vector<double> &q, double *dq) {
+  const size_t n = q.size();
+  *dq = n > 1 ? (q[n - 1] - q[0]) / (double)(n - 1) : 0.0;
+  double scale = 0.0;
+  for (double v : q) scale = std::max(scale, std::fabs(v));
+  for (size_t k = 0; k < n; ++k)
+    if (std::fabs(q[k] - (q[0] + (double)k * *dq)) > 1e-13 * scale) return false;
+  return true;
+}
+
+static int run(xs_ctx *ctx, const double *xyz_in, size_t n, const unsigned *el_count, unsigned n_el,
+               const unsigned *blk_count, unsigned nblocks, const double *q_in, unsigned nq,
+               double *S_out, bool device) {
+  const unsigned ntypes = ntypes_of(n_el);
+  const unsigned nparts = nblocks * (nblocks + 1) / 2;
+  const size_t ssize = (size_t)ntypes * nparts * nq;
+  // tiles that stay inside one (element, block) segment
+  std::vector<Tile> tiles;
+  std::vector<uint32_t> el_first(n_el + 1);
+  uint32_t off = 0;
+  for (unsigned e = 0; e < n_el; ++e) {
+    el_first[e] = (uint32_t)tiles.size();
+    for (unsigned b = 0; b < nblocks; ++b) {
+      const uint32_t cnt = blk_count ? blk_count[e * nblocks + b] : el_count[e];
+      for (uint32_t t = 0; t < cnt; t += kTT)
+        tiles.push_back(Tile{off + t, std::min<uint32_t>(kTT, cnt - t), e, b});
+      off += cnt;
+    }
+  }
+  el_first[n_el] = (uint32_t)tiles.size();
+  std::vector<Item> items;
+  unsigned p = 0;
+  for (unsigned a = 0; a < n_el; ++a)
+    for (unsigned b = a; b < n_el; ++b, ++p)
+      for (uint32_t ti = el_first[a]; ti < el_first[a + 1]; ++ti)
+        for (uint32_t tj = (a == b ? ti : el_first[b]); tj < el_first[b + 1]; ++tj) {
+          const uint32_t bi = tiles[ti].blk, bj = tiles[tj].blk;
+          const uint32_t lo = std::min(bi, bj), hi = std::max(bi, bj);
+          items.push_back(Item{ti, tj, p, nblocks * lo - (lo * (lo + 1)) / 2 + hi});
+        }
+  std::vector<double> qh(nq);
+  if (device) {
+    XS_HIP_CHECK(hipMemcpyAsync(qh.data(), q_in, (size_t)nq * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  } else {
+    std::memcpy(qh.data(), q_in, (size_t)nq * 8);
+  }
+  double dq = 0.0;
+  const bool uniform = uniform_mesh(qh, &dq);
+
+  const size_t tb = tiles.size() * sizeof(Tile), ib = items.size() * sizeof(Item);
+  XS_TRY(ensure_workspace(ctx, Carver::need({device ? 0 : 3 * n * 8, device ? 0 : (size_t)nq * 8,
+                                             device ? 0 : ssize * 8, tb, ib})));
+  Carver cv(ctx->ws);
+  const double *xyz = xyz_in, *qd = q_in;
+  double *Sd = S_out;
+  if (!device) {
+    double *x2 = cv.take<double>(3 * n);
+    double *q2 = cv.take<double>(nq);
+    Sd = cv.take<double>(ssize);
+    XS_HIP_CHECK(hipMemcpyAsync(x2, xyz_in, 3 * n * 8, hipMemcpyHostToDevice, ctx->stream));
+    XS_HIP_CHECK(hipMemcpyAsync(q2, q_in, (size_t)nq * 8, hipMemcpyHostToDevice, ctx->stream));
+    xyz = x2;
+    qd = q2;
+  }
+  Tile *td = cv.take<Tile>(tiles.size());
+  Item *idv = cv.take<Item>(items.size());
+  XS_TRY(ensure_pinned(ctx, tb + ib + 256));
+  // the pinned buffer may still feed an earlier upload: wait for the stream
+  XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  std::memcpy(ctx->pin, tiles.data(), tb);
+  std::memcpy(static_cast<char *>(ctx->pin) + align_up(tb, 256), items.data(), ib);
+  if (tb) XS_HIP_CHECK(hipMemcpyAsync(td, ctx->pin, tb, hipMemcpyHostToDevice, ctx->stream));
+  if (ib)
+    XS_HIP_CHECK(hipMemcpyAsync(idv, static_cast<char *>(ctx->pin) + align_up(tb, 256), ib,
+                                hipMemcpyHostToDevice, ctx->stream));
+  XS_HIP_CHECK(hipMemsetAsync(Sd, 0, ssize * 8, ctx->stream));
+  if (!items.empty()) {
+    hipEvent_t ev;
+    XS_TRY(prof_begin(ctx, "debye", &ev));
+    dim3 grid((unsigned)items.size(), (nq + kQBlock - 1) / kQBlock);
+    if (uniform)
+      hipLaunchKernelGGL(debye_kernel<true>, grid, dim3(kBlock), 0, ctx->stream, xyz, xyz + n,
+                         xyz + 2 * n, td, idv, qd, nq, dq, nparts, Sd);
+    else
+      hipLaunchKernelGGL(debye_kernel<false>, grid, dim3(kBlock), 0, ctx->stream, xyz, xyz + n,
+                         xyz + 2 * n, td, idv, qd, nq, dq, nparts, Sd);
+    XS_HIP_CHECK(hipGetLastError());
+    XS_TRY(prof_end(ctx, "debye", ev));
+  }
+  if (!device) {
+    XS_HIP_CHECK(hipMemcpyAsync(S_out, Sd, ssize * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  } else {
+    // keep the pinned staging alive until the uploads have been consumed
+    XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  }
+  return XS_OK;
+}
+
+static int check_args(const double *xyz, size_t n, const unsigned *el_count, unsigned n_el,
+                      const double *q, unsigned nq, double *S) {
+  if (!el_count || !q || !S || (!xyz && n)) return fail(XS_ERR_ARG, "debye: null pointer");
+  if (n_el == 0 || n_el > 16) return fail(XS_ERR_RANGE, "debye: n_el must be in [1, 16]");
+  size_t tot = 0;
+  for (unsigned e = 0; e < n_el; ++e) tot += el_count[e];
+  if (tot != n) return fail(XS_ERR_ARG, "debye: sum(el_count) != n_atoms");
+  if (nq == 0) return fail(XS_ERR_ARG, "debye: nq must be > 0");
+  return XS_OK;
+}
+
+}  // namespace debye
+}  // namespace xs
+
+using namespace xs;
+
+extern "C" int xs_debye_pairsum(xs_ctx *ctx, const double *xyz, size_t n_atoms,
+                                const unsigned *el_count, unsigned n_el, const double *q,
+                                unsigned nq, double *S, unsigned flags) {
+  XS_TRY(activate(ctx));
+  XS_TRY(debye::check_args(xyz, n_atoms, el_count, n_el, q, nq, S));
+  return debye::run(ctx, xyz, n_atoms, el_count, n_el, nullptr, 1, q, nq, S,
+                    flags & XS_DEVICE_PTRS);
+}
+
+extern "C" int xs_partial_debye_pairsum(xs_ctx *ctx, const double *xyz, size_t n_atoms,
+                                        const unsigned *el_count, unsigned n_el,
+                                        const unsigned *blk_count, unsigned nblocks,
+                                        const double *q, unsigned nq, double *S,
+                                        unsigned flags) {
+  XS_TRY(activate(ctx));
+  XS_TRY(debye::check_args(xyz, n_atoms, el_count, n_el, q, nq, S));
+  if (!blk_count || nblocks == 0) return fail(XS_ERR_ARG, "partial debye: no blocks");
+  for (unsigned e = 0; e < n_el; ++e) {
+    size_t t = 0;
+    for (unsigned b = 0; b < nblocks; ++b) t += blk_count[e * nblocks + b];
+    if (t != el_count[e]) return fail(XS_ERR_ARG, "partial debye: blk_count does not sum to el_count");
+  }
+  return debye::run(ctx, xyz, n_atoms, el_count, n_el, blk_count, nblocks, q, nq, S,
+                    flags & XS_DEVICE_PTRS);
+}

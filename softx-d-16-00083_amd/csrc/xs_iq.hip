@@ -1,0 +1,219 @@
+// H3 histogram -> pair sums and H4 RDF/PDF/rPDF on gfx950.
+//
+// H3: S_p(q) = sum_k h_p[k] sin(q r_k)/(q r_k), r_k = (k+1/2) bin.
+//   One thread per q, one work group per (256 q, 2048-bin chunk, pair type).
+//   The chunk's counts (as double, exact below 2^53) and 1/(k+1/2) are staged
+//   in LDS; every lane reads the same LDS word (broadcast) and empty bins are
+//   skipped uniformly.  The terms are FP64: at low q the intensity is a
+//   difference of the pair sum and the self term that cancels by up to four
+//   orders of magnitude, which FP32 terms (6e-8 relative each) cannot resolve
+//   to 1e-5.  FP64 vector instructions issue at the non-packed FP32 rate on
+//   gfx950, and the sine costs four FMAs per bin: sin/cos((k+1/2) q bin) is
+//   advanced from bin to bin by the rotation with angle q bin, seeded once per
+//   chunk from the series in xs_sincos.h (drift after 2048 steps ~1e-13).
+//   Chunk sums are scaled by 1/(q bin) and added into the result with FP64
+//   atomics.  The reference's denominator is q r + 1e-8 (CalcFunctions.cpp:861);
+//   1/(x + e) = (1/x)(1 - e/x) + O((e/x)^2) is applied per term, because at the
+//   cancelling low-q points that 1e-8 shifts I(q) by ~1e-6 relative.
+//   q = 0 gives S = 0 as the reference does (sin 0 = 0).
+// H4: the reference's PDF formulas (CalcFunctions.cpp:919-957) in FP64, one
+//   thread per bin, same operation order and no contraction, so the result
+//   equals the FP64 CPU path bit for bit for the same histogram.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "xs_internal.h"
+#include "xs_sincos.h"
+
+namespace xs {
+namespace iq {
+
+constexpr int kQBlock = 256;
+constexpr int kChunk = 2048;
+constexpr double kPi = 3.14159265358979323846;
+
+__global__ void fill_f64(double *__restrict__ p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+__global__ __launch_bounds__(kQBlock) void pairsum_kernel(const unsigned long long *__restrict__ hist,
+                                                          unsigned nhist, const double *__restrict__ q,
+                                                          unsigned nq, double bin,
+                                                          double *__restrict__ S) {
+  __shared__ double2 bins[kChunk];
+  const unsigned p = blockIdx.z;
+  const unsigned k0 = blockIdx.y * kChunk;
+  const unsigned kn = min((unsigned)kChunk, nhist - k0);
+  const unsigned long long *h = hist + (size_t)p * nhist + k0;
+  for (unsigned b = threadIdx.x; b < kn; b += kQBlock)
+    bins[b] = make_double2((double)h[b], 1.0 / ((double)(k0 + b) + 0.5));
+  __syncthreads();
+  const unsigned iq = blockIdx.x * kQBlock + threadIdx.x;
+  if (iq >= nq) return;
+  const double qv = q[iq];
+  if (qv == 0.0) return;
+  const double turns = qv * bin / (2.0 * kPi);  // phase step per bin, in turns
+  double s, c, ds, dc;
+  sincos_2pi(((double)k0 + 0.5) * turns, &s, &c);
+  sincos_2pi(turns, &ds, &dc);
+  const double eps_q = 1e-8 / (qv * bin);
+  double acc = 0.0;
+  for (unsigned b = 0; b < kn; ++b) {
+    const double2 e = bins[b];
+    if (e.x != 0.0) acc = fma(e.x * e.y * fma(-eps_q, e.y, 1.0), s, acc);
+    const double sn = fma(s, dc, c * ds);
+    c = fma(c, dc, -(s * ds));
+    s = sn;
+  }
+  if (acc != 0.0) atomicAdd(&S[(size_t)p * nq + iq], acc / (qv * bin));
+}
+
+struct PdfArgs {
+  const unsigned long long *hist;
+  double *pdf;
+  unsigned nhist, n_el, ntot;
+  int type;
+  int cpu_radius;
+  double bin, rho0;
+  unsigned el_count[16];
+  double sl[16];
+};
+
+__global__ void pdf_kernel(PdfArgs A) {
+#pragma clang fp contract(off)
+  const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= A.nhist) return;
+  double fav2 = 0;
+  for (unsigned e = 0; e < A.n_el; ++e) fav2 += A.sl[e] * A.el_count[e];
+  fav2 /= A.ntot;
+  fav2 *= fav2;
+  const double nt2 = (double)A.ntot * (double)A.ntot;
+  double total = 0.0;
+  size_t start = 0;
+  unsigned p = 0;
+  for (unsigned a = 0; a < A.n_el; ++a) {
+    for (unsigned b = a; b < A.n_el; ++b, ++p, start += A.nhist) {
+      const double wab = A.sl[a] * A.sl[b] / fav2;
+      const size_t i = start + k;
+      const double cnt = (double)A.hist[i];
+      const double r = ((A.cpu_radius ? (double)i : (double)k) + 0.5) * A.bin;
+      double v;
+      if (A.type == XS_PDF_RDF) {
+        v = cnt * (2. / (A.bin * A.ntot));
+      } else if (A.type == XS_PDF_PDF) {
+        const double mult = 0.5 / (kPi * A.bin * A.rho0 * A.ntot);
+        v = cnt * mult / (r * r);
+      } else {
+        const double mult = 2. / (A.bin * A.ntot);
+        const double sub =
+            (b > a) ? 8. * kPi * A.rho0 * (double)A.el_count[a] * (double)A.el_count[b] / nt2
+                    : 4. * kPi * A.rho0 * ((double)A.el_count[a] * (double)A.el_count[a]) / nt2;
+        v = cnt * mult / r - sub * r;
+      }
+      A.pdf[A.nhist + i] = v;
+      total += v * wab;
+    }
+  }
+  A.pdf[k] = total;
+}
+
+}  // namespace iq
+}  // namespace xs
+
+using namespace xs;
+
+extern "C" int xs_hist_pairsum(xs_ctx *ctx, const uint64_t *hist, unsigned ntypes, unsigned nhist,
+                               double bin, const double *q, unsigned nq, double *S,
+                               unsigned flags) {
+  XS_TRY(activate(ctx));
+  if (!hist || !q || !S) return fail(XS_ERR_ARG, "xs_hist_pairsum: null pointer");
+  if (!(bin > 0.0) || !std::isfinite(bin)) return fail(XS_ERR_ARG, "xs_hist_pairsum: bin must be > 0");
+  if (nhist > (1u << 24)) return fail(XS_ERR_RANGE, "xs_hist_pairsum: nhist > 2^24");
+  if (ntypes == 0 || nq == 0 || nhist == 0) return XS_OK;
+  const bool device = flags & XS_DEVICE_PTRS;
+  const size_t hsize = (size_t)ntypes * nhist, ssize = (size_t)ntypes * nq;
+  XS_TRY(ensure_workspace(ctx, Carver::need({device ? 0 : hsize * 8, device ? 0 : ssize * 8,
+                                             device ? 0 : (size_t)nq * 8})));
+  Carver cv(ctx->ws);
+  const unsigned long long *hd = reinterpret_cast<const unsigned long long *>(hist);
+  double *Sd = S;
+  const double *qd = q;
+  if (!device) {
+    auto *h2 = cv.take<unsigned long long>(hsize);
+    Sd = cv.take<double>(ssize);
+    auto *q2 = cv.take<double>(nq);
+    XS_HIP_CHECK(hipMemcpyAsync(h2, hist, hsize * 8, hipMemcpyHostToDevice, ctx->stream));
+    XS_HIP_CHECK(hipMemcpyAsync(q2, q, (size_t)nq * 8, hipMemcpyHostToDevice, ctx->stream));
+    hd = h2;
+    qd = q2;
+  }
+  const unsigned zb = (unsigned)std::min<size_t>((ssize + 255) / 256, 4096);
+  hipLaunchKernelGGL(iq::fill_f64, dim3(zb), dim3(256), 0, ctx->stream, Sd, ssize);
+  hipEvent_t ev;
+  XS_TRY(prof_begin(ctx, "hist_pairsum", &ev));
+  dim3 grid((nq + iq::kQBlock - 1) / iq::kQBlock, (nhist + iq::kChunk - 1) / iq::kChunk, ntypes);
+  hipLaunchKernelGGL(iq::pairsum_kernel, grid, dim3(iq::kQBlock), 0, ctx->stream, hd, nhist, qd, nq,
+                     bin, Sd);
+  XS_HIP_CHECK(hipGetLastError());
+  XS_TRY(prof_end(ctx, "hist_pairsum", ev));
+  if (!device) {
+    XS_HIP_CHECK(hipMemcpyAsync(S, Sd, ssize * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  }
+  return XS_OK;
+}
+
+extern "C" int xs_pdf(xs_ctx *ctx, const uint64_t *hist, unsigned n_el, const unsigned *el_count,
+                      unsigned nhist, double bin, int pdf_type, double rho0, const double *sl,
+                      double *pdf, unsigned flags) {
+  XS_TRY(activate(ctx));
+  if (!hist || !el_count || !sl || !pdf) return fail(XS_ERR_ARG, "xs_pdf: null pointer");
+  if (n_el == 0 || n_el > 16) return fail(XS_ERR_RANGE, "xs_pdf: n_el must be in [1, 16]");
+  if (pdf_type < XS_PDF_RDF || pdf_type > XS_PDF_RPDF) return fail(XS_ERR_ARG, "xs_pdf: bad type");
+  if (!(bin > 0.0)) return fail(XS_ERR_ARG, "xs_pdf: bin must be > 0");
+  if (nhist == 0) return XS_OK;
+  iq::PdfArgs A{};
+  unsigned ntot = 0;
+  for (unsigned e = 0; e < n_el; ++e) {
+    A.el_count[e] = el_count[e];
+    A.sl[e] = sl[e];
+    ntot += el_count[e];
+  }
+  if (ntot == 0) return fail(XS_ERR_ARG, "xs_pdf: no atoms");
+  const unsigned ntypes = ntypes_of(n_el);
+  const bool device = flags & XS_DEVICE_PTRS;
+  const size_t hsize = (size_t)ntypes * nhist, psize = (size_t)(1 + ntypes) * nhist;
+  Carver cv(nullptr);
+  if (!device) {
+    XS_TRY(ensure_workspace(ctx, Carver::need({hsize * 8, psize * 8})));
+    cv = Carver(ctx->ws);
+  }
+  A.hist = reinterpret_cast<const unsigned long long *>(hist);
+  A.pdf = pdf;
+  if (!device) {
+    auto *h2 = cv.take<unsigned long long>(hsize);
+    A.pdf = cv.take<double>(psize);
+    XS_HIP_CHECK(hipMemcpyAsync(h2, hist, hsize * 8, hipMemcpyHostToDevice, ctx->stream));
+    A.hist = h2;
+  }
+  A.nhist = nhist;
+  A.n_el = n_el;
+  A.ntot = ntot;
+  A.type = pdf_type;
+  A.cpu_radius = (flags & XS_PDF_CPU_RADIUS) ? 1 : 0;
+  A.bin = bin;
+  A.rho0 = rho0;
+  hipEvent_t ev;
+  XS_TRY(prof_begin(ctx, "pdf", &ev));
+  hipLaunchKernelGGL(iq::pdf_kernel, dim3((nhist + 255) / 256), dim3(256), 0, ctx->stream, A);
+  XS_HIP_CHECK(hipGetLastError());
+  XS_TRY(prof_end(ctx, "pdf", ev));
+  if (!device) {
+    XS_HIP_CHECK(hipMemcpyAsync(pdf, A.pdf, psize * 8, hipMemcpyDeviceToHost, ctx->stream));
+    XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  }
+  return XS_OK;
+}

@@ -1,0 +1,126 @@
+"""H1 parity: the HIP histogram equals the oracle bit for bit (FP32 binning
+against or_hist_f32, FP64 binning against or_hist_f64, which matches the
+reference CPU executable)."""
+import numpy as np
+import pytest
+try:
+    import torch  # noqa: F401  (first import can take a minute on a fresh box)
+except ImportError:
+    torch = None
+
+import xs_oracle as O
+from conftest import golden
+from xansons_hip import structures as st
+
+pytestmark = pytest.mark.gpu
+
+
+def both_modes(ctx, ens, bin_, nhist=None):
+    x64 = ens.xyz64()
+    nh = nhist or O.hist_size(x64, bin_)
+    g32, ov32 = ctx.hist(ens.xyz, ens.el_count, bin_, nh, f64=False)
+    o32, oo32 = O.hist_f32(ens.xyz, ens.el_count, bin_, nh)
+    g64, ov64 = ctx.hist(x64, ens.el_count, bin_, nh, f64=True)
+    o64, oo64 = O.hist_f64(x64, ens.el_count, bin_, nh)
+    return (g32, ov32, o32, oo32), (g64, ov64, o64, oo64)
+
+
+def assert_same(pair, label):
+    g, ov, o, oo = pair
+    print(f"{label}: total={int(g.sum())} mismatched_bins={int(np.sum(g != o))} ovf={ov}/{oo}", flush=True)
+    assert ov == oo
+    assert np.array_equal(g, o)
+
+
+def test_fcc_sphere_c1(ctx):
+    g = golden("fcc1985")
+    ens = st.Ensemble(g["xyz"], list(g["el_count"]), ["Au"])
+    r32, r64 = both_modes(ctx, ens, 0.01)
+    assert_same(r32, "c1 f32")
+    assert_same(r64, "c1 f64")
+    # FP64 binning also equals the reference executable's own counts
+    rec = np.rint(g["rdf"] * 0.01 * 1985 / 2).astype(np.uint64)
+    assert np.array_equal(r64[0][0, :rec.size], rec)
+
+
+def test_rocksalt_small_three_types(ctx):
+    g = golden("nacl_small")
+    ens = st.Ensemble(g["xyz"], list(g["el_count"]), ["Na", "Cl"])
+    r32, r64 = both_modes(ctx, ens, 0.005)
+    assert_same(r32, "nacl f32")
+    assert_same(r64, "nacl f64")
+
+
+@pytest.mark.parametrize("counts", [[1], [2], [1023], [1024, 1], [1025, 0, 3], [2049, 700],
+                                    [5, 6, 7, 3000]])
+def test_ragged_sizes(ctx, counts):
+    ens = st.random_cluster([max(c, 0) for c in counts], box=8.0 + 0.02 * sum(counts) ** (1 / 3) * 10,
+                            min_dist=0.05, seed=sum(counts)) if sum(counts) <= 4000 else None
+    r32, r64 = both_modes(ctx, ens, 0.01)
+    assert_same(r32, f"ragged {counts} f32")
+    assert_same(r64, f"ragged {counts} f64")
+
+
+def test_sliding_window_large_nhist(ctx):
+    """nhist far above the LDS window: the window slides and flushes."""
+    ens = st.rocksalt_cube(ncell=18, vacancy=0.02, sigma=0.08, seed_vac=31, seed_jit=32)
+    r32, r64 = both_modes(ctx, ens, 0.001)
+    assert r32[0].shape[1] > 100_000
+    assert_same(r32, "window f32")
+    assert_same(r64, "window f64")
+
+
+def test_direct_global_path_tiny_bin(ctx):
+    """Bins so fine that one tile pair spans more than the LDS window: those
+    pairs take global atomics directly."""
+    ens = st.random_cluster([1500, 1200], box=40.0, min_dist=0.5, seed=77)
+    r32, r64 = both_modes(ctx, ens, 0.0002)
+    assert_same(r32, "direct f32")
+    assert_same(r64, "direct f64")
+
+
+def test_overflow_when_nhist_too_small(ctx):
+    ens = st.fcc_sphere(radius=9.0, seed=3)
+    r32, r64 = both_modes(ctx, ens, 0.01, nhist=700)
+    assert r32[1] > 0
+    assert_same(r32, "overflow f32")
+    assert_same(r64, "overflow f64")
+
+
+def test_device_pointers_match_host(ctx):
+    import torch
+    ens = st.fcc_sphere(radius=12.0, seed=4)
+    nh = O.hist_size(ens.xyz64(), 0.01)
+    host, _ = ctx.hist(ens.xyz, ens.el_count, 0.01, nh)
+    dev = torch.device("cuda", 0)
+    x = torch.from_numpy(np.ascontiguousarray(ens.xyz)).to(dev)
+    h = torch.empty(nh, dtype=torch.int64, device=dev)
+    ov = torch.empty(1, dtype=torch.int64, device=dev)
+    ctx.hist_dev(x.data_ptr(), ens.n, ens.el_count, 0.01, nh, h.data_ptr(), ov.data_ptr())
+    ctx.synchronize()
+    assert np.array_equal(h.cpu().numpy().astype(np.uint64), host[0])
+    assert int(ov.item()) == 0
+
+
+def test_rocksalt_million_properties(ctx):
+    """Full 980,219-atom rock-salt cube: exact totals, no overflow, repeatable,
+    FP32/FP64 agree to bin-edge pairs, and a 24k-atom subset is bit-exact."""
+    ens = st.rocksalt_cube()
+    x64 = ens.xyz64()
+    nh = O.hist_size(x64, 0.005)
+    h1, ov1 = ctx.hist(ens.xyz, ens.el_count, 0.005, nh)
+    h2, ov2 = ctx.hist(ens.xyz, ens.el_count, 0.005, nh)
+    na, nb = ens.el_count
+    assert ov1 == 0 and ov2 == 0
+    assert [int(v) for v in h1.sum(axis=1)] == [na * (na - 1) // 2, na * nb, nb * (nb - 1) // 2]
+    assert np.array_equal(h1, h2)
+    h64, ov64 = ctx.hist(x64, ens.el_count, 0.005, nh, f64=True)
+    moved = np.abs(h1.astype(np.int64) - h64.astype(np.int64)).sum() // 2
+    print(f"c2 f32 vs f64 moved pairs {moved} of {int(h1.sum())}", flush=True)
+    assert moved < 1e-3 * h1.sum()
+    sub = st.Ensemble(np.ascontiguousarray(np.concatenate(
+        [ens.element(0)[:, ::40], ens.element(1)[:, ::40]], axis=1)),
+        [ens.element(0)[:, ::40].shape[1], ens.element(1)[:, ::40].shape[1]], ens.names)
+    r32, r64 = both_modes(ctx, sub, 0.005, nhist=nh)
+    assert_same(r32, "c2 subset f32")
+    assert_same(r64, "c2 subset f64")
