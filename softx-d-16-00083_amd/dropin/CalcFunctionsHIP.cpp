@@ -1,0 +1,323 @@
+// HIP drop-in for the XaNSoNS CPU calculators.
+//
+// Defines the four calculators with the CPU signatures that main.cpp declares
+// and calls (main.cpp:65-68, 151-233) and computes them on an MI355X through
+// the C ABI of libxs_hip (include/xs_hip.h).  Compiled against the
+// reference's own typedefs.h (config, block, vect3d); linked in place of the
+// CPU calculator definitions (CalcFunctions.cpp:546-1092).
+//
+// What stays on the host, in FP64, exactly as the CPU path does it:
+//   the form-factor weighting x 2 f_a f_b, the self term N_el f_el^2, the
+//   division by Ntot and the polarisation factor (CalcFunctions.cpp:592-626,
+//   704-751, 872-906), the Euler grid and rotation matrices, the polar
+//   (q, phi) mesh with cos(theta) = 1 - sin^2(theta), the 2D normalisation and
+//   the phi average (CalcFunctions.cpp:999-1075).
+// What runs on the GPU: the pair sums (direct and histogram), the histogram
+//   itself (FP64 binning, so the counts equal calcHist's) and the PDFs.
+//
+// Ownership and output conventions follow the reference: outputs are
+// allocated here with new[] and released by main (main.cpp:239-245).
+// Errors: a failing device call prints the library message and exits with a
+// non-zero status (the reference printed and continued on garbage,
+// CalcFunctionsCUDA.cu:878-879).
+//
+// Environment:
+//   XANSONS_HIP_DEVICE   device index (default 0)
+//   XANSONS_PDF_RADIUS   "cpu" (default): PDF/rPDF radius of the CPU loop,
+//                        which offsets pair types p >= 1 by p*Nhist bins
+//                        (CalcFunctions.cpp:942, 950); "local": the radius of
+//                        the CUDA/OpenCL kernels (CalcFunctionsCUDA.cu:1057)
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "typedefs.h"
+#include "xs_hip.h"
+
+extern int myid, numprocs;
+
+namespace {
+
+xs_ctx *g_ctx = nullptr;
+
+void release_ctx() {
+  if (g_ctx) xs_free(g_ctx);
+  g_ctx = nullptr;
+}
+
+void check(int rc, const char *where) {
+  if (rc != XS_OK) {
+    std::fprintf(stderr, "Error in %s: %s (code %d)\n", where, xs_last_error(), rc);
+    std::exit(EXIT_FAILURE);
+  }
+}
+
+xs_ctx *ctx() {
+  if (!g_ctx) {
+    const char *dev = std::getenv("XANSONS_HIP_DEVICE");
+    check(xs_init(dev ? std::atoi(dev) : 0, &g_ctx), "xs_init");
+    std::atexit(release_ctx);
+  }
+  return g_ctx;
+}
+
+// Element-major structure-of-arrays copy of the ensemble (CalcFunctions.cpp:159-196
+// order; the CUDA path flattens the same way, CalcFunctionsCUDA.cu:1548-1552).
+std::vector<double> flatten(const vector<vect3d<double> > *ra, const unsigned *NatomEl,
+                            unsigned Nel, size_t *n_out) {
+  size_t n = 0;
+  for (unsigned e = 0; e < Nel; ++e) n += NatomEl[e];
+  std::vector<double> xyz(3 * n);
+  size_t o = 0;
+  for (unsigned e = 0; e < Nel; ++e)
+    for (unsigned i = 0; i < NatomEl[e]; ++i, ++o) {
+      xyz[o] = ra[e][i].x;
+      xyz[n + o] = ra[e][i].y;
+      xyz[2 * n + o] = ra[e][i].z;
+    }
+  *n_out = n;
+  return xyz;
+}
+
+double weight(const config *cfg, const vector<double *> &FF, const vector<double> &SL,
+              unsigned el, unsigned iq) {
+  return cfg->source == xray ? FF[el][iq] : SL[el];
+}
+
+void polar_1d(double *I, const double *q, double lambda, unsigned Nq) {
+  for (unsigned iq = 0; iq < Nq; ++iq) {
+    const double st = q[iq] * (lambda * 0.25 / PI);
+    const double c2t = 1. - 2. * SQR(st);
+    I[iq] *= 0.5 * (1. + SQR(c2t));
+  }
+}
+
+// I(q) from pair sums S[p*Nq + iq]: the combination of CalcFunctions.cpp:592-626.
+void combine_1d(double *I, const double *S, const config *cfg, const unsigned *NatomEl,
+                const vector<double *> &FF, const vector<double> &SL, const double *q,
+                unsigned Ntot) {
+  const unsigned Nq = cfg->q.N;
+  for (unsigned iq = 0; iq < Nq; ++iq) I[iq] = 0;
+  unsigned p = 0;
+  for (unsigned a = 0; a < cfg->Nel; ++a)
+    for (unsigned b = a; b < cfg->Nel; ++b, ++p)
+      for (unsigned iq = 0; iq < Nq; ++iq)
+        I[iq] += S[(size_t)p * Nq + iq] * weight(cfg, FF, SL, a, iq) * weight(cfg, FF, SL, b, iq) * 2.;
+  for (unsigned e = 0; e < cfg->Nel; ++e)
+    for (unsigned iq = 0; iq < Nq; ++iq) I[iq] += NatomEl[e] * SQR(weight(cfg, FF, SL, e, iq));
+  for (unsigned iq = 0; iq < Nq; ++iq) I[iq] /= Ntot;
+  if (cfg->PolarFactor) polar_1d(I, q, cfg->lambda, Nq);
+}
+
+float seconds_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration_cast<std::chrono::duration<float> >(std::chrono::steady_clock::now() - t0)
+      .count();
+}
+
+// Rotation matrix rows for intrinsic Euler angles, R = R_a1(e.x) R_a2(e.y) R_a3(e.z),
+// axes per convention as numbered in typedefs.h:59-70.
+void rotation(const vect3d<double> &e, unsigned convention, double R[9]) {
+  static const int axes[12][3] = {{0, 2, 0}, {0, 1, 0}, {1, 0, 1}, {1, 2, 1}, {2, 1, 2}, {2, 0, 2},
+                                  {0, 2, 1}, {0, 1, 2}, {1, 0, 2}, {1, 2, 0}, {2, 1, 0}, {2, 0, 1}};
+  const double ang[3] = {e.x, e.y, e.z};
+  double acc[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  for (int s = 0; s < 3; ++s) {
+    const int ax = axes[convention][s], u = (ax + 1) % 3, v = (ax + 2) % 3;
+    const double c = std::cos(ang[s]), sn = std::sin(ang[s]);
+    double M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    M[ax * 3 + ax] = 1;
+    M[u * 3 + u] = c;
+    M[u * 3 + v] = -sn;
+    M[v * 3 + u] = sn;
+    M[v * 3 + v] = c;
+    double T[9];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        T[i * 3 + j] = acc[i * 3] * M[j] + acc[i * 3 + 1] * M[3 + j] + acc[i * 3 + 2] * M[6 + j];
+    std::memcpy(acc, T, sizeof(T));
+  }
+  std::memcpy(R, acc, sizeof(acc));
+}
+
+}  // namespace
+
+void calcIntDebye(double **I, const config *cfg, const unsigned int *NatomEl,
+                  const vector<vect3d<double> > *ra, vector<double *> FF, vector<double> SL,
+                  const double *q, unsigned int Ntot, int NumOMPthreads) {
+  (void)NumOMPthreads;
+  const auto t0 = std::chrono::steady_clock::now();
+  const unsigned Nq = cfg->q.N, P = cfg->Nel * (cfg->Nel + 1) / 2;
+  size_t n = 0;
+  std::vector<double> xyz = flatten(ra, NatomEl, cfg->Nel, &n);
+  std::vector<double> S((size_t)P * Nq, 0.0);
+  check(xs_debye_pairsum(ctx(), xyz.data(), n, NatomEl, cfg->Nel, q, Nq, S.data(), 0),
+        "calcIntDebye");
+  *I = new double[Nq];
+  combine_1d(*I, S.data(), cfg, NatomEl, FF, SL, q, Ntot);
+  if (!myid) cout << "1D pattern calculation time: " << seconds_since(t0) << " s" << endl;
+}
+
+void calcIntPartialDebye(double **I, const config *cfg, const unsigned int *NatomEl,
+                         const vector<vect3d<double> > *ra, vector<double *> FF, vector<double> SL,
+                         const double *q, const block *Block, unsigned int Ntot,
+                         int NumOMPthreads) {
+  (void)NumOMPthreads;
+  const auto t0 = std::chrono::steady_clock::now();
+  const unsigned Nq = cfg->q.N, Nb = cfg->Nblocks, Nel = cfg->Nel;
+  const unsigned Nparts = Nb * (Nb + 1) / 2, P = Nel * (Nel + 1) / 2;
+  // atoms of element e in block b (CalcFunctions.cpp:650-661)
+  std::vector<unsigned> blk((size_t)Nel * Nb, 0);
+  for (unsigned e = 0; e < Nel; ++e)
+    for (unsigned b = 0; b < Nb; ++b)
+      for (unsigned t = 0; t < Block[b].Nid; ++t)
+        if (Block[b].id[t] == e) {
+          blk[(size_t)e * Nb + b] = Block[b].NatomElAll[t];
+          break;
+        }
+  size_t n = 0;
+  std::vector<double> xyz = flatten(ra, NatomEl, Nel, &n);
+  std::vector<double> S((size_t)P * Nparts * Nq, 0.0);
+  check(xs_partial_debye_pairsum(ctx(), xyz.data(), n, NatomEl, Nel, blk.data(), Nb, q, Nq,
+                                 S.data(), 0),
+        "calcIntPartialDebye");
+  const size_t Isize = (size_t)Nparts * Nq;
+  *I = new double[Isize + Nq];
+  double *out = *I;
+  for (size_t i = 0; i < Isize + Nq; ++i) out[i] = 0;
+  unsigned p = 0;
+  for (unsigned a = 0; a < Nel; ++a)
+    for (unsigned b = a; b < Nel; ++b, ++p)
+      for (unsigned part = 0; part < Nparts; ++part)
+        for (unsigned iq = 0; iq < Nq; ++iq)
+          out[(size_t)(part + 1) * Nq + iq] += S[((size_t)p * Nparts + part) * Nq + iq] *
+                                               weight(cfg, FF, SL, a, iq) *
+                                               weight(cfg, FF, SL, b, iq) * 2.;
+  for (unsigned e = 0; e < Nel; ++e)
+    for (unsigned b = 0; b < Nb; ++b) {
+      const size_t start = (size_t)Nq * (1 + (Nb * b - (b * (b + 1)) / 2 + b));
+      for (unsigned iq = 0; iq < Nq; ++iq)
+        out[start + iq] += blk[(size_t)e * Nb + b] * SQR(weight(cfg, FF, SL, e, iq));
+    }
+  for (size_t i = Nq; i < Isize + Nq; ++i) out[i] /= Ntot;
+  if (cfg->PolarFactor)
+    for (unsigned part = 1; part < Nparts + 1; ++part) polar_1d(out + (size_t)part * Nq, q, cfg->lambda, Nq);
+  for (unsigned part = 1; part < Nparts + 1; ++part)
+    for (unsigned iq = 0; iq < Nq; ++iq) out[iq] += out[(size_t)part * Nq + iq];
+  if (!myid) cout << "1D pattern calculation time: " << seconds_since(t0) << " s" << endl;
+}
+
+void calcPDFandDebye(double **I, double **PDF, const config *cfg, const unsigned int *NatomEl,
+                     const vector<vect3d<double> > *ra, vector<double *> FF, vector<double> SL,
+                     const double *q, unsigned int Ntot, int NumOMPthreads) {
+  (void)NumOMPthreads;
+  auto t0 = std::chrono::steady_clock::now();
+  const unsigned Nel = cfg->Nel, P = Nel * (Nel + 1) / 2, Nhist = cfg->Nhist;
+  size_t n = 0;
+  std::vector<double> xyz = flatten(ra, NatomEl, Nel, &n);
+  std::vector<uint64_t> hist((size_t)P * Nhist, 0);
+  uint64_t overflow = 0;
+  check(xs_hist(ctx(), xyz.data(), n, NatomEl, Nel, cfg->hist_bin, Nhist, hist.data(), &overflow,
+                XS_F64_BINNING),
+        "calcPDFandDebye (histogram)");
+  if (overflow && !myid)
+    cerr << "Warning: " << overflow << " pair distances fell beyond the last histogram bin." << endl;
+  if (!myid) cout << "Histogram calculation time: " << seconds_since(t0) << " s" << endl;
+  if (cfg->scenario > Debye_hist) {
+    t0 = std::chrono::steady_clock::now();
+    const char *mode = std::getenv("XANSONS_PDF_RADIUS");
+    const unsigned flags = (mode && std::strcmp(mode, "local") == 0) ? 0u : XS_PDF_CPU_RADIUS;
+    *PDF = new double[(size_t)(1 + P) * Nhist];
+    check(xs_pdf(ctx(), hist.data(), Nel, NatomEl, Nhist, cfg->hist_bin, (int)cfg->PDFtype, cfg->p0,
+                 SL.data(), *PDF, flags),
+          "calcPDFandDebye (PDF)");
+    if (!myid) cout << "PDF calculation time: " << seconds_since(t0) << " s" << endl;
+  }
+  if (cfg->scenario == Debye_hist || cfg->scenario == DebyePDF) {
+    t0 = std::chrono::steady_clock::now();
+    const unsigned Nq = cfg->q.N;
+    std::vector<double> S((size_t)P * Nq, 0.0);
+    check(xs_hist_pairsum(ctx(), hist.data(), P, Nhist, cfg->hist_bin, q, Nq, S.data(), 0),
+          "calcPDFandDebye (intensity)");
+    *I = new double[Nq];
+    double *out = *I;
+    for (unsigned iq = 0; iq < Nq; ++iq) out[iq] = 0;
+    unsigned p = 0;
+    for (unsigned a = 0; a < Nel; ++a)
+      for (unsigned b = a; b < Nel; ++b, ++p)
+        for (unsigned iq = 0; iq < Nq; ++iq)
+          out[iq] += 2. * S[(size_t)p * Nq + iq] * weight(cfg, FF, SL, a, iq) * weight(cfg, FF, SL, b, iq);
+    for (unsigned e = 0; e < Nel; ++e)
+      for (unsigned iq = 0; iq < Nq; ++iq) out[iq] += NatomEl[e] * SQR(weight(cfg, FF, SL, e, iq));
+    for (unsigned iq = 0; iq < Nq; ++iq) out[iq] /= Ntot;
+    if (cfg->PolarFactor) polar_1d(out, q, cfg->lambda, Nq);
+    if (!myid) cout << "1D pattern calculation time: " << seconds_since(t0) << " s" << endl;
+  }
+}
+
+void calcInt2D(double ***I2D, double **I, const config *cfg, const unsigned int *NatomEl,
+               const vector<vect3d<double> > *ra, vector<double *> FF, vector<double> SL,
+               const double *q, unsigned int Ntot, int NumOMPthreads) {
+  (void)NumOMPthreads;
+  const auto t0 = std::chrono::steady_clock::now();
+  const unsigned Nq = cfg->q.N, Nfi = cfg->Nfi, Nel = cfg->Nel;
+  const size_t N2D = (size_t)Nq * Nfi;
+  // polar mesh: q (cos(theta) cos(phi), cos(theta) sin(phi), -sin(theta)) with
+  // sin(theta) = q lambda/4pi and cos(theta) = 1 - sin^2(theta) (CalcFunctions.cpp:999-1005, 1031-1032)
+  std::vector<double> qvec(3 * N2D), w((size_t)Nel * N2D);
+  const double dfi = 2. * PI / Nfi;
+  for (unsigned iq = 0; iq < Nq; ++iq) {
+    const double st = q[iq] * (cfg->lambda * 0.25 / PI);
+    const double ct = 1. - SQR(st);
+    for (unsigned ifi = 0; ifi < Nfi; ++ifi) {
+      const size_t k = (size_t)iq * Nfi + ifi;
+      qvec[3 * k] = ct * std::cos(ifi * dfi) * q[iq];
+      qvec[3 * k + 1] = ct * std::sin(ifi * dfi) * q[iq];
+      qvec[3 * k + 2] = -st * q[iq];
+      for (unsigned e = 0; e < Nel; ++e) w[(size_t)e * N2D + k] = weight(cfg, FF, SL, e, iq);
+    }
+  }
+  // Euler midpoint grid (CalcFunctions.cpp:1007-1017)
+  const mesh3d &E = cfg->Euler;
+  const double da = E.N.x < 2 ? 0 : (E.max.x - E.min.x) / E.N.x;
+  const double db = E.N.y < 2 ? 0 : (E.max.y - E.min.y) / E.N.y;
+  const double dg = E.N.z < 2 ? 0 : (E.max.z - E.min.z) / E.N.z;
+  const unsigned Nrot = E.N.x * E.N.y * E.N.z;
+  std::vector<double> rot((size_t)Nrot * 9);
+  unsigned r = 0;
+  for (unsigned ia = 0; ia < E.N.x; ++ia)
+    for (unsigned ib = 0; ib < E.N.y; ++ib)
+      for (unsigned ig = 0; ig < E.N.z; ++ig, ++r) {
+        vect3d<double> e(E.min.x + (ia + 0.5) * da, E.min.y + (ib + 0.5) * db, E.min.z + (ig + 0.5) * dg);
+        rotation(e, cfg->EulerConvention, &rot[(size_t)r * 9]);
+      }
+  size_t n = 0;
+  std::vector<double> xyz = flatten(ra, NatomEl, Nel, &n);
+  std::vector<double> acc(N2D, 0.0);
+  check(xs_amp2d(ctx(), xyz.data(), n, NatomEl, Nel, qvec.data(), N2D, rot.data(), Nrot, w.data(),
+                 acc.data(), 0),
+        "calcInt2D");
+  *I = new double[Nq];
+  *I2D = new double *[Nq];
+  const double norm = 1. / (Ntot * E.N.x * E.N.y * E.N.z);
+  for (unsigned iq = 0; iq < Nq; ++iq) {
+    (*I2D)[iq] = new double[Nfi];
+    double f = 1.0;
+    if (cfg->PolarFactor) {
+      const double st = q[iq] * (cfg->lambda * 0.25 / PI);
+      const double c2t = 1. - 2. * SQR(st);
+      f = 0.5 * (1. + SQR(c2t));
+    }
+    double s = 0;
+    for (unsigned ifi = 0; ifi < Nfi; ++ifi) {
+      double v = acc[(size_t)iq * Nfi + ifi] * norm;
+      if (cfg->PolarFactor) v *= f;
+      (*I2D)[iq][ifi] = v;
+      s += v;
+    }
+    (*I)[iq] = s / Nfi;
+  }
+  if (!myid) cout << "2D pattern calculation time: " << seconds_since(t0) << " s" << endl;
+}
