@@ -9,7 +9,13 @@ Workload (one step): the 980,219-atom two-element rock-salt cube
 Inputs are resident in HBM before the timed region; outputs stay on the GPU.
 value = pairs per second of the whole step (Gpairs/s).
 
+A second object, "ensemble", times the batched path on 1,000 nanoparticles
+(500-200,000 atoms, 1-4 elements, bin 0.01 A, 2,048 q; histograms, pair sums
+and rPDF of every particle): one plan, every run replayed as a hipGraph,
+against the same kernels called structure by structure.
+
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                  [--ensemble N] [--copy-probe]
 """
 from __future__ import annotations
 
@@ -122,17 +128,251 @@ def cpu_baseline(ens, nh, stride=16):
                 sample=sample + "; C restatement of calcHist (FP64, zig-zag rows)", seconds=t)
 
 
+def copy_probe(nbytes=1 << 30):
+    """One device-to-device copy of a known size, to scale the HBM counters."""
+    import torch
+    a = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes}))
+
+
+def ensemble_leg(ctx, count, steps, warmup):
+    """Batched (one plan, graph replays) vs structure-by-structure runs of the
+    same kernels on `count` nanoparticles: histograms (FP64 binning), pair
+    sums on 2,048 q and rPDF per particle.  Both write the same layout, which
+    is compared after the timed runs."""
+    import torch
+    import xansons_hip as xh
+    from xansons_hip import structures as st2
+    bin_, nq = 0.01, 2048
+    t0 = time.perf_counter()
+    parts = st2.particle_ensemble(count)
+    t_gen = time.perf_counter() - t0
+    pairs = sum(p.n * (p.n - 1) // 2 for p in parts)
+    q = st2.q_mesh(0.2, 25.0, nq)
+    # rho0 as GetAtomicDensity (CalcFunctions.cpp:212-223): N over the sphere
+    # around the centroid that reaches the farthest atom
+    def density(x):
+        r = np.sqrt(((x - x.mean(axis=1, keepdims=True)) ** 2).sum(axis=0)).max()
+        return x.shape[1] / (4.0 / 3.0 * np.pi * max(r, 1e-3) ** 3)
+
+    rho = np.array([density(p.xyz64()) for p in parts])
+    sl = np.concatenate([np.linspace(2.0, 9.0, p.n_el) for p in parts])
+    bctx = xh.Context(0)          # own non-default stream (graph capture)
+    t0 = time.perf_counter()
+    b = xh.Batch(bctx, [(p.xyz64(), p.el_count) for p in parts], bin_, f64=True)
+    t_plan = time.perf_counter() - t0
+    dev = torch.device("cuda", 0)
+    qd = torch.from_numpy(q).to(dev)
+    hist = torch.empty(b.hist_len, dtype=torch.int64, device=dev)
+    S = torch.empty(b.ntypes_total * nq, dtype=torch.float64, device=dev)
+    pdf = torch.empty(b.pdf_len, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def batched():
+        b.run_dev(hist.data_ptr(), qd.data_ptr(), nq, S.data_ptr(), "rPDF", rho, sl,
+                  pdf.data_ptr(), graph=True)
+
+    for _ in range(warmup):
+        batched()
+    bctx.synchronize()
+    bctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batched()
+    bctx.synchronize()
+    t_b = (time.perf_counter() - t0) / steps
+    g_ms = bctx.profile_read("batch_graph")
+    bctx.profile(False)
+    # kernel split of one eager run (same launches, outside the graph)
+    bctx.profile(True)
+    b.run_dev(hist.data_ptr(), qd.data_ptr(), nq, S.data_ptr(), "rPDF", rho, sl, pdf.data_ptr())
+    bctx.synchronize()
+    split_ms = {k: round(bctx.profile_read(k)[0], 3) for k in ("hist", "batch_pairsum", "batch_pdf")}
+    bctx.profile(False)
+
+    # structure by structure, same kernels, outputs in the batch layout
+    xyz_d = [torch.from_numpy(p.xyz64()).to(dev) for p in parts]
+    hist2 = torch.empty_like(hist)
+    S2 = torch.empty_like(S)
+    pdf2 = torch.empty_like(pdf)
+    torch.cuda.synchronize()
+
+    def plain():
+        e0 = 0
+        for s, p in enumerate(parts):
+            P = p.n_el * (p.n_el + 1) // 2
+            nh = int(b.nhist[s])
+            hp = hist2.data_ptr() + 8 * int(b.hist_off[s])
+            bctx.hist_dev(xyz_d[s].data_ptr(), p.n, p.el_count, bin_, nh, hp, None, f64=True)
+            bctx.hist_pairsum_dev(hp, P, nh, bin_, qd.data_ptr(), nq,
+                                  S2.data_ptr() + 8 * int(b.type_first[s]) * nq)
+            bctx.pdf_dev(hp, p.el_count, nh, bin_, "rPDF", rho[s], sl[e0:e0 + p.n_el],
+                         pdf2.data_ptr() + 8 * int(b.pdf_off[s]), cpu_radius=True)
+            e0 += p.n_el
+
+    plain()
+    bctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plain()
+    bctx.synchronize()
+    t_p = (time.perf_counter() - t0) / steps
+    torch.cuda.synchronize()
+    hist_eq = bool(torch.equal(hist, hist2))
+    pdf_eq = bool(torch.equal(pdf, pdf2))
+    s_rel = float(((S - S2).abs().max() / S2.abs().max()).item())
+    ovf = int(b.overflow().sum())
+    b.close()
+    bctx.close()
+    hist_ms = split_ms["hist"]
+    return {
+        "workload": f"{count} nanoparticles, N log-uniform 500-200,000 (mean {sum(p.n for p in parts) // count}), "
+                    "1-4 elements, fcc/bcc sphere, cube, core-shell; bin 0.01 A; 2,048 q 0.2-25 A^-1; "
+                    "histograms + pair sums + rPDF per particle",
+        "pairs": pairs,
+        "patterns_per_s_batched": round(count / t_b, 2),
+        "patterns_per_s_plain": round(count / t_p, 2),
+        "speedup": round(t_p / t_b, 3),
+        "ms_batched": round(t_b * 1e3, 2),
+        "ms_plain": round(t_p * 1e3, 2),
+        "gpairs_per_s_batched": round(pairs / t_b / 1e9, 2),
+        "graph_ms": round(g_ms[0] / max(g_ms[1], 1), 3),
+        "kernels_ms_eager_run": split_ms,
+        "hist_roofline": {"bound": "valu", "achieved": round(OPS_PER_PAIR * pairs / (hist_ms * 1e-3) / 1e12, 3),
+                          "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                          "frac": round(OPS_PER_PAIR * pairs / (hist_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)},
+        "checks": {"hist_equal_plain": hist_eq, "pdf_equal_plain": pdf_eq,
+                   "pairsum_max_rel_diff_vs_plain": s_rel, "overflow": ovf},
+        "setup_s": {"generate": round(t_gen, 2), "plan": round(t_plan, 2)},
+    }
+
+
+def pairsum_matrix_leg(ctx, steps):
+    """Histogram -> pair sums for many histograms on one bin grid: 256
+    two-element particles (768 histograms, 16,384 bins of 0.004 A), 8,192 q
+    over 0.2-25 A^-1.  The FP64 vector kernel against the int8 matrix-core
+    product (XS_PAIRSUM_MFMA); errors against an FP64 product on the CPU."""
+    import torch
+    import xansons_hip as xh
+    nh, bin_, nq = 16384, 0.004, 8192
+    parts = st.c4_particles()
+    b = xh.Batch(ctx, [(p.xyz64(), p.el_count) for p in parts], bin_, nhist=[nh] * len(parts))
+    dev = torch.device("cuda", 0)
+    hist = torch.empty(b.hist_len, dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()
+    b.run_dev(hist.data_ptr())
+    ctx.synchronize()
+    ovf = int(b.overflow().sum())
+    ncols = b.ntypes_total
+    b.close()
+    q = st.q_mesh(0.2, 25.0, nq)
+    qd = torch.from_numpy(q).to(dev)
+    Sv = torch.empty(ncols * nq, dtype=torch.float64, device=dev)
+    Sm = torch.empty_like(Sv)
+    torch.cuda.synchronize()
+
+    def timed(fn, names):
+        fn()
+        ctx.synchronize()
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        ctx.synchronize()
+        t = (time.perf_counter() - t0) / steps
+        k = {n: ctx.profile_read(n)[0] / steps for n in names}
+        ctx.profile(False)
+        return t, k
+
+    tv, kv = timed(lambda: ctx.hist_pairsum_dev(hist.data_ptr(), ncols, nh, bin_, qd.data_ptr(), nq,
+                                                Sv.data_ptr()), ["hist_pairsum"])
+    tm, km = timed(lambda: ctx.hist_pairsum_dev(hist.data_ptr(), ncols, nh, bin_, qd.data_ptr(), nq,
+                                                Sm.data_ptr(), mfma=True),
+                   ["pairsum_slices", "pairsum_mfma"])
+    H = hist.view(ncols, nh).cpu().numpy().view(np.uint64)
+    Svh = Sv.view(ncols, nq).cpu().numpy()
+    Smh = Sm.view(ncols, nq).cpu().numpy()
+    # FP64 reference on the first 16 particles (48 columns)
+    cols = 48
+    r = (np.arange(nh) + 0.5) * bin_
+    qr = np.outer(q, r)
+    A = np.sin(qr) / (qr + 1e-8)
+    del qr
+    Se = (A @ H[:cols].astype(np.float64).T).T
+    del A
+
+    def s_err(S):
+        return float((np.abs(S[:cols] - Se).max(axis=1) / np.abs(Se).max(axis=1)).max())
+
+    def i_err(S):
+        worst = 0.0
+        for p_i in range(cols // 3):
+            ec = parts[p_i].el_count
+            sl = (3.63, 9.577)
+            n = sum(ec)
+
+            def inten(SS):
+                return (ec[0] * sl[0] ** 2 + ec[1] * sl[1] ** 2
+                        + 2 * (sl[0] * sl[0] * SS[0] + sl[0] * sl[1] * SS[1] + sl[1] * sl[1] * SS[2])) / n
+            Ie = inten(Se[3 * p_i:3 * p_i + 3])
+            Ig = inten(S[3 * p_i:3 * p_i + 3])
+            m = np.abs(Ie) >= 1e-3 * np.abs(Ie).max()
+            worst = max(worst, float((np.abs(Ig - Ie)[m] / np.abs(Ie[m])).max()))
+        return worst
+
+    cmax = int(H.max())
+    nb = 1
+    while cmax >> (7 * nb):
+        nb += 1
+    flop = 2.0 * nq * nh * ncols
+    mm_ms = km["pairsum_mfma"]
+    int8_ops = 5 * nb * flop
+    return {
+        "workload": f"256 two-element particles (fcc / core-shell spheres, 531-7,803 atoms), "
+                    f"{ncols} histograms x {nh} bins (0.004 A), {nq} q 0.2-25 A^-1",
+        "columns": ncols, "nhist": nh, "nq": nq, "max_count": cmax, "count_slices": nb,
+        "overflow": ovf,
+        "ms_fp64_vector": round(tv * 1e3, 3), "ms_mfma": round(tm * 1e3, 3),
+        "kernel_ms": {"hist_pairsum": round(kv["hist_pairsum"], 3),
+                      "pairsum_slices": round(km["pairsum_slices"], 3),
+                      "pairsum_mfma": round(mm_ms, 3)},
+        "speedup": round(tv / tm, 2),
+        "fp64_equiv_tflops_mfma": round(flop / tm / 1e12, 2),
+        "roofline_mfma": {"bound": "mfma", "achieved": round(int8_ops / (mm_ms * 1e-3) / 1e12, 1),
+                          "peak": 5000.0, "unit": "TOP/s (int8)",
+                          "frac": round(int8_ops / (mm_ms * 1e-3) / 1e12 / 5000.0, 4),
+                          "int8_products": 5 * nb},
+        "max_rel_err_S_vs_fp64": {"fp64_vector": s_err(Svh), "mfma": s_err(Smh)},
+        "max_rel_err_I_vs_fp64": {"fp64_vector": i_err(Svh), "mfma": i_err(Smh)},
+        "error_sample": f"first {cols // 3} particles ({cols} histograms), all {nq} q; "
+                        "S: max |dS| / max |S| per histogram; I: pointwise where I >= 1e-3 max I",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble", type=int, default=1000,
+                    help="particles in the batched-ensemble leg (0 skips it)")
+    ap.add_argument("--no-matrix", action="store_true",
+                    help="skip the many-histogram pair-sum leg (FP64 vector vs int8 matrix cores)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported: "
+              "this backend has no multi-GPU path.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
 
     import torch
     import xansons_hip as xh
@@ -187,6 +427,12 @@ def main():
                         "hist_kernel_ms": round(kern32["hist"][0] / max(kern32["hist"][1], 1), 3)},
         "checks": {"pair_totals_exact": totals_ok, "overflow": int(ovf.item())},
     }
+    if not args.no_matrix:
+        line["pairsum_matrix"] = pairsum_matrix_leg(ctx, max(1, args.steps))
+    if args.ensemble > 0:
+        del hist, ovf
+        torch.cuda.empty_cache()
+        line["ensemble"] = ensemble_leg(ctx, args.ensemble, max(1, args.steps // 2), 1)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(ens, nh)
     ctx.close()

@@ -45,6 +45,8 @@ extern "C" {
 #define XS_DEVICE_PTRS 0x1u     /* array arguments are device pointers */
 #define XS_F64_BINNING 0x2u     /* xs_hist: xyz is double, bin as the CPU path (double) */
 #define XS_PDF_CPU_RADIUS 0x4u  /* xs_pdf: radius offset of the CPU path for p >= 1 */
+#define XS_BATCH_GRAPH 0x8u     /* xs_batch_run: capture once, replay as a hipGraph */
+#define XS_PAIRSUM_MFMA 0x10u   /* xs_hist_pairsum: int8 matrix-core product */
 
 /* PDF types, typedefs.h:54-56 */
 #define XS_PDF_RDF 0
@@ -52,6 +54,7 @@ extern "C" {
 #define XS_PDF_RPDF 2
 
 typedef struct xs_ctx xs_ctx;
+typedef struct xs_batch xs_batch;
 
 /* Open a context on a device and create its stream.
  * Replaces SetDeviceCuda (CalcFunctionsCUDA.cu:1485-1519), which falls back
@@ -105,7 +108,13 @@ int xs_hist(xs_ctx *ctx, const void *xyz, size_t n_atoms,
  * r_k = (k+1/2) bin, empty bins skipped, S = 0 where q = 0.  FP64 terms,
  * sin advanced from bin to bin by a rotation, FP64 atomics across blocks.
  * Replaces the bin loop of calcInt1DHist (CalcFunctions.cpp:844-871) and
- * calcIntHistKernelXray/Neutron (CalcFunctionsCUDA.cu:923-985). */
+ * calcIntHistKernelXray/Neutron (CalcFunctionsCUDA.cu:923-985).
+ * XS_PAIRSUM_MFMA evaluates the same sums as one product S = H A^T on the
+ * int8 matrix cores: A (the sinc table) split into five int8 slices, the
+ * counts into exact 7-bit slices, int32 accumulation; the result differs
+ * from the FP64 sum by about 2^-35 sum_k h[k] (needs counts < 2^35 and a
+ * workspace of ~5 nq nhist bytes).  It pays off for many histograms (pair
+ * types of many structures) on one bin grid. */
 int xs_hist_pairsum(xs_ctx *ctx, const uint64_t *hist, unsigned ntypes,
                     unsigned nhist, double bin, const double *q, unsigned nq,
                     double *S, unsigned flags);
@@ -150,6 +159,51 @@ int xs_amp2d(xs_ctx *ctx, const double *xyz, size_t n_atoms,
              const unsigned *el_count, unsigned n_el, const double *qvec,
              size_t nqp, const double *rot, unsigned nrot, const double *ff,
              double *I, unsigned flags);
+
+/* Ensembles.  The reference computes one structure per run of main
+ * (main.cpp:151-233: calcPDFandDebye per configuration); an ensemble of
+ * particles is a sequence of such runs.  A batch plan holds every structure
+ * on the device so that one xs_batch_run computes all of them.
+ *
+ * xs_batch_plan: structure s has n_el[s] elements; its el_count entries
+ * follow those of structure s-1 in el_count; xyz[s] is its element-major
+ * double SoA (3 * N_s values, host memory).  Coordinates are binned in double
+ * with XS_F64_BINNING (calcHist, CalcFunctions.cpp:784-788), else rounded to
+ * float and binned as calcHistKernel (CalcFunctionsCUDA.cu:832-833).
+ * nhist[s] = 0 (or nhist == NULL) takes GetHistSize's bin count
+ * (CalcFunctions.cpp:224-243).  Each structure needs at least one atom. */
+int xs_batch_plan(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el,
+                  const unsigned *el_count, const double *const *xyz,
+                  double bin, const unsigned *nhist, unsigned flags,
+                  xs_batch **plan);
+
+/* Layout of a plan's outputs.  Structure s: histograms at hist_off[s]
+ * (ntypes_s * nhist[s] bins), PDFs at pdf_off[s] ((1+ntypes_s) * nhist[s]
+ * values, total first), pair sums at rows type_first[s] .. of S.
+ * totals = {hist_len, pdf_len, ntypes_total}.  Any output may be NULL. */
+int xs_batch_info(const xs_batch *plan, unsigned *nhist,
+                  unsigned long long *hist_off, unsigned long long *pdf_off,
+                  unsigned long long *type_first, unsigned long long *totals);
+
+/* Histograms of every structure (as xs_hist), then, if S != NULL and nq > 0,
+ * pair sums S[t*nq+iq] of every pair type t (as xs_hist_pairsum, one q mesh
+ * for all), then, if pdf != NULL, PDFs of pdf_type (as xs_pdf) with rho0[s]
+ * and sl[] in el_count order.  XS_DEVICE_PTRS: hist, q, S and pdf are device
+ * pointers and the call returns without waiting; else host arrays and the
+ * call is synchronous.  XS_BATCH_GRAPH: the launches are captured into a
+ * hipGraph on the first call and replayed by later calls with the same
+ * pointers, nq, pdf_type and flags (needs a non-default stream).
+ * Overlapping runs of one plan race; use one plan per stream. */
+int xs_batch_run(xs_batch *plan, uint64_t *hist, const double *q, unsigned nq,
+                 double *S, int pdf_type, const double *rho0, const double *sl,
+                 double *pdf, unsigned flags);
+
+/* Per-structure overflow counts of the last run (host array, synchronous). */
+int xs_batch_overflow(xs_batch *plan, uint64_t *overflow);
+
+/* Free a plan.  xs_free frees the plans of its context that are still
+ * alive; a plan must not be used or freed after its context is freed. */
+int xs_batch_free(xs_batch *plan);
 
 #ifdef __cplusplus
 }

@@ -20,20 +20,34 @@ int activate(xs_ctx *ctx) {
   return XS_OK;
 }
 
-int ensure_workspace(xs_ctx *ctx, size_t bytes) {
-  if (bytes <= ctx->ws_size) return XS_OK;
-  size_t want = align_up(bytes + bytes / 4, size_t(1) << 20);
+// Grow a device buffer (after draining the stream that may still read it).
+static int ensure_device(xs_ctx *ctx, void **buf, size_t *cap, size_t bytes, size_t gran,
+                         const char *what) {
+  if (bytes <= *cap) return XS_OK;
+  size_t want = align_up(bytes + bytes / 4, gran);
   XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  if (ctx->ws) XS_HIP_CHECK(hipFree(ctx->ws));
-  ctx->ws = nullptr;
-  ctx->ws_size = 0;
-  if (hipMalloc(&ctx->ws, want) != hipSuccess) {
-    ctx->ws = nullptr;
+  if (*buf) XS_HIP_CHECK(hipFree(*buf));
+  *buf = nullptr;
+  *cap = 0;
+  if (hipMalloc(buf, want) != hipSuccess) {
+    *buf = nullptr;
     (void)hipGetLastError();
-    return fail(XS_ERR_NOMEM, "hipMalloc of " + std::to_string(want) + " workspace bytes failed");
+    return fail(XS_ERR_NOMEM, "hipMalloc of " + std::to_string(want) + " " + what + " bytes failed");
   }
-  ctx->ws_size = want;
+  *cap = want;
   return XS_OK;
+}
+
+int ensure_workspace(xs_ctx *ctx, size_t bytes) {
+  return ensure_device(ctx, &ctx->ws, &ctx->ws_size, bytes, size_t(1) << 20, "workspace");
+}
+
+int ensure_aux(xs_ctx *ctx, size_t bytes) {
+  return ensure_device(ctx, &ctx->aux, &ctx->aux_size, bytes, size_t(1) << 16, "table");
+}
+
+int ensure_matrix(xs_ctx *ctx, size_t bytes) {
+  return ensure_device(ctx, &ctx->mat, &ctx->mat_size, bytes, size_t(1) << 20, "matrix");
 }
 
 int ensure_pinned(xs_ctx *ctx, size_t bytes) {
@@ -147,13 +161,15 @@ int xs_free(xs_ctx *ctx) {
   if (!ctx) return XS_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  while (!ctx->plans.empty()) destroy_batch(*ctx->plans.begin());
   for (auto &r : ctx->prof_pending) {
     (void)hipEventDestroy(r.start);
     (void)hipEventDestroy(r.stop);
   }
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
   if (ctx->ws) (void)hipFree(ctx->ws);
-  if (ctx->hist_items_dev) (void)hipFree(ctx->hist_items_dev);
+  if (ctx->aux) (void)hipFree(ctx->aux);
+  if (ctx->mat) (void)hipFree(ctx->mat);
   if (ctx->pin) (void)hipHostFree(ctx->pin);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;

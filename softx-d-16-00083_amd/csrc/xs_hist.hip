@@ -1,56 +1,71 @@
-// H1: pair-distance histogram on gfx950.
+// H1: pair-distance histograms on gfx950, for one structure or an ensemble.
 //
-// Pipeline on the context stream:
-//   1. bounding box of all atoms (hist_bbox_partial, hist_grid_setup), read back
-//   2. a uniform grid of cubic cells sized for ~800 atoms of the most numerous
-//      element (and small enough that two cell diagonals fit the LDS window);
-//      per-element counting sort by cell (hist_cell_count, read back, host
-//      prefix sums, hist_scatter).  A tile is one cell, or a 1024-atom chunk of
-//      a fuller cell, so its atoms lie in a cube.  Counts are order
-//      independent: the permutation does not change the histogram.
-//   3. per-tile bounding boxes (hist_tile_boxes)
-//   4. histogram kernel, one work item per (I tile, run of J tiles):
-//        - the I tile's 1024 atoms sit one per thread in registers
-//        - the J tiles are sorted by the lowest bin their pairs with the I tile
-//          can reach (from the two boxes), and swept in that order
-//        - counts go to a circular LDS window of WCAP uint32 bins that slides
-//          up the histogram; bins that the remaining J tiles cannot reach are
-//          flushed to the global uint64 histogram with one atomic per non-zero
-//          bin, so flush traffic is O(nhist) per item, not O(pairs)
-//        - a pair whose bin is outside the window (never, unless a tile's
-//          distance span exceeds WCAP bins) takes a global atomic directly.
+// Tiling.  Each structure's atoms are counting-sorted by the cell of a uniform
+// grid of cubic cells sized for ~kFill atoms of its most numerous element (and
+// no wider than two cell diagonals fit the LDS window).  A tile is one cell,
+// or a 1024-atom chunk of a fuller cell, so its atoms lie in a cube; a tile
+// carries its bounding box.  Counts do not depend on atom order, so the
+// permutation leaves the histogram unchanged.
 //
-// The bin expression is the reference's (unsigned)(r / bin) with
-// r = sqrt((dx*dx + dy*dy) + dz*dz): CalcFunctions.cpp:784-788 (double) and
-// CalcFunctionsCUDA.cu:832-833 (float).  It is evaluated with correctly
-// rounded sqrt and division and without contraction, so the counts are the
-// oracle's bit for bit.
+// Kernel.  One work item = (pair type of a structure, I tile, run of J tiles).
+// The I tile's atoms sit one per thread in registers; J tiles are sorted by
+// the lowest bin their pairs with the I tile can reach (from the two boxes)
+// and swept in that order; J atoms are staged 256 at a time in LDS and read
+// by every lane at the same address.  Counts go to a circular LDS window of
+// WCAP uint32 bins that only slides up; bins the remaining J tiles cannot
+// reach are flushed to the uint64 histogram with one atomic per non-zero bin,
+// so flush traffic per item is at most nhist + WCAP whatever the pair count.
+// A tile pair whose span exceeds the window takes global atomics per pair.
+//
+// Binning.  The reference's (unsigned)(r / bin), r = sqrt((dx*dx+dy*dy)+dz*dz):
+//   double (CalcFunctions.cpp:784-788, the CPU path) with a fast exact path,
+//   float  (CalcFunctionsCUDA.cu:832-833, the GPU path) in IEEE single precision.
+// Counts equal the oracle's bit for bit in both modes.
+//
+// Pipelines.
+//   xs_hist, device input: bbox and per-cell counts are computed on the GPU and
+//     read back (two small synchronisations) to build tiles and work items.
+//   xs_hist, host input, and xs_batch_plan: the sort, tiles, boxes and work
+//     items are built on the host and uploaded.
+//   xs_batch_run (xs_batch.hip): zero fills, histograms, pair sums and PDFs
+//     are launches only, optionally captured once into a hipGraph.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <numeric>
+#include <vector>
 
 #include "xs_internal.h"
 
 namespace xs {
 namespace hist {
 
-constexpr int kTile = 1024;     // atoms per tile == threads per work group
-constexpr int kStage = 256;     // J atoms staged in LDS per step
-constexpr int kSortCap = 2048;  // J tiles sorted per work item
+constexpr int kTile = 1024;      // atoms per tile == threads per work group
+constexpr int kStage = 256;      // J atoms staged in LDS per step
+constexpr int kSortCap = 2048;   // J tiles sorted per work item
 constexpr int kMaxEl = 16;
-constexpr int kMaxTypes = kMaxEl * (kMaxEl + 1) / 2;
-constexpr int kBoxBlocks = 512; // partial bounding boxes
-
-struct Item {
-  uint32_t p, ti, jstart, jcount;
-};
-
-struct Geom {
-  uint32_t off[kMaxEl], cnt[kMaxEl], tile_base[kMaxEl], ntiles[kMaxEl];
-};
+constexpr int kBoxBlocks = 512;  // partial bounding boxes
+constexpr double kFill = 800.0;  // target atoms per cell of the largest element
+constexpr uint32_t kMaxCellsPerAxis = 256;
 
 struct Tile {
-  uint32_t off, cnt;  // global atom range in sorted order
+  uint32_t off, cnt;  // atom range in the sorted arrays
+};
+
+// One pair type of one structure.
+struct TypeDesc {
+  unsigned long long hist_off;  // offset of its nhist bins in the output
+  uint32_t nhist;
+  uint32_t a_base, a_n;         // I tiles: global tile index base and count
+  uint32_t b_base, b_n;         // J tiles
+  uint32_t same;                // like-element pair type: j > i within a tile
+  uint32_t ovf;                 // overflow counter index
+  uint32_t pad;
+  double bin, inv_bin, band;
+};
+
+struct Item {
+  uint32_t type, ti, jstart, jcount;
 };
 
 struct Grid {
@@ -61,20 +76,16 @@ struct Grid {
 
 template <class T>
 struct Args {
-  const T *x, *y, *z;           // sorted coordinates
+  const T *x, *y, *z;
   const Tile *tiles;
-  const double *boxes;          // 6 doubles per tile: lo xyz, hi xyz
+  const double *boxes;  // 6 per tile: lo xyz, hi xyz
+  const TypeDesc *types;
   const Item *items;
-  unsigned long long *hist;     // ntypes * nhist
+  unsigned long long *hist;
   unsigned long long *overflow;
-  T bin;
-  double bin_d, inv_bin_d, band;
-  uint32_t nhist;
-  Geom g;
-  uint8_t ta[kMaxTypes], tb[kMaxTypes];
 };
 
-// ---------------------------------------------------------------- sorting
+// ---------------------------------------------------------------- prep kernels
 
 template <class T>
 __global__ void hist_bbox_partial(const T *__restrict__ x, const T *__restrict__ y,
@@ -82,7 +93,7 @@ __global__ void hist_bbox_partial(const T *__restrict__ x, const T *__restrict__
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
-    double v[3] = {(double)x[i], (double)y[i], (double)z[i]};
+    const double v[3] = {(double)x[i], (double)y[i], (double)z[i]};
     for (int a = 0; a < 3; ++a) {
       lo[a] = fmin(lo[a], v[a]);
       hi[a] = fmax(hi[a], v[a]);
@@ -105,7 +116,6 @@ __global__ void hist_bbox_partial(const T *__restrict__ x, const T *__restrict__
   if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-// bbox[0..2] = lo, bbox[3..5] = hi of all atoms
 __global__ void hist_bbox_reduce(const double *__restrict__ part, int nparts,
                                  double *__restrict__ bbox) {
   __shared__ double s[6][256];
@@ -128,9 +138,13 @@ __global__ void hist_bbox_reduce(const double *__restrict__ part, int nparts,
   if (threadIdx.x < 6) bbox[threadIdx.x] = s[threadIdx.x][0];
 }
 
+struct ElemRanges {
+  uint32_t off[kMaxEl], cnt[kMaxEl];
+};
+
 template <class T>
 __global__ void hist_cell_count(const T *__restrict__ x, const T *__restrict__ y,
-                                const T *__restrict__ z, Geom g, Grid grid,
+                                const T *__restrict__ z, ElemRanges g, Grid grid,
                                 uint32_t *__restrict__ keys, uint32_t *__restrict__ count) {
   const int e = blockIdx.y;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -151,7 +165,7 @@ __global__ void hist_cell_count(const T *__restrict__ x, const T *__restrict__ y
 
 template <class T>
 __global__ void hist_scatter(const T *__restrict__ x, const T *__restrict__ y,
-                             const T *__restrict__ z, Geom g, uint32_t m,
+                             const T *__restrict__ z, ElemRanges g, uint32_t m,
                              const uint32_t *__restrict__ keys, uint32_t *__restrict__ start,
                              T *__restrict__ sx, T *__restrict__ sy, T *__restrict__ sz) {
   const int e = blockIdx.y;
@@ -168,12 +182,11 @@ template <class T>
 __global__ void hist_tile_boxes(const T *__restrict__ x, const T *__restrict__ y,
                                 const T *__restrict__ z, const Tile *__restrict__ tiles,
                                 double *__restrict__ boxes) {
-  const uint32_t tg = blockIdx.x;
-  const Tile t = tiles[tg];
+  const Tile t = tiles[blockIdx.x];
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (uint32_t i = threadIdx.x; i < t.cnt; i += blockDim.x) {
-    size_t gi = (size_t)t.off + i;
-    double v[3] = {(double)x[gi], (double)y[gi], (double)z[gi]};
+    const size_t gi = (size_t)t.off + i;
+    const double v[3] = {(double)x[gi], (double)y[gi], (double)z[gi]};
     for (int a = 0; a < 3; ++a) {
       lo[a] = fmin(lo[a], v[a]);
       hi[a] = fmax(hi[a], v[a]);
@@ -193,24 +206,24 @@ __global__ void hist_tile_boxes(const T *__restrict__ x, const T *__restrict__ y
       }
     __syncthreads();
   }
-  if (threadIdx.x < 6) boxes[(size_t)tg * 6 + threadIdx.x] = s[threadIdx.x][0];
+  if (threadIdx.x < 6) boxes[(size_t)blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ void fill_u64(unsigned long long *__restrict__ p, size_t n, unsigned long long v) {
+__global__ void fill_u64(unsigned long long *__restrict__ p, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
-    p[i] = v;
+    p[i] = 0ull;
 }
 
-// ---------------------------------------------------------------- histogram
+// ---------------------------------------------------------------- histogram kernel
 
 // Conservative bin range [kl, ku] of all pairs between two boxes.
 __device__ __forceinline__ void box_bins(const double *__restrict__ bi, const double *__restrict__ bj,
                                          double bin, uint32_t nhist, uint32_t &kl, uint32_t &ku) {
   double lo2 = 0.0, hi2 = 0.0;
   for (int a = 0; a < 3; ++a) {
-    double gap = fmax(fmax(bj[a] - bi[3 + a], bi[a] - bj[3 + a]), 0.0);
-    double ext = fmax(bj[3 + a] - bi[a], bi[3 + a] - bj[a]);
+    const double gap = fmax(fmax(bj[a] - bi[3 + a], bi[a] - bj[3 + a]), 0.0);
+    const double ext = fmax(bj[3 + a] - bi[a], bi[3 + a] - bj[a]);
     lo2 += gap * gap;
     hi2 += ext * ext;
   }
@@ -245,29 +258,30 @@ __device__ __forceinline__ void flush_window(uint32_t *win, uint32_t lo, uint32_
 //   than nhist*2^-30 away from an integer, its floor is the IEEE result.
 //   Otherwise -- about nhist*2^-29 of the pairs, and every NaN -- the IEEE
 //   sqrt and division are evaluated.  `band` = 1/2 - nhist*2^-30 - 2^-40.
+//   The estimate stays below 2^31: the host rejects structures whose
+//   bounding-box diagonal spans 2^31 bins.
 // float: the IEEE single-precision expression of the reference GPU kernel,
 //   evaluated directly (correctly rounded sqrtf and division).
 template <class T>
-__device__ __forceinline__ uint32_t bin_index(T s, T bin, double inv_bin_d, double band,
+__device__ __forceinline__ uint32_t bin_index(T s, T bin, double inv_bin, double band,
                                               uint32_t nhist);
 
 template <>
-__device__ __forceinline__ uint32_t bin_index<double>(double s, double bin, double inv_bin_d,
+__device__ __forceinline__ uint32_t bin_index<double>(double s, double bin, double inv_bin,
                                                       double band, uint32_t nhist) {
   const double y = __builtin_amdgcn_rsq(s);
   double g = s * y;
   g = fma(g, fma(-g, 0.5 * y, 0.5), g);
-  const double kd = g * inv_bin_d;
-  // kd < 2^31 here: the host rejects boxes whose diagonal exceeds 2^31 bins
+  const double kd = g * inv_bin;
   if (__builtin_expect(fabs(__builtin_amdgcn_fract(kd) - 0.5) < band, 1)) return (uint32_t)kd;
   const double t = __builtin_sqrt(s) / bin;
   return t < (double)nhist ? (uint32_t)t : nhist;
 }
 
 template <>
-__device__ __forceinline__ uint32_t bin_index<float>(float s, float bin, double inv_bin_d,
+__device__ __forceinline__ uint32_t bin_index<float>(float s, float bin, double inv_bin,
                                                      double band, uint32_t nhist) {
-  (void)inv_bin_d;
+  (void)inv_bin;
   (void)band;
   const float t = __builtin_sqrtf(s) / bin;
   return t < (float)nhist ? (uint32_t)t : nhist;
@@ -276,8 +290,8 @@ __device__ __forceinline__ uint32_t bin_index<float>(float s, float bin, double 
 template <class T, int WCAP, bool DIAG>
 __device__ __forceinline__ void pair_block(T xi, T yi, T zi, uint32_t il, const T *sx,
                                            const T *sy, const T *sz, uint32_t cn,
-                                           uint32_t jbase, T bin, double inv_bin_d,
-                                           double band, uint32_t nhist, uint32_t B, uint32_t wlim,
+                                           uint32_t jbase, T bin, double inv_bin, double band,
+                                           uint32_t nhist, uint32_t B, uint32_t wlim,
                                            uint32_t *win, unsigned long long *__restrict__ hp,
                                            unsigned long long *__restrict__ ovf) {
 #pragma clang fp contract(off)
@@ -289,7 +303,7 @@ __device__ __forceinline__ void pair_block(T xi, T yi, T zi, uint32_t il, const 
     const T dz = zi - sz[j];
     T s = dx * dx + dy * dy;
     s = s + dz * dz;
-    const uint32_t k = bin_index<T>(s, bin, inv_bin_d, band, nhist);
+    const uint32_t k = bin_index<T>(s, bin, inv_bin, band, nhist);
     if (__builtin_expect(k - B < wlim, 1)) {
       atomicAdd(&win[k & (WCAP - 1)], 1u);
     } else if (k < nhist) {
@@ -307,10 +321,9 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
   __shared__ T sx[kStage], sy[kStage], sz[kStage];
 
   const Item it = A.items[blockIdx.x];
-  const uint32_t a = A.ta[it.p], b = A.tb[it.p];
-  const bool same = a == b;
+  const TypeDesc td = A.types[it.type];
   const uint32_t tid = threadIdx.x;
-  const uint32_t tiI = A.g.tile_base[a] + it.ti;
+  const uint32_t tiI = td.a_base + it.ti;
   const Tile I = A.tiles[tiI];
   const bool ivalid = tid < I.cnt;
   T xi = 0, yi = 0, zi = 0;
@@ -320,19 +333,21 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
     yi = A.y[gi];
     zi = A.z[gi];
   }
+  const T bin = (T)td.bin;
+  const uint32_t nhist = td.nhist;
   const double *bi = A.boxes + 6 * (size_t)tiI;
-  const uint32_t ntb = A.g.ntiles[b];
   const uint32_t n = it.jcount;
   uint32_t npow = 1;
   while (npow < n) npow <<= 1;
-  unsigned long long *hp = A.hist + (size_t)it.p * A.nhist;
+  unsigned long long *hp = A.hist + td.hist_off;
+  unsigned long long *ovf = A.overflow + td.ovf;
 
   for (uint32_t m = tid; m < npow; m += kTile) {
     unsigned long long key = ~0ull;
     if (m < n) {
-      const uint32_t tj = (it.jstart + m) % ntb;
+      const uint32_t tj = (it.jstart + m) % td.b_n;
       uint32_t kl, ku;
-      box_bins(bi, A.boxes + 6 * (size_t)(A.g.tile_base[b] + tj), A.bin_d, A.nhist, kl, ku);
+      box_bins(bi, A.boxes + 6 * (size_t)(td.b_base + tj), td.bin, nhist, kl, ku);
       key = ((unsigned long long)kl << 32) | m;
     }
     keys[m] = key;
@@ -359,22 +374,22 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
   uint32_t B = (uint32_t)(keys[0] >> 32);
   for (uint32_t s = 0; s < n; ++s) {
     const uint32_t m = (uint32_t)(keys[s] & 0xffffffffu);
-    const uint32_t tj = (it.jstart + m) % ntb;
-    const uint32_t tjg = A.g.tile_base[b] + tj;
+    const uint32_t tj = (it.jstart + m) % td.b_n;
+    const uint32_t tjg = td.b_base + tj;
     uint32_t kl, ku;
-    box_bins(bi, A.boxes + 6 * (size_t)tjg, A.bin_d, A.nhist, kl, ku);
+    box_bins(bi, A.boxes + 6 * (size_t)tjg, td.bin, nhist, kl, ku);
     uint32_t wlim = 0;
     if (ku - kl + 1 <= (uint32_t)WCAP) {
       const uint32_t nb = ku + 1 > (uint32_t)WCAP ? ku + 1 - WCAP : 0u;
       if (nb > B) {
-        flush_window<WCAP>(win, B, min(min(nb, B + (uint32_t)WCAP), A.nhist), hp);
+        flush_window<WCAP>(win, B, min(min(nb, B + (uint32_t)WCAP), nhist), hp);
         B = nb;
         __syncthreads();
       }
-      wlim = min((uint32_t)WCAP, A.nhist - B);
+      wlim = min((uint32_t)WCAP, nhist - B);
     }
     const Tile J = A.tiles[tjg];
-    const bool diag = same && tj == it.ti;
+    const bool diag = td.same && tj == it.ti;
     for (uint32_t s0 = 0; s0 < J.cnt; s0 += kStage) {
       const uint32_t cn = min((uint32_t)kStage, J.cnt - s0);
       if (tid < cn) {
@@ -385,67 +400,19 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
       __syncthreads();
       if (ivalid) {
         if (diag)
-          pair_block<T, WCAP, true>(xi, yi, zi, tid, sx, sy, sz, cn, s0, A.bin, A.inv_bin_d,
-                                    A.band, A.nhist, B, wlim, win, hp, A.overflow);
+          pair_block<T, WCAP, true>(xi, yi, zi, tid, sx, sy, sz, cn, s0, bin, td.inv_bin, td.band,
+                                    nhist, B, wlim, win, hp, ovf);
         else
-          pair_block<T, WCAP, false>(xi, yi, zi, tid, sx, sy, sz, cn, s0, A.bin, A.inv_bin_d,
-                                     A.band, A.nhist, B, wlim, win, hp, A.overflow);
+          pair_block<T, WCAP, false>(xi, yi, zi, tid, sx, sy, sz, cn, s0, bin, td.inv_bin, td.band,
+                                     nhist, B, wlim, win, hp, ovf);
       }
       __syncthreads();
     }
   }
-  flush_window<WCAP>(win, B, min(B + (uint32_t)WCAP, A.nhist), hp);
+  flush_window<WCAP>(win, B, min(B + (uint32_t)WCAP, nhist), hp);
 }
 
-// ---------------------------------------------------------------- host side
-
-// Work items over the element tile lists.  Like-element pairs: tile ti pairs
-// with itself and the next h tiles cyclically (h = (n-1)/2, or n/2 for the
-// first half when n is even), so every unordered tile pair is visited once and
-// items have equal length.  Cross-element pairs: each I tile against the J
-// tiles in two halves.  Longest items first.
-static void build_items(const Geom &g, unsigned n_el, std::vector<Item> &items) {
-  items.clear();
-  unsigned p = 0;
-  for (unsigned a = 0; a < n_el; ++a) {
-    for (unsigned b = a; b < n_el; ++b, ++p) {
-      const uint32_t nta = g.ntiles[a], ntb = g.ntiles[b];
-      if (!nta || !ntb) continue;
-      if (a == b) {
-        for (uint32_t ti = 0; ti < nta; ++ti) {
-          uint32_t h = (nta & 1) ? (nta - 1) / 2 : (ti < nta / 2 ? nta / 2 : nta / 2 - 1);
-          uint32_t total = h + 1;
-          for (uint32_t c = 0; c < total; c += kSortCap)
-            items.push_back(Item{p, ti, (ti + c) % ntb, std::min<uint32_t>(kSortCap, total - c)});
-        }
-      } else {
-        const uint32_t parts = ntb >= 2 ? 2 : 1;
-        const uint32_t per = std::min<uint32_t>((ntb + parts - 1) / parts, kSortCap);
-        for (uint32_t ti = 0; ti < nta; ++ti)
-          for (uint32_t j0 = 0; j0 < ntb; j0 += per)
-            items.push_back(Item{p, ti, j0, std::min(per, ntb - j0)});
-      }
-    }
-  }
-  std::stable_sort(items.begin(), items.end(),
-                   [](const Item &u, const Item &v) { return u.jcount > v.jcount; });
-}
-
-template <class T, int WCAP>
-static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems) {
-  hipEvent_t ev;
-  XS_TRY(prof_begin(ctx, "hist", &ev));
-  hipLaunchKernelGGL((hist_kernel<T, WCAP>), dim3((unsigned)nitems), dim3(kTile), 0, ctx->stream, A);
-  XS_HIP_CHECK(hipGetLastError());
-  XS_TRY(prof_end(ctx, "hist", ev));
-  return XS_OK;
-}
-
-// Cell edge: cubes holding about kFill atoms of the most numerous element at
-// its mean density over the bounding box, no wider than two diagonals of the
-// window allow when the histogram is longer than the window.
-constexpr double kFill = 800.0;
-constexpr uint32_t kMaxCellsPerAxis = 256;
+// ---------------------------------------------------------------- host planning
 
 static Grid make_grid(const double bbox[6], unsigned max_count, double bin, uint32_t nhist,
                       uint32_t wcap) {
@@ -455,9 +422,8 @@ static Grid make_grid(const double bbox[6], unsigned max_count, double bin, uint
     gr.lo[a] = bbox[a];
     ext[a] = std::max(bbox[3 + a] - bbox[a], 0.0);
   }
-  // volume of the box, with degenerate axes widened to the typical spacing
-  double span = std::max({ext[0], ext[1], ext[2], 1e-6});
-  double spacing = span / std::max(1.0, std::cbrt((double)std::max(max_count, 1u)));
+  const double span = std::max({ext[0], ext[1], ext[2], 1e-6});
+  const double spacing = span / std::max(1.0, std::cbrt((double)std::max(max_count, 1u)));
   for (int a = 0; a < 3; ++a) vol *= std::max(ext[a], spacing);
   double cell = std::cbrt(kFill * vol / std::max(max_count, 1u));
   if (nhist > wcap) cell = std::min(cell, (wcap - 8) * bin / (2.0 * std::sqrt(3.0)));
@@ -465,204 +431,526 @@ static Grid make_grid(const double bbox[6], unsigned max_count, double bin, uint
   cell = std::max(cell, 1e-9);
   gr.inv_cell = 1.0 / cell;
   for (int a = 0; a < 3; ++a)
-    gr.g[a] = (uint32_t)std::min<double>(kMaxCellsPerAxis, std::max(1.0, std::ceil(ext[a] / cell + 1e-9)));
+    gr.g[a] = (uint32_t)std::min<double>(kMaxCellsPerAxis,
+                                         std::max(1.0, std::ceil(ext[a] / cell + 1e-9)));
   return gr;
 }
 
+static uint32_t wcap_for(uint32_t nhist) { return nhist <= 8192 ? 8192u : 32768u; }
+
+static double band_for(uint32_t nhist) {
+  return 0.5 - std::ldexp((double)nhist, -30) - std::ldexp(1.0, -40);
+}
+
+static int check_span(const double bbox[6], double bin) {
+  for (int a = 0; a < 6; ++a)
+    if (!std::isfinite(bbox[a])) return fail(XS_ERR_ARG, "histogram: non-finite coordinates");
+  double d2 = 0;
+  for (int a = 0; a < 3; ++a) d2 += (bbox[3 + a] - bbox[a]) * (bbox[3 + a] - bbox[a]);
+  if (std::sqrt(d2) / bin >= 2147483648.0)
+    return fail(XS_ERR_RANGE, "histogram: the structure spans more than 2^31 bins");
+  return XS_OK;
+}
+
+// Work items of pair types [first, last) (appended).  Like-element pairs:
+// tile ti with itself and the next h tiles cyclically (h = (n-1)/2, or n/2 for
+// the first half when n is even), so each unordered tile pair is visited once
+// and items have equal length.  Cross-element pairs: each I tile against the J
+// tiles in two halves.
+static void add_items(const std::vector<TypeDesc> &types, uint32_t first, uint32_t last,
+                      std::vector<Item> &items) {
+  for (uint32_t t = first; t < last; ++t) {
+    const TypeDesc &d = types[t];
+    if (!d.a_n || !d.b_n) continue;
+    if (d.same) {
+      const uint32_t nt = d.a_n;
+      for (uint32_t ti = 0; ti < nt; ++ti) {
+        const uint32_t h = (nt & 1) ? (nt - 1) / 2 : (ti < nt / 2 ? nt / 2 : nt / 2 - 1);
+        const uint32_t total = h + 1;
+        for (uint32_t c = 0; c < total; c += kSortCap)
+          items.push_back(Item{t, ti, (ti + c) % nt, std::min<uint32_t>(kSortCap, total - c)});
+      }
+    } else {
+      const uint32_t parts = d.b_n >= 2 ? 2 : 1;
+      const uint32_t per = std::min<uint32_t>((d.b_n + parts - 1) / parts, kSortCap);
+      for (uint32_t ti = 0; ti < d.a_n; ++ti)
+        for (uint32_t j0 = 0; j0 < d.b_n; j0 += per)
+          items.push_back(Item{t, ti, j0, std::min(per, d.b_n - j0)});
+    }
+  }
+}
+
+// Longest first (estimated pairs: I tile atoms x atoms of its J run).
+static void order_items(std::vector<Item> &items, const std::vector<TypeDesc> &types,
+                        const std::vector<Tile> &tiles) {
+  std::vector<std::pair<double, uint32_t>> cost(items.size());
+  for (size_t i = 0; i < items.size(); ++i) {
+    const Item &it = items[i];
+    const TypeDesc &d = types[it.type];
+    double jn = 0;
+    for (uint32_t m = 0; m < it.jcount; ++m) jn += tiles[d.b_base + (it.jstart + m) % d.b_n].cnt;
+    cost[i] = {-(double)tiles[d.a_base + it.ti].cnt * jn, (uint32_t)i};
+  }
+  std::stable_sort(cost.begin(), cost.end());
+  std::vector<Item> out(items.size());
+  for (size_t i = 0; i < items.size(); ++i) out[i] = items[cost[i].second];
+  items.swap(out);
+}
+
+// Type descriptors of one structure whose element e has tiles [ebase[e], ebase[e]+en[e]).
+static void add_types(unsigned n_el, const uint32_t *ebase, const uint32_t *en, uint32_t nhist,
+                      double bin, unsigned long long hist_off, uint32_t ovf,
+                      std::vector<TypeDesc> &types) {
+  unsigned long long off = hist_off;
+  for (unsigned a = 0; a < n_el; ++a)
+    for (unsigned b = a; b < n_el; ++b, off += nhist) {
+      TypeDesc d{};
+      d.hist_off = off;
+      d.nhist = nhist;
+      d.a_base = ebase[a];
+      d.a_n = en[a];
+      d.b_base = ebase[b];
+      d.b_n = en[b];
+      d.same = a == b;
+      d.ovf = ovf;
+      d.bin = bin;
+      d.inv_bin = 1.0 / bin;
+      d.band = band_for(nhist);
+      types.push_back(d);
+    }
+}
+
+// Host-side counting sort of one structure into the sorted arrays (appended),
+// with its tiles and boxes.  xyz: SoA of the structure, element-major.
 template <class T>
-static int run(xs_ctx *ctx, const T *xyz_in, size_t n, const unsigned *el_count, unsigned n_el,
-               double bin, unsigned nhist, uint64_t *hist_out, uint64_t *ovf_out, bool device) {
+static void sort_structure_host(const T *xyz, size_t n, const unsigned *el_count, unsigned n_el,
+                                double bin, uint32_t nhist, std::vector<T> &sx, std::vector<T> &sy,
+                                std::vector<T> &sz, std::vector<Tile> &tiles,
+                                std::vector<double> &boxes, uint32_t *ebase, uint32_t *en) {
+  double bbox[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int a = 0; a < 3; ++a)
+    for (size_t i = 0; i < n; ++i) {
+      const double v = (double)xyz[a * n + i];
+      bbox[a] = std::min(bbox[a], v);
+      bbox[3 + a] = std::max(bbox[3 + a], v);
+    }
+  if (n == 0)
+    for (int a = 0; a < 6; ++a) bbox[a] = 0.0;
+  unsigned max_count = 0;
+  for (unsigned e = 0; e < n_el; ++e) max_count = std::max(max_count, el_count[e]);
+  const Grid gr = make_grid(bbox, max_count, bin, nhist, wcap_for(nhist));
+  const uint32_t m = gr.g[0] * gr.g[1] * gr.g[2];
+  std::vector<uint32_t> key(n), cnt(m), pos(m);
+  size_t off = 0;
+  for (unsigned e = 0; e < n_el; ++e) {
+    std::fill(cnt.begin(), cnt.end(), 0u);
+    for (size_t i = off; i < off + el_count[e]; ++i) {
+      uint32_t c[3];
+      for (int a = 0; a < 3; ++a) {
+        const double f = ((double)xyz[a * n + i] - gr.lo[a]) * gr.inv_cell;
+        const int ci = std::isfinite(f) ? (int)f : 0;
+        c[a] = (uint32_t)std::min(std::max(ci, 0), (int)gr.g[a] - 1);
+      }
+      key[i] = (c[0] * gr.g[1] + c[1]) * gr.g[2] + c[2];
+      ++cnt[key[i]];
+    }
+    const size_t base = sx.size();
+    uint32_t p = (uint32_t)base;
+    ebase[e] = (uint32_t)tiles.size();
+    for (uint32_t c = 0; c < m; ++c) {
+      pos[c] = p;
+      for (uint32_t t = 0; t < cnt[c]; t += kTile)
+        tiles.push_back(Tile{p + t, std::min<uint32_t>(kTile, cnt[c] - t)});
+      p += cnt[c];
+    }
+    en[e] = (uint32_t)tiles.size() - ebase[e];
+    sx.resize(base + el_count[e]);
+    sy.resize(base + el_count[e]);
+    sz.resize(base + el_count[e]);
+    for (size_t i = off; i < off + el_count[e]; ++i) {
+      const uint32_t d = pos[key[i]]++;
+      sx[d] = xyz[i];
+      sy[d] = xyz[n + i];
+      sz[d] = xyz[2 * n + i];
+    }
+    off += el_count[e];
+  }
+  for (size_t t = boxes.size() / 6; t < tiles.size(); ++t) {
+    double b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = tiles[t].off; i < tiles[t].off + tiles[t].cnt; ++i) {
+      const double v[3] = {(double)sx[i], (double)sy[i], (double)sz[i]};
+      for (int a = 0; a < 3; ++a) {
+        b[a] = std::min(b[a], v[a]);
+        b[3 + a] = std::max(b[3 + a], v[a]);
+      }
+    }
+    boxes.insert(boxes.end(), b, b + 6);
+  }
+}
+
+template <class T>
+static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wcap, bool prof) {
+  if (!nitems) return XS_OK;
+  hipEvent_t ev = nullptr;
+  if (prof) XS_TRY(prof_begin(ctx, "hist", &ev));
+  if (wcap == 8192)
+    hipLaunchKernelGGL((hist_kernel<T, 8192>), dim3((unsigned)nitems), dim3(kTile), 0, ctx->stream, A);
+  else
+    hipLaunchKernelGGL((hist_kernel<T, 32768>), dim3((unsigned)nitems), dim3(kTile), 0, ctx->stream, A);
+  XS_HIP_CHECK(hipGetLastError());
+  if (prof) XS_TRY(prof_end(ctx, "hist", ev));
+  return XS_OK;
+}
+
+// ---------------------------------------------------------------- xs_hist
+
+template <class T>
+static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_count,
+                      unsigned n_el, double bin, unsigned nhist, unsigned long long *hist_d,
+                      unsigned long long *ovf_user) {
   const unsigned ntypes = ntypes_of(n_el);
   const size_t hsize = (size_t)ntypes * nhist;
-  const uint32_t wcap = nhist <= 8192 ? 8192u : 32768u;
-  Geom g{};
-  uint32_t off = 0;
+  const uint32_t wcap = wcap_for(nhist);
+  ElemRanges g{};
   unsigned max_count = 0;
+  uint32_t off = 0;
   for (unsigned e = 0; e < n_el; ++e) {
     g.off[e] = off;
     g.cnt[e] = el_count[e];
     off += el_count[e];
     max_count = std::max(max_count, el_count[e]);
   }
+  // 1. bounding box on the GPU, read back
   XS_TRY(ensure_pinned(ctx, 4096));
-
-  // 1. bounding box: on the host for host input; on the device (small
-  //    dedicated workspace, read back) for device input.  Nothing else is
-  //    enqueued before the workspace is sized below.
-  double bbox[6] = {0, 0, 0, 0, 0, 0};
-  if (n > 0) {
-    if (!device) {
-      for (int a = 0; a < 3; ++a) {
-        bbox[a] = INFINITY;
-        bbox[3 + a] = -INFINITY;
-      }
-      for (int a = 0; a < 3; ++a)
-        for (size_t i = 0; i < n; ++i) {
-          const double v = (double)xyz_in[a * n + i];
-          bbox[a] = std::min(bbox[a], v);
-          bbox[3 + a] = std::max(bbox[3 + a], v);
-        }
-    } else {
-      XS_TRY(ensure_workspace(ctx, Carver::need({kBoxBlocks * 6 * sizeof(double), 8 * sizeof(double)})));
-      Carver cb(ctx->ws);
-      double *part = cb.take<double>(kBoxBlocks * 6);
-      double *bbox_d = cb.take<double>(8);
-      const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, kBoxBlocks);
-      hipLaunchKernelGGL((hist_bbox_partial<T>), dim3(nb), dim3(256), 0, ctx->stream, xyz_in,
-                         xyz_in + n, xyz_in + 2 * n, n, part);
-      hipLaunchKernelGGL(hist_bbox_reduce, dim3(1), dim3(256), 0, ctx->stream, part, (int)nb, bbox_d);
-      XS_HIP_CHECK(hipGetLastError());
-      XS_HIP_CHECK(hipMemcpyAsync(ctx->pin, bbox_d, 6 * sizeof(double), hipMemcpyDeviceToHost,
-                                  ctx->stream));
-      XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-      std::memcpy(bbox, ctx->pin, sizeof(bbox));
-    }
-    for (int a = 0; a < 6; ++a)
-      if (!std::isfinite(bbox[a])) return fail(XS_ERR_ARG, "xs_hist: non-finite coordinates");
-    const double diag = std::sqrt((bbox[3] - bbox[0]) * (bbox[3] - bbox[0]) +
-                                  (bbox[4] - bbox[1]) * (bbox[4] - bbox[1]) +
-                                  (bbox[5] - bbox[2]) * (bbox[5] - bbox[2]));
-    if (diag / bin >= 2147483648.0)
-      return fail(XS_ERR_RANGE, "xs_hist: the structure spans more than 2^31 bins");
+  XS_TRY(ensure_workspace(ctx, Carver::need({kBoxBlocks * 6 * sizeof(double), 8 * sizeof(double)})));
+  double bbox[6];
+  {
+    Carver cb(ctx->ws);
+    double *part = cb.take<double>(kBoxBlocks * 6);
+    double *bbox_d = cb.take<double>(8);
+    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, kBoxBlocks);
+    hipLaunchKernelGGL((hist_bbox_partial<T>), dim3(nb), dim3(256), 0, ctx->stream, xyz, xyz + n,
+                       xyz + 2 * n, n, part);
+    hipLaunchKernelGGL(hist_bbox_reduce, dim3(1), dim3(256), 0, ctx->stream, part, (int)nb, bbox_d);
+    XS_HIP_CHECK(hipGetLastError());
+    XS_HIP_CHECK(hipMemcpyAsync(ctx->pin, bbox_d, 6 * sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    std::memcpy(bbox, ctx->pin, sizeof(bbox));
   }
+  XS_TRY(check_span(bbox, bin));
   const Grid grid = make_grid(bbox, max_count, bin, nhist, wcap);
   const uint32_t m = grid.g[0] * grid.g[1] * grid.g[2];
 
-  // 2. one workspace sizing for the whole call (upper bounds for tiles/items)
-  const size_t max_tiles = n / kTile + (size_t)n_el * m + n_el;
-  const size_t max_items = max_tiles * (2 + max_tiles / kSortCap) + (size_t)n_el * n_el;
-  const size_t need = Carver::need({device ? 0 : 3 * n * sizeof(T), 3 * n * sizeof(T),
-                                    n * sizeof(uint32_t), (size_t)n_el * m * sizeof(uint32_t),
-                                    (size_t)n_el * m * sizeof(uint32_t), max_tiles * sizeof(Tile),
-                                    max_tiles * 6 * sizeof(double), max_items * sizeof(Item),
-                                    device ? 0 : hsize * sizeof(uint64_t), sizeof(uint64_t)});
-  XS_TRY(ensure_workspace(ctx, need));
-  XS_TRY(ensure_pinned(ctx, Carver::need({(size_t)n_el * m * sizeof(uint32_t),
-                                          (size_t)n_el * m * sizeof(uint32_t),
-                                          max_tiles * sizeof(Tile), max_items * sizeof(Item)})));
+  // 2. workspace for the rest of the call.  A non-empty cell holds at least
+  //    one atom, so tiles <= n/kTile + min(n, n_el*m) + n_el.
+  const size_t max_tiles = n / kTile + std::min<size_t>(n, (size_t)n_el * m) + n_el;
+  const size_t cb = (size_t)n_el * m * sizeof(uint32_t);
+  XS_TRY(ensure_workspace(ctx, Carver::need({3 * n * sizeof(T), n * sizeof(uint32_t), cb, cb,
+                                             max_tiles * 48, sizeof(unsigned long long)})));
+  XS_TRY(ensure_pinned(ctx, cb));
   Carver cv(ctx->ws);
-  T *stage = device ? nullptr : cv.take<T>(3 * n);
   T *sorted = cv.take<T>(3 * n);
   uint32_t *keys = cv.take<uint32_t>(n);
   uint32_t *count = cv.take<uint32_t>((size_t)n_el * m);
   uint32_t *start_d = cv.take<uint32_t>((size_t)n_el * m);
-  Tile *tiles_d = cv.take<Tile>(max_tiles);
   double *boxes = cv.take<double>(max_tiles * 6);
-  Item *items_d = cv.take<Item>(max_items);
-  unsigned long long *hist_d =
-      device ? reinterpret_cast<unsigned long long *>(hist_out) : cv.take<unsigned long long>(hsize);
-  unsigned long long *ovf_d = (device && ovf_out) ? reinterpret_cast<unsigned long long *>(ovf_out)
-                                                  : cv.take<unsigned long long>(1);
-  const T *xyz = xyz_in;
-  if (!device && n) {
-    XS_HIP_CHECK(hipMemcpyAsync(stage, xyz_in, 3 * n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
-    xyz = stage;
-  }
-  const T *x = xyz, *y = xyz + n, *z = xyz + 2 * n;
-  T *sxp = sorted, *syp = sorted + n, *szp = sorted + 2 * n;
+  unsigned long long *ovf_own = cv.take<unsigned long long>(1);
+  unsigned long long *ovf_d = ovf_user ? ovf_user : ovf_own;
 
   hipEvent_t ev;
   XS_TRY(prof_begin(ctx, "hist_prep", &ev));
-  {
-    const unsigned zb = (unsigned)std::min<size_t>((hsize + 255) / 256, 4096);
-    hipLaunchKernelGGL(fill_u64, dim3(std::max(zb, 1u)), dim3(256), 0, ctx->stream, hist_d, hsize, 0ull);
-    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, ctx->stream, ovf_d, (size_t)1, 0ull);
-  }
+  const unsigned zb = (unsigned)std::min<size_t>((hsize + 255) / 256, 4096);
+  hipLaunchKernelGGL(fill_u64, dim3(std::max(zb, 1u)), dim3(256), 0, ctx->stream, hist_d, hsize);
+  hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, ctx->stream, ovf_d, (size_t)1);
+  XS_HIP_CHECK(hipMemsetAsync(count, 0, cb, ctx->stream));
+  const dim3 eg((max_count + 255) / 256, n_el);
+  hipLaunchKernelGGL((hist_cell_count<T>), eg, dim3(256), 0, ctx->stream, xyz, xyz + n, xyz + 2 * n,
+                     g, grid, keys, count);
+  XS_HIP_CHECK(hipGetLastError());
+  // 3. read the cell counts back; tiles, types and items on the host
+  XS_HIP_CHECK(hipMemcpyAsync(ctx->pin, count, cb, hipMemcpyDeviceToHost, ctx->stream));
+  XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  const uint32_t *count_h = static_cast<const uint32_t *>(ctx->pin);
+  std::vector<uint32_t> start_h((size_t)n_el * m);
   std::vector<Tile> tiles;
-  std::vector<Item> items;
-  if (n > 0 && ntypes > 0) {
-    // 3. cell counts, read back; prefix sums, tiles and work items on the host
-    const size_t cbytes = (size_t)n_el * m * sizeof(uint32_t);
-    XS_HIP_CHECK(hipMemsetAsync(count, 0, cbytes, ctx->stream));
-    const dim3 eg((max_count + 255) / 256, n_el);
-    if (max_count)
-      hipLaunchKernelGGL((hist_cell_count<T>), eg, dim3(256), 0, ctx->stream, x, y, z, g, grid, keys,
-                         count);
-    XS_HIP_CHECK(hipGetLastError());
-    Carver pc(ctx->pin);
-    uint32_t *count_h = pc.take<uint32_t>((size_t)n_el * m);
-    uint32_t *start_h = pc.take<uint32_t>((size_t)n_el * m);
-    Tile *tiles_h = pc.take<Tile>(max_tiles);
-    Item *items_h = pc.take<Item>(max_items);
-    XS_HIP_CHECK(hipMemcpyAsync(count_h, count, cbytes, hipMemcpyDeviceToHost, ctx->stream));
-    XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    for (unsigned e = 0; e < n_el; ++e) {
-      g.tile_base[e] = (uint32_t)tiles.size();
-      uint32_t pos = g.off[e];
-      for (uint32_t c = 0; c < m; ++c) {
-        const uint32_t cnt = count_h[(size_t)e * m + c];
-        start_h[(size_t)e * m + c] = pos;
-        for (uint32_t t = 0; t < cnt; t += kTile)
-          tiles.push_back(Tile{pos + t, std::min<uint32_t>(kTile, cnt - t)});
-        pos += cnt;
-      }
-      g.ntiles[e] = (uint32_t)tiles.size() - g.tile_base[e];
+  uint32_t ebase[kMaxEl], en[kMaxEl];
+  for (unsigned e = 0; e < n_el; ++e) {
+    ebase[e] = (uint32_t)tiles.size();
+    uint32_t pos = g.off[e];
+    for (uint32_t c = 0; c < m; ++c) {
+      const uint32_t k = count_h[(size_t)e * m + c];
+      start_h[(size_t)e * m + c] = pos;
+      for (uint32_t t = 0; t < k; t += kTile)
+        tiles.push_back(Tile{pos + t, std::min<uint32_t>(kTile, k - t)});
+      pos += k;
     }
-    build_items(g, n_el, items);
-    if (tiles.size() > max_tiles || items.size() > max_items)
-      return fail(XS_ERR_RANGE, "xs_hist: tile bound exceeded");
-    std::memcpy(tiles_h, tiles.data(), tiles.size() * sizeof(Tile));
-    std::memcpy(items_h, items.data(), items.size() * sizeof(Item));
-    XS_HIP_CHECK(hipMemcpyAsync(start_d, start_h, cbytes, hipMemcpyHostToDevice, ctx->stream));
-    if (!tiles.empty())
-      XS_HIP_CHECK(hipMemcpyAsync(tiles_d, tiles_h, tiles.size() * sizeof(Tile),
-                                  hipMemcpyHostToDevice, ctx->stream));
-    if (!items.empty())
-      XS_HIP_CHECK(hipMemcpyAsync(items_d, items_h, items.size() * sizeof(Item),
-                                  hipMemcpyHostToDevice, ctx->stream));
-    if (max_count)
-      hipLaunchKernelGGL((hist_scatter<T>), eg, dim3(256), 0, ctx->stream, x, y, z, g, m, keys,
-                         start_d, sxp, syp, szp);
-    if (!tiles.empty())
-      hipLaunchKernelGGL((hist_tile_boxes<T>), dim3((unsigned)tiles.size()), dim3(256), 0,
-                         ctx->stream, sxp, syp, szp, tiles_d, boxes);
-    XS_HIP_CHECK(hipGetLastError());
+    en[e] = (uint32_t)tiles.size() - ebase[e];
   }
+  if (tiles.size() > max_tiles) return fail(XS_ERR_RANGE, "xs_hist: tile bound exceeded");
+  std::vector<TypeDesc> types;
+  add_types(n_el, ebase, en, nhist, bin, 0ull, 0u, types);
+  std::vector<Item> items;
+  add_items(types, 0, (uint32_t)types.size(), items);
+  order_items(items, types, tiles);
+  // tables in their own buffer, sized exactly (the stream is idle here)
+  const size_t tb = tiles.size() * sizeof(Tile), yb = types.size() * sizeof(TypeDesc),
+               ib = items.size() * sizeof(Item);
+  XS_TRY(ensure_aux(ctx, Carver::need({tb, yb, ib})));
+  Carver ca(ctx->aux);
+  Tile *tiles_d = ca.take<Tile>(tiles.size());
+  TypeDesc *types_d = ca.take<TypeDesc>(types.size());
+  Item *items_d = ca.take<Item>(items.size());
+  XS_HIP_CHECK(hipMemcpy(start_d, start_h.data(), cb, hipMemcpyHostToDevice));
+  if (tb) XS_HIP_CHECK(hipMemcpy(tiles_d, tiles.data(), tb, hipMemcpyHostToDevice));
+  XS_HIP_CHECK(hipMemcpy(types_d, types.data(), yb, hipMemcpyHostToDevice));
+  if (ib) XS_HIP_CHECK(hipMemcpy(items_d, items.data(), ib, hipMemcpyHostToDevice));
+  T *sxp = sorted, *syp = sorted + n, *szp = sorted + 2 * n;
+  hipLaunchKernelGGL((hist_scatter<T>), eg, dim3(256), 0, ctx->stream, xyz, xyz + n, xyz + 2 * n, g, m,
+                     keys, start_d, sxp, syp, szp);
+  if (!tiles.empty())
+    hipLaunchKernelGGL((hist_tile_boxes<T>), dim3((unsigned)tiles.size()), dim3(256), 0, ctx->stream,
+                       sxp, syp, szp, tiles_d, boxes);
+  XS_HIP_CHECK(hipGetLastError());
   XS_TRY(prof_end(ctx, "hist_prep", ev));
+  Args<T> A{sxp, syp, szp, tiles_d, boxes, types_d, items_d, hist_d, ovf_d};
+  return launch_hist<T>(ctx, A, items.size(), wcap, true);
+}
 
-  // 4. histogram
-  if (!items.empty()) {
-    Args<T> A{};
-    A.x = sxp;
-    A.y = syp;
-    A.z = szp;
-    A.tiles = tiles_d;
-    A.boxes = boxes;
-    A.items = items_d;
-    A.hist = hist_d;
-    A.overflow = ovf_d;
-    A.bin = (T)bin;
-    A.bin_d = bin;
-    A.inv_bin_d = 1.0 / bin;
-    A.band = 0.5 - std::ldexp((double)nhist, -30) - std::ldexp(1.0, -40);
-    A.nhist = nhist;
-    A.g = g;
-    unsigned p = 0;
-    for (unsigned a = 0; a < n_el; ++a)
-      for (unsigned b = a; b < n_el; ++b, ++p) {
-        A.ta[p] = (uint8_t)a;
-        A.tb[p] = (uint8_t)b;
+template <class T>
+static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_count, unsigned n_el,
+                    double bin, unsigned nhist, uint64_t *hist_out, uint64_t *ovf_out) {
+  const unsigned ntypes = ntypes_of(n_el);
+  const size_t hsize = (size_t)ntypes * nhist;
+  {
+    double bbox[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int a = 0; a < 3; ++a)
+      for (size_t i = 0; i < n; ++i) {
+        bbox[a] = std::min(bbox[a], (double)xyz[a * n + i]);
+        bbox[3 + a] = std::max(bbox[3 + a], (double)xyz[a * n + i]);
       }
-    if (wcap == 8192)
-      XS_TRY((launch_hist<T, 8192>(ctx, A, items.size())));
-    else
-      XS_TRY((launch_hist<T, 32768>(ctx, A, items.size())));
+    XS_TRY(check_span(bbox, bin));
   }
+  std::vector<T> sx, sy, sz;
+  std::vector<Tile> tiles;
+  std::vector<double> boxes;
+  uint32_t ebase[kMaxEl], en[kMaxEl];
+  sort_structure_host<T>(xyz, n, el_count, n_el, bin, nhist, sx, sy, sz, tiles, boxes, ebase, en);
+  std::vector<TypeDesc> types;
+  add_types(n_el, ebase, en, nhist, bin, 0ull, 0u, types);
+  std::vector<Item> items;
+  add_items(types, 0, (uint32_t)types.size(), items);
+  order_items(items, types, tiles);
 
-  if (!device) {
-    XS_HIP_CHECK(hipMemcpyAsync(hist_out, hist_d, hsize * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                                ctx->stream));
-    XS_HIP_CHECK(hipMemcpyAsync(ctx->pin, ovf_d, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
-    XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    if (ovf_out) std::memcpy(ovf_out, ctx->pin, sizeof(uint64_t));
-  } else {
-    // the pinned staging feeds the uploads above; keep it untouched until they ran
-    XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  const size_t nt = std::max<size_t>(tiles.size(), 1), ni = std::max<size_t>(items.size(), 1);
+  XS_TRY(ensure_workspace(ctx, Carver::need({3 * n * sizeof(T), nt * sizeof(Tile), nt * 48,
+                                             ntypes * sizeof(TypeDesc), ni * sizeof(Item),
+                                             hsize * 8, 8})));
+  Carver cv(ctx->ws);
+  T *sorted = cv.take<T>(3 * n);
+  Tile *tiles_d = cv.take<Tile>(nt);
+  double *boxes_d = cv.take<double>(nt * 6);
+  TypeDesc *types_d = cv.take<TypeDesc>(ntypes);
+  Item *items_d = cv.take<Item>(ni);
+  unsigned long long *hist_d = cv.take<unsigned long long>(hsize);
+  unsigned long long *ovf_d = cv.take<unsigned long long>(1);
+  // synchronous uploads from pageable host vectors
+  XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  XS_HIP_CHECK(hipMemcpy(sorted, sx.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  XS_HIP_CHECK(hipMemcpy(sorted + n, sy.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  XS_HIP_CHECK(hipMemcpy(sorted + 2 * n, sz.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  if (!tiles.empty()) {
+    XS_HIP_CHECK(hipMemcpy(tiles_d, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    XS_HIP_CHECK(hipMemcpy(boxes_d, boxes.data(), boxes.size() * 8, hipMemcpyHostToDevice));
   }
+  XS_HIP_CHECK(hipMemcpy(types_d, types.data(), types.size() * sizeof(TypeDesc), hipMemcpyHostToDevice));
+  if (!items.empty())
+    XS_HIP_CHECK(hipMemcpy(items_d, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice));
+  const unsigned zb = (unsigned)std::min<size_t>((hsize + 255) / 256, 4096);
+  hipLaunchKernelGGL(fill_u64, dim3(std::max(zb, 1u)), dim3(256), 0, ctx->stream, hist_d, hsize);
+  hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, ctx->stream, ovf_d, (size_t)1);
+  Args<T> A{sorted, sorted + n, sorted + 2 * n, tiles_d, boxes_d, types_d, items_d, hist_d, ovf_d};
+  XS_TRY(launch_hist<T>(ctx, A, items.size(), wcap_for(nhist), true));
+  XS_HIP_CHECK(hipMemcpyAsync(hist_out, hist_d, hsize * 8, hipMemcpyDeviceToHost, ctx->stream));
+  XS_HIP_CHECK(hipMemcpyAsync(ctx->pin, ovf_d, 8, hipMemcpyDeviceToHost, ctx->stream));
+  XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (ovf_out) std::memcpy(ovf_out, ctx->pin, 8);
   return XS_OK;
 }
 
 }  // namespace hist
+}  // namespace xs
+
+// ---------------------------------------------------------------- batch plans
+
+namespace xs {
+namespace hist {
+
+static void free_plan(xs_batch *b) {
+  if (!b) return;
+  if (b->ctx) b->ctx->plans.erase(b);
+  for (void *p : {b->coords, b->tiles, b->boxes, b->types, b->items, (void *)b->ovf})
+    if (p) (void)hipFree(p);
+  release_batch_run(b->run);
+  delete b;
+}
+
+template <class T>
+static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const unsigned *el_count,
+                      const double *const *xyz, double bin, const unsigned *nhist_in, bool f64,
+                      xs_batch **out) {
+  auto *b = new xs_batch();
+  b->ctx = ctx;
+  b->f64 = f64;
+  b->bin = bin;
+  b->nstruct = nstruct;
+  std::vector<T> sx, sy, sz, tmp;
+  std::vector<Tile> tiles;
+  std::vector<double> boxes;
+  std::vector<TypeDesc> types;
+  size_t ec = 0;
+  unsigned long long hoff = 0;
+  for (unsigned s = 0; s < nstruct; ++s) {
+    const unsigned ne = n_el[s];
+    if (ne == 0 || ne > (unsigned)kMaxEl) {
+      free_plan(b);
+      return fail(XS_ERR_RANGE, "xs_batch_plan: n_el must be in [1, 16]");
+    }
+    size_t n = 0;
+    for (unsigned e = 0; e < ne; ++e) n += el_count[ec + e];
+    if (n == 0 || n >= (size_t(1) << 31)) {
+      free_plan(b);
+      return fail(XS_ERR_RANGE, "xs_batch_plan: a structure must hold 1 to 2^31-1 atoms");
+    }
+    const double *X = xyz[s];
+    double bbox[6];
+    for (int a = 0; a < 3; ++a) bbox[a] = bbox[3 + a] = X[a * n];
+    for (int a = 0; a < 3; ++a)
+      for (size_t i = 0; i < n; ++i) {
+        bbox[a] = std::min(bbox[a], X[a * n + i]);
+        bbox[3 + a] = std::max(bbox[3 + a], X[a * n + i]);
+      }
+    const int rc = check_span(bbox, bin);
+    if (rc != XS_OK) {
+      free_plan(b);
+      return rc;
+    }
+    // bin count as the reference's GetHistSize (CalcFunctions.cpp:224-243) unless given
+    uint32_t nh = nhist_in ? nhist_in[s] : 0u;
+    if (!nh) {
+      const double dx = bbox[3] - bbox[0], dy = bbox[4] - bbox[1], dz = bbox[5] - bbox[2];
+      nh = (uint32_t)(std::sqrt(dx * dx + dy * dy + dz * dz) / bin) + 1;
+    }
+    if (!f64 && nh > (1u << 24)) {
+      free_plan(b);
+      return fail(XS_ERR_RANGE, "xs_batch_plan: nhist > 2^24 needs XS_F64_BINNING");
+    }
+    tmp.resize(3 * n);
+    for (size_t i = 0; i < 3 * n; ++i) tmp[i] = (T)X[i];
+    uint32_t ebase[kMaxEl], en[kMaxEl];
+    sort_structure_host<T>(tmp.data(), n, el_count + ec, ne, bin, nh, sx, sy, sz, tiles, boxes, ebase,
+                           en);
+    const size_t t0 = types.size();
+    add_types(ne, ebase, en, nh, bin, hoff, s, types);
+    b->type_first.push_back(t0);
+    for (size_t t = t0; t < types.size(); ++t) {
+      b->type_hist_off.push_back(types[t].hist_off);
+      b->type_nhist.push_back(nh);
+    }
+    b->pdf_off.push_back(b->pdf_len);
+    b->pdf_len += (unsigned long long)(1 + ntypes_of(ne)) * nh;
+    b->n_el.push_back(ne);
+    b->el_first.push_back(ec);
+    b->nhist.push_back(nh);
+    b->ntot.push_back((unsigned)n);
+    b->hist_off.push_back(hoff);
+    for (unsigned e = 0; e < ne; ++e) b->el_count.push_back(el_count[ec + e]);
+    hoff += (unsigned long long)ntypes_of(ne) * nh;
+    ec += ne;
+  }
+  b->hist_len = hoff;
+  b->ntypes_total = types.size();
+  // work items: the 8192-bin window class, then the 32768 class, each longest first
+  std::vector<Item> all, small, large;
+  add_items(types, 0, (uint32_t)types.size(), all);
+  for (const Item &it : all) (types[it.type].nhist <= 8192 ? small : large).push_back(it);
+  order_items(small, types, tiles);
+  order_items(large, types, tiles);
+  b->items8192 = small.size();
+  b->items32768 = large.size();
+  small.insert(small.end(), large.begin(), large.end());
+  const size_t nc = std::max<size_t>(sx.size(), 1);
+  b->natoms = nc;
+  const bool ok =
+      hipMalloc(&b->coords, 3 * nc * sizeof(T)) == hipSuccess &&
+      hipMalloc(&b->tiles, std::max<size_t>(tiles.size(), 1) * sizeof(Tile)) == hipSuccess &&
+      hipMalloc(&b->boxes, std::max<size_t>(boxes.size(), 6) * 8) == hipSuccess &&
+      hipMalloc(&b->types, std::max<size_t>(types.size(), 1) * sizeof(TypeDesc)) == hipSuccess &&
+      hipMalloc(&b->items, std::max<size_t>(small.size(), 1) * sizeof(Item)) == hipSuccess &&
+      hipMalloc((void **)&b->ovf, std::max<size_t>(nstruct, 1) * 8) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    free_plan(b);
+    return fail(XS_ERR_NOMEM, "xs_batch_plan: device allocation failed");
+  }
+  T *c = static_cast<T *>(b->coords);
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess && !sx.empty()) {
+    e = hipMemcpy(c, sx.data(), sx.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c + nc, sy.data(), sy.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c + 2 * nc, sz.data(), sz.size() * sizeof(T), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess && !tiles.empty())
+    e = hipMemcpy(b->tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !boxes.empty())
+    e = hipMemcpy(b->boxes, boxes.data(), boxes.size() * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !types.empty())
+    e = hipMemcpy(b->types, types.data(), types.size() * sizeof(TypeDesc), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !small.empty())
+    e = hipMemcpy(b->items, small.data(), small.size() * sizeof(Item), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    free_plan(b);
+    return fail(XS_ERR_HIP, std::string("xs_batch_plan: upload failed: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  ctx->plans.insert(b);
+  return XS_OK;
+}
+
+// Histogram launches of a plan (zero fills + kernels): no host work, capturable.
+template <class T>
+static int plan_hist(xs_batch *b, unsigned long long *hist_d, bool prof) {
+  xs_ctx *ctx = b->ctx;
+  const unsigned zb = (unsigned)std::min<unsigned long long>((b->hist_len + 255) / 256, 8192);
+  hipLaunchKernelGGL(fill_u64, dim3(std::max(zb, 1u)), dim3(256), 0, ctx->stream, hist_d,
+                     (size_t)b->hist_len);
+  hipLaunchKernelGGL(fill_u64, dim3(1), dim3(256), 0, ctx->stream, b->ovf, (size_t)b->nstruct);
+  XS_HIP_CHECK(hipGetLastError());
+  const T *c = static_cast<const T *>(b->coords);
+  const Item *items = static_cast<const Item *>(b->items);
+  Args<T> A{c,
+            c + b->natoms,
+            c + 2 * b->natoms,
+            static_cast<const Tile *>(b->tiles),
+            static_cast<const double *>(b->boxes),
+            static_cast<const TypeDesc *>(b->types),
+            items,
+            hist_d,
+            b->ovf};
+  XS_TRY(launch_hist<T>(ctx, A, b->items8192, 8192, prof));
+  A.items = items + b->items8192;
+  XS_TRY(launch_hist<T>(ctx, A, b->items32768, 32768, prof));
+  return XS_OK;
+}
+
+}  // namespace hist
+
+void destroy_batch(xs_batch *b) { hist::free_plan(b); }
+
+int batch_hist_launch(xs_batch *b, unsigned long long *hist_d, bool prof) {
+  return b->f64 ? hist::plan_hist<double>(b, hist_d, prof) : hist::plan_hist<float>(b, hist_d, prof);
+}
+
 }  // namespace xs
 
 using namespace xs;
@@ -684,9 +972,70 @@ extern "C" int xs_hist(xs_ctx *ctx, const void *xyz, size_t n_atoms, const unsig
   if (!f64 && nhist > (1u << 24))
     return fail(XS_ERR_RANGE, "xs_hist: nhist > 2^24 needs XS_F64_BINNING");
   const bool device = flags & XS_DEVICE_PTRS;
+  if (n_atoms == 0) {
+    const size_t hsize = (size_t)ntypes_of(n_el) * nhist;
+    if (device) {
+      XS_HIP_CHECK(hipMemsetAsync(hist, 0, hsize * 8, ctx->stream));
+      if (overflow) XS_HIP_CHECK(hipMemsetAsync(overflow, 0, 8, ctx->stream));
+    } else {
+      std::memset(hist, 0, hsize * 8);
+      if (overflow) *overflow = 0;
+    }
+    return XS_OK;
+  }
+  auto *h = reinterpret_cast<unsigned long long *>(hist);
+  auto *o = reinterpret_cast<unsigned long long *>(overflow);
+  if (device) {
+    if (f64)
+      return hist::run_device<double>(ctx, static_cast<const double *>(xyz), n_atoms, el_count, n_el,
+                                      bin, nhist, h, o);
+    return hist::run_device<float>(ctx, static_cast<const float *>(xyz), n_atoms, el_count, n_el, bin,
+                                   nhist, h, o);
+  }
   if (f64)
-    return hist::run<double>(ctx, static_cast<const double *>(xyz), n_atoms, el_count, n_el, bin,
-                             nhist, hist, overflow, device);
-  return hist::run<float>(ctx, static_cast<const float *>(xyz), n_atoms, el_count, n_el, bin, nhist,
-                          hist, overflow, device);
+    return hist::run_host<double>(ctx, static_cast<const double *>(xyz), n_atoms, el_count, n_el, bin,
+                                  nhist, hist, overflow);
+  return hist::run_host<float>(ctx, static_cast<const float *>(xyz), n_atoms, el_count, n_el, bin,
+                               nhist, hist, overflow);
+}
+
+extern "C" int xs_batch_plan(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el,
+                             const unsigned *el_count, const double *const *xyz, double bin,
+                             const unsigned *nhist, unsigned flags, xs_batch **plan) {
+  XS_TRY(activate(ctx));
+  if (!plan || !n_el || !el_count || !xyz) return fail(XS_ERR_ARG, "xs_batch_plan: null pointer");
+  *plan = nullptr;
+  if (nstruct == 0) return fail(XS_ERR_ARG, "xs_batch_plan: no structures");
+  if (!(bin > 0.0) || !std::isfinite(bin)) return fail(XS_ERR_ARG, "xs_batch_plan: bin must be > 0");
+  for (unsigned s = 0; s < nstruct; ++s)
+    if (!xyz[s]) return fail(XS_ERR_ARG, "xs_batch_plan: null structure");
+  if (flags & XS_F64_BINNING)
+    return hist::plan_build<double>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, true, plan);
+  return hist::plan_build<float>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, false, plan);
+}
+
+extern "C" int xs_batch_free(xs_batch *plan) {
+  if (!plan) return XS_OK;
+  (void)hipSetDevice(plan->ctx->device);
+  (void)hipStreamSynchronize(plan->ctx->stream);
+  hist::free_plan(plan);
+  return XS_OK;
+}
+
+extern "C" int xs_batch_info(const xs_batch *plan, unsigned *nhist, unsigned long long *hist_off,
+                             unsigned long long *pdf_off, unsigned long long *type_first,
+                             unsigned long long *totals) {
+  if (!plan) return fail(XS_ERR_ARG, "xs_batch_info: null plan");
+  for (unsigned s = 0; s < plan->nstruct; ++s) {
+    if (nhist) nhist[s] = plan->nhist[s];
+    if (hist_off) hist_off[s] = plan->hist_off[s];
+    if (pdf_off) pdf_off[s] = plan->pdf_off[s];
+    if (type_first) type_first[s] = plan->type_first[s];
+  }
+  if (totals) {
+    totals[0] = plan->hist_len;
+    totals[1] = plan->pdf_len;
+    totals[2] = plan->ntypes_total;
+  }
+  return XS_OK;
 }

@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -41,7 +42,43 @@ struct ProfRecord {
   hipEvent_t start, stop;
 };
 
+// Per-plan state of xs_batch_run: the captured graph and its key, and the
+// device buffers the run owns.
+struct BatchRunState {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void *key_hist = nullptr, *key_S = nullptr, *key_pdf = nullptr, *key_q = nullptr;
+  unsigned key_nq = 0, key_flags = 0;
+  int key_pdf_type = -1;
+  std::vector<double> key_rho0, key_sl;
+  void *buf = nullptr;   // pair-sum and PDF descriptors, host-mode staging
+  size_t buf_cap = 0;
+};
+
 }  // namespace xs
+
+// A histogram plan over an ensemble of structures (built by xs_batch_plan).
+struct xs_batch {
+  xs_ctx *ctx = nullptr;
+  bool f64 = true;
+  double bin = 0;
+  unsigned nstruct = 0;
+  std::vector<unsigned> n_el, el_count, nhist, ntot;
+  std::vector<size_t> el_first;                   // first el_count index of each structure
+  std::vector<unsigned long long> hist_off;       // per structure, in bins
+  std::vector<unsigned long long> pdf_off;        // per structure, in doubles
+  std::vector<unsigned long long> type_first;     // first pair type of each structure
+  std::vector<unsigned long long> type_hist_off;  // per pair type
+  std::vector<uint32_t> type_nhist;
+  unsigned long long hist_len = 0, pdf_len = 0;
+  size_t ntypes_total = 0;
+  size_t items8192 = 0, items32768 = 0;
+  size_t natoms = 0;      // per axis in the sorted coordinate arrays
+  void *coords = nullptr;  // sorted SoA (float or double)
+  void *tiles = nullptr, *boxes = nullptr, *types = nullptr, *items = nullptr;
+  unsigned long long *ovf = nullptr;  // one overflow counter per structure
+  xs::BatchRunState run;
+};
 
 struct xs_ctx {
   int device = 0;
@@ -53,15 +90,18 @@ struct xs_ctx {
   // Device workspace carved per call (grown on demand, never shrunk).
   void *ws = nullptr;
   size_t ws_size = 0;
+  // Device buffer for per-call tables (tiles, work items), grown on demand.
+  void *aux = nullptr;
+  size_t aux_size = 0;
+  // Device buffer of the matrix-core pair sums (operand slices), grown on demand.
+  void *mat = nullptr;
+  size_t mat_size = 0;
   // Pinned host staging for small uploads (item lists, parameter tables).
   void *pin = nullptr;
   size_t pin_size = 0;
 
-  // Cached device-side work list of the histogram kernel.
-  std::vector<uint32_t> hist_items_key;
-  void *hist_items_dev = nullptr;
-  size_t hist_items_cap = 0;
-  size_t hist_items_n = 0;
+  // Batch plans made on this context (freed by xs_free if still alive).
+  std::set<xs_batch *> plans;
 
   bool profiling = false;
   std::vector<xs::ProfRecord> prof_pending;
@@ -75,12 +115,52 @@ namespace xs {
 int activate(xs_ctx *ctx);
 // Ensure ctx->ws holds at least `bytes` (synchronises the stream if it grows).
 int ensure_workspace(xs_ctx *ctx, size_t bytes);
+// Ensure ctx->aux holds at least `bytes` (synchronises the stream if it grows).
+int ensure_aux(xs_ctx *ctx, size_t bytes);
+// Ensure ctx->mat holds at least `bytes` (synchronises the stream if it grows).
+int ensure_matrix(xs_ctx *ctx, size_t bytes);
 // Ensure ctx->pin holds at least `bytes` (synchronises the stream if it grows).
 int ensure_pinned(xs_ctx *ctx, size_t bytes);
 
 // Profiling: wrap a launch with events when collection is on.
 int prof_begin(xs_ctx *ctx, const char *name, hipEvent_t *start);
 int prof_end(xs_ctx *ctx, const char *name, hipEvent_t start);
+
+void release_batch_run(BatchRunState &r);
+// Free a plan's device memory and unregister it from its context.
+void destroy_batch(xs_batch *b);
+// Launches of a batch plan: zero fills and kernels only, no host work or
+// synchronisation, so they can be captured into a graph.  `prof` wraps them
+// in profiling events (not while capturing).
+int batch_hist_launch(xs_batch *b, unsigned long long *hist_d, bool prof);
+
+// Pair sums of many histograms: one chunk = up to 2048 bins of one pair type.
+struct PairsumChunk {
+  unsigned long long hist_off;  // first bin of the chunk in the histogram array
+  uint32_t type;                // row of S (S[type * nq + iq])
+  uint32_t k0, kn, pad;         // bin index of the chunk start within its histogram, length
+};
+constexpr unsigned kPairsumChunk = 2048;
+// S[ntypes * nq] is zeroed and accumulated.
+int batch_pairsum_launch(xs_ctx *ctx, const unsigned long long *hist, const PairsumChunk *chunks,
+                         size_t nchunks, size_t ntypes, double bin, const double *q, unsigned nq,
+                         double *S, bool prof);
+
+// H3 as an int8 matrix-core product (device pointers, S[ncols * nq] overwritten).
+int pairsum_mfma(xs_ctx *ctx, const unsigned long long *hist, unsigned ncols, unsigned nhist,
+                 double bin, const double *q, unsigned nq, double *S);
+
+// PDFs of many structures; blk0 = first 256-bin block of the structure.
+struct PdfDesc {
+  unsigned long long hist_off, pdf_off;
+  uint32_t nhist, n_el, ntot, blk0;
+  double rho0;
+  uint32_t el_count[16];
+  double sl[16];
+};
+int batch_pdf_launch(xs_ctx *ctx, const unsigned long long *hist, const PdfDesc *desc,
+                     unsigned nstruct, unsigned nblocks, int pdf_type, bool cpu_radius, double bin,
+                     double *pdf, bool prof);
 
 // Bump allocator over the workspace for one call.
 struct Carver {

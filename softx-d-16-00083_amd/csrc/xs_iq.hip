@@ -30,7 +30,7 @@ namespace xs {
 namespace iq {
 
 constexpr int kQBlock = 256;
-constexpr int kChunk = 2048;
+constexpr int kChunk = (int)kPairsumChunk;
 constexpr double kPi = 3.14159265358979323846;
 
 __global__ void fill_f64(double *__restrict__ p, size_t n) {
@@ -39,22 +39,9 @@ __global__ void fill_f64(double *__restrict__ p, size_t n) {
     p[i] = 0.0;
 }
 
-__global__ __launch_bounds__(kQBlock) void pairsum_kernel(const unsigned long long *__restrict__ hist,
-                                                          unsigned nhist, const double *__restrict__ q,
-                                                          unsigned nq, double bin,
-                                                          double *__restrict__ S) {
-  __shared__ double2 bins[kChunk];
-  const unsigned p = blockIdx.z;
-  const unsigned k0 = blockIdx.y * kChunk;
-  const unsigned kn = min((unsigned)kChunk, nhist - k0);
-  const unsigned long long *h = hist + (size_t)p * nhist + k0;
-  for (unsigned b = threadIdx.x; b < kn; b += kQBlock)
-    bins[b] = make_double2((double)h[b], 1.0 / ((double)(k0 + b) + 0.5));
-  __syncthreads();
-  const unsigned iq = blockIdx.x * kQBlock + threadIdx.x;
-  if (iq >= nq) return;
-  const double qv = q[iq];
-  if (qv == 0.0) return;
+// Sum over one staged chunk of bins [k0, k0 + kn) for one q (qv != 0).
+__device__ __forceinline__ double pairsum_chunk(const double2 *bins, unsigned k0, unsigned kn,
+                                                double qv, double bin) {
   const double turns = qv * bin / (2.0 * kPi);  // phase step per bin, in turns
   double s, c, ds, dc;
   sincos_2pi(((double)k0 + 0.5) * turns, &s, &c);
@@ -68,7 +55,47 @@ __global__ __launch_bounds__(kQBlock) void pairsum_kernel(const unsigned long lo
     c = fma(c, dc, -(s * ds));
     s = sn;
   }
+  return acc;
+}
+
+__device__ __forceinline__ void stage_chunk(double2 *bins, const unsigned long long *__restrict__ h,
+                                            unsigned k0, unsigned kn) {
+  for (unsigned b = threadIdx.x; b < kn; b += kQBlock)
+    bins[b] = make_double2((double)h[b], 1.0 / ((double)(k0 + b) + 0.5));
+}
+
+__global__ __launch_bounds__(kQBlock) void pairsum_kernel(const unsigned long long *__restrict__ hist,
+                                                          unsigned nhist, const double *__restrict__ q,
+                                                          unsigned nq, double bin,
+                                                          double *__restrict__ S) {
+  __shared__ double2 bins[kChunk];
+  const unsigned p = blockIdx.z;
+  const unsigned k0 = blockIdx.y * kChunk;
+  const unsigned kn = min((unsigned)kChunk, nhist - k0);
+  stage_chunk(bins, hist + (size_t)p * nhist + k0, k0, kn);
+  __syncthreads();
+  const unsigned iq = blockIdx.x * kQBlock + threadIdx.x;
+  if (iq >= nq) return;
+  const double qv = q[iq];
+  if (qv == 0.0) return;
+  const double acc = pairsum_chunk(bins, k0, kn, qv, bin);
   if (acc != 0.0) atomicAdd(&S[(size_t)p * nq + iq], acc / (qv * bin));
+}
+
+// Many histograms: blockIdx.x = chunk, blockIdx.y = block of 256 q.
+__global__ __launch_bounds__(kQBlock) void pairsum_batch_kernel(
+    const unsigned long long *__restrict__ hist, const PairsumChunk *__restrict__ chunks,
+    const double *__restrict__ q, unsigned nq, double bin, double *__restrict__ S) {
+  __shared__ double2 bins[kChunk];
+  const PairsumChunk ch = chunks[blockIdx.x];
+  stage_chunk(bins, hist + ch.hist_off, ch.k0, ch.kn);
+  __syncthreads();
+  const unsigned iq = blockIdx.y * kQBlock + threadIdx.x;
+  if (iq >= nq) return;
+  const double qv = q[iq];
+  if (qv == 0.0) return;
+  const double acc = pairsum_chunk(bins, ch.k0, ch.kn, qv, bin);
+  if (acc != 0.0) atomicAdd(&S[(size_t)ch.type * nq + iq], acc / (qv * bin));
 }
 
 struct PdfArgs {
@@ -82,45 +109,102 @@ struct PdfArgs {
   double sl[16];
 };
 
-__global__ void pdf_kernel(PdfArgs A) {
+// All PDF values of bin k of one structure: partials at pdf[nhist + p*nhist + k],
+// total at pdf[k].  Operation order of CalcFunctions.cpp:919-957.
+__device__ __forceinline__ void pdf_point(const unsigned long long *__restrict__ hist,
+                                          double *__restrict__ pdf, unsigned k, unsigned nhist,
+                                          unsigned n_el, unsigned ntot, const unsigned *el_count,
+                                          const double *sl, double rho0, int type, int cpu_radius,
+                                          double bin) {
 #pragma clang fp contract(off)
-  const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= A.nhist) return;
   double fav2 = 0;
-  for (unsigned e = 0; e < A.n_el; ++e) fav2 += A.sl[e] * A.el_count[e];
-  fav2 /= A.ntot;
+  for (unsigned e = 0; e < n_el; ++e) fav2 += sl[e] * el_count[e];
+  fav2 /= ntot;
   fav2 *= fav2;
-  const double nt2 = (double)A.ntot * (double)A.ntot;
+  const double nt2 = (double)ntot * (double)ntot;
   double total = 0.0;
   size_t start = 0;
-  unsigned p = 0;
-  for (unsigned a = 0; a < A.n_el; ++a) {
-    for (unsigned b = a; b < A.n_el; ++b, ++p, start += A.nhist) {
-      const double wab = A.sl[a] * A.sl[b] / fav2;
+  for (unsigned a = 0; a < n_el; ++a) {
+    for (unsigned b = a; b < n_el; ++b, start += nhist) {
+      const double wab = sl[a] * sl[b] / fav2;
       const size_t i = start + k;
-      const double cnt = (double)A.hist[i];
-      const double r = ((A.cpu_radius ? (double)i : (double)k) + 0.5) * A.bin;
+      const double cnt = (double)hist[i];
+      const double r = ((cpu_radius ? (double)i : (double)k) + 0.5) * bin;
       double v;
-      if (A.type == XS_PDF_RDF) {
-        v = cnt * (2. / (A.bin * A.ntot));
-      } else if (A.type == XS_PDF_PDF) {
-        const double mult = 0.5 / (kPi * A.bin * A.rho0 * A.ntot);
+      if (type == XS_PDF_RDF) {
+        v = cnt * (2. / (bin * ntot));
+      } else if (type == XS_PDF_PDF) {
+        const double mult = 0.5 / (kPi * bin * rho0 * ntot);
         v = cnt * mult / (r * r);
       } else {
-        const double mult = 2. / (A.bin * A.ntot);
-        const double sub =
-            (b > a) ? 8. * kPi * A.rho0 * (double)A.el_count[a] * (double)A.el_count[b] / nt2
-                    : 4. * kPi * A.rho0 * ((double)A.el_count[a] * (double)A.el_count[a]) / nt2;
+        const double mult = 2. / (bin * ntot);
+        const double sub = (b > a) ? 8. * kPi * rho0 * (double)el_count[a] * (double)el_count[b] / nt2
+                                   : 4. * kPi * rho0 * ((double)el_count[a] * (double)el_count[a]) / nt2;
         v = cnt * mult / r - sub * r;
       }
-      A.pdf[A.nhist + i] = v;
+      pdf[nhist + i] = v;
       total += v * wab;
     }
   }
-  A.pdf[k] = total;
+  pdf[k] = total;
+}
+
+__global__ void pdf_kernel(PdfArgs A) {
+  const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= A.nhist) return;
+  pdf_point(A.hist, A.pdf, k, A.nhist, A.n_el, A.ntot, A.el_count, A.sl, A.rho0, A.type,
+            A.cpu_radius, A.bin);
+}
+
+// Many structures: 256-bin blocks laid end to end, structure found by binary
+// search over the descriptors' first block.
+__global__ void pdf_batch_kernel(const unsigned long long *__restrict__ hist, double *__restrict__ pdf,
+                                 const PdfDesc *__restrict__ desc, unsigned nstruct, int type,
+                                 int cpu_radius, double bin) {
+  unsigned lo = 0, hi = nstruct - 1;
+  while (lo < hi) {
+    const unsigned mid = (lo + hi + 1) / 2;
+    if (desc[mid].blk0 <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const PdfDesc &d = desc[lo];
+  const unsigned k = (blockIdx.x - d.blk0) * blockDim.x + threadIdx.x;
+  if (k >= d.nhist) return;
+  pdf_point(hist + d.hist_off, pdf + d.pdf_off, k, d.nhist, d.n_el, d.ntot, d.el_count, d.sl, d.rho0,
+            type, cpu_radius, bin);
 }
 
 }  // namespace iq
+
+int batch_pairsum_launch(xs_ctx *ctx, const unsigned long long *hist, const PairsumChunk *chunks,
+                         size_t nchunks, size_t ntypes, double bin, const double *q, unsigned nq,
+                         double *S, bool prof) {
+  const size_t ssize = ntypes * nq;
+  const unsigned zb = (unsigned)std::min<size_t>((ssize + 255) / 256, 8192);
+  hipLaunchKernelGGL(iq::fill_f64, dim3(std::max(zb, 1u)), dim3(256), 0, ctx->stream, S, ssize);
+  if (!nchunks) return XS_OK;
+  hipEvent_t ev = nullptr;
+  if (prof) XS_TRY(prof_begin(ctx, "batch_pairsum", &ev));
+  hipLaunchKernelGGL(iq::pairsum_batch_kernel, dim3((unsigned)nchunks, (nq + iq::kQBlock - 1) / iq::kQBlock),
+                     dim3(iq::kQBlock), 0, ctx->stream, hist, chunks, q, nq, bin, S);
+  XS_HIP_CHECK(hipGetLastError());
+  if (prof) XS_TRY(prof_end(ctx, "batch_pairsum", ev));
+  return XS_OK;
+}
+
+int batch_pdf_launch(xs_ctx *ctx, const unsigned long long *hist, const PdfDesc *desc,
+                     unsigned nstruct, unsigned nblocks, int pdf_type, bool cpu_radius, double bin,
+                     double *pdf, bool prof) {
+  if (!nblocks) return XS_OK;
+  hipEvent_t ev = nullptr;
+  if (prof) XS_TRY(prof_begin(ctx, "batch_pdf", &ev));
+  hipLaunchKernelGGL(iq::pdf_batch_kernel, dim3(nblocks), dim3(256), 0, ctx->stream, hist, pdf, desc,
+                     nstruct, pdf_type, cpu_radius ? 1 : 0, bin);
+  XS_HIP_CHECK(hipGetLastError());
+  if (prof) XS_TRY(prof_end(ctx, "batch_pdf", ev));
+  return XS_OK;
+}
+
 }  // namespace xs
 
 using namespace xs;
@@ -150,15 +234,19 @@ extern "C" int xs_hist_pairsum(xs_ctx *ctx, const uint64_t *hist, unsigned ntype
     hd = h2;
     qd = q2;
   }
-  const unsigned zb = (unsigned)std::min<size_t>((ssize + 255) / 256, 4096);
-  hipLaunchKernelGGL(iq::fill_f64, dim3(zb), dim3(256), 0, ctx->stream, Sd, ssize);
-  hipEvent_t ev;
-  XS_TRY(prof_begin(ctx, "hist_pairsum", &ev));
-  dim3 grid((nq + iq::kQBlock - 1) / iq::kQBlock, (nhist + iq::kChunk - 1) / iq::kChunk, ntypes);
-  hipLaunchKernelGGL(iq::pairsum_kernel, grid, dim3(iq::kQBlock), 0, ctx->stream, hd, nhist, qd, nq,
-                     bin, Sd);
-  XS_HIP_CHECK(hipGetLastError());
-  XS_TRY(prof_end(ctx, "hist_pairsum", ev));
+  if (flags & XS_PAIRSUM_MFMA) {
+    XS_TRY(pairsum_mfma(ctx, hd, ntypes, nhist, bin, qd, nq, Sd));
+  } else {
+    const unsigned zb = (unsigned)std::min<size_t>((ssize + 255) / 256, 4096);
+    hipLaunchKernelGGL(iq::fill_f64, dim3(zb), dim3(256), 0, ctx->stream, Sd, ssize);
+    hipEvent_t ev;
+    XS_TRY(prof_begin(ctx, "hist_pairsum", &ev));
+    dim3 grid((nq + iq::kQBlock - 1) / iq::kQBlock, (nhist + iq::kChunk - 1) / iq::kChunk, ntypes);
+    hipLaunchKernelGGL(iq::pairsum_kernel, grid, dim3(iq::kQBlock), 0, ctx->stream, hd, nhist, qd,
+                       nq, bin, Sd);
+    XS_HIP_CHECK(hipGetLastError());
+    XS_TRY(prof_end(ctx, "hist_pairsum", ev));
+  }
   if (!device) {
     XS_HIP_CHECK(hipMemcpyAsync(S, Sd, ssize * 8, hipMemcpyDeviceToHost, ctx->stream));
     XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));

@@ -1,0 +1,281 @@
+// H3 as a matrix product on the int8 matrix cores (v_mfma_i32_32x32x32_i8).
+//
+// S[c][q] = sum_k A[q][k] H[c][k],  A[q][k] = sin(q r_k) / (q r_k + 1e-8),
+// r_k = (k + 1/2) bin, H = histogram counts (one column c per pair type).
+//
+// Exact-integer splitting.  A is evaluated in FP64 and split into NA signed
+// int8 slices, A = sum_i a_i 2^(-6-7i) + e with |a_0| <= 64, |a_i| <= 64 and
+// |e| <= 2^(-7 NA - 0) (the rounding residual of the last slice); counts are
+// split exactly into NB 7-bit slices, H = sum_j b_j 2^(7j), b_j in [0, 127].
+// Every slice product sum_k a_i b_j is computed exactly by the int8 MFMA in
+// int32.  Products with the same d = j - i share the scale 2^(7d-6) and one
+// int32 accumulator; accumulators are moved into FP64 every 32,768 bins,
+// before any can overflow (|acc| <= 5 * 64 * 127 * 32768 < 2^31).  All
+// NA * NB products are kept: dropping a_4 b_0 would leave an error of
+// 2^-28 per count on bins with small counts.  With NA = 5 the result differs
+// from the FP64 sum by at most 2^-35 sum_k H[c][k] per entry, against the
+// 2^-24-per-term of an FP32 (or split-BF16, FP32 accumulated) product; the
+// cancellation of I(q) at its minima amplifies this by up to ~10^4, which
+// FP32 accumulation cannot absorb.
+//
+// Layout.  Slices are stored pre-tiled in the MFMA operand layout: for a
+// 32 x 32 (rows x k) tile, lane l holds row l % 32 and k = 16 (l / 32) + j in
+// byte j of a 16-byte register group, so a wave loads a tile as one
+// contiguous 1 KiB read.  A and B use the same k assignment, which is all the
+// product needs.  Accumulator element 4i + jj of lane l is row
+// 8i + 4 (l / 32) + jj, column l % 32.
+//
+// Work groups: 4 waves stacked along q (128 q) x NT tiles of 32 columns;
+// tiles are ordered so that the work groups that share a q block run on the
+// same XCD (consecutive work-group ids go to different XCDs).
+#include <algorithm>
+#include <cmath>
+
+#include "xs_internal.h"
+#include "xs_sincos.h"
+
+namespace xs {
+namespace mm {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int kNA = 5;             // sinc slices
+constexpr int kMaxNB = 5;          // count slices (counts < 2^35)
+constexpr unsigned kFlush = 1024;  // k tiles (of 32 bins) per int32 accumulation
+constexpr double kPi = 3.14159265358979323846;
+
+__global__ void max_u64(const unsigned long long *__restrict__ h, size_t n,
+                        unsigned long long *__restrict__ out) {
+  unsigned long long m = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    m = max(m, h[i]);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned long long)__shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+
+__global__ void zero_u64(unsigned long long *p) { *p = 0ull; }
+
+// One thread per (q tile, k tile, lane): 16 sinc values, NA slices of 16 bytes.
+__global__ void slice_sinc(const double *__restrict__ q, unsigned nq, unsigned nhist, double bin,
+                           unsigned MT, unsigned KT, v4i *__restrict__ As) {
+  const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t total = (size_t)MT * KT * 64;
+  if (tid >= total) return;
+  const unsigned lane = tid & 63;
+  const size_t tile = tid >> 6;
+  const unsigned kt = tile % KT;
+  const unsigned mt = tile / KT;
+  const unsigned row = mt * 32 + (lane & 31);
+  const unsigned k0 = kt * 32 + 16 * (lane >> 5);
+  int8_t sl[kNA][16];
+  const double qv = row < nq ? q[row] : 0.0;
+  double s = 0, c = 1, ds = 0, dc = 1;
+  if (qv != 0.0) {
+    const double turns = qv * bin / (2.0 * kPi);
+    sincos_2pi(((double)k0 + 0.5) * turns, &s, &c);
+    sincos_2pi(turns, &ds, &dc);
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const unsigned k = k0 + j;
+    double v = 0.0;
+    if (qv != 0.0 && k < nhist) v = s / (qv * (((double)k + 0.5) * bin) + 1e-8);
+    const double sn = fma(s, dc, c * ds);
+    c = fma(c, dc, -(s * ds));
+    s = sn;
+    v *= 64.0;
+#pragma unroll
+    for (int i = 0; i < kNA; ++i) {
+      const double a = rint(v);
+      sl[i][j] = (int8_t)a;
+      v = (v - a) * 128.0;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kNA; ++i) {
+    v4i w;
+    __builtin_memcpy(&w, sl[i], 16);
+    As[(size_t)i * total + tid] = w;
+  }
+}
+
+// One thread per (column tile, k tile, lane): 16 counts, NB 7-bit slices.
+__global__ void slice_counts(const unsigned long long *__restrict__ hist, unsigned ncols,
+                             unsigned nhist, unsigned NTn, unsigned KT, int nb,
+                             v4i *__restrict__ Bs) {
+  const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t total = (size_t)NTn * KT * 64;
+  if (tid >= total) return;
+  const unsigned lane = tid & 63;
+  const size_t tile = tid >> 6;
+  const unsigned kt = tile % KT;
+  const unsigned nt = tile / KT;
+  const unsigned col = nt * 32 + (lane & 31);
+  const unsigned k0 = kt * 32 + 16 * (lane >> 5);
+  unsigned long long cnt[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j)
+    cnt[j] = (col < ncols && k0 + j < nhist) ? hist[(size_t)col * nhist + k0 + j] : 0ull;
+  for (int s = 0; s < nb; ++s) {
+    int8_t b[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) b[j] = (int8_t)((cnt[j] >> (7 * s)) & 127ull);
+    v4i w;
+    __builtin_memcpy(&w, b, 16);
+    Bs[(size_t)s * total + tid] = w;
+  }
+}
+
+template <int NA, int NB, int NT>
+__global__ __launch_bounds__(256) void pairsum_mfma_kernel(const v4i *__restrict__ As,
+                                                           const v4i *__restrict__ Bs, unsigned MT,
+                                                           unsigned NTn, unsigned KT, unsigned nq,
+                                                           unsigned ncols, double *__restrict__ S) {
+  constexpr int DMIN = -(NA - 1);
+  constexpr int DMAX = NB - 1;
+  constexpr int NG = DMAX - DMIN + 1;
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned ngrp = NTn / NT;
+  unsigned tile = blockIdx.x;
+  if ((gridDim.x & 7u) == 0) tile = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  const unsigned mg = tile / ngrp, ng = tile % ngrp;
+  const unsigned mt = mg * 4 + wave;
+  const size_t sA = (size_t)MT * KT * 64, sB = (size_t)NTn * KT * 64;
+  const v4i *ap = As + (size_t)mt * KT * 64 + lane;
+  const v4i *bp = Bs + (size_t)ng * NT * KT * 64 + lane;
+
+  double out[NT][16];
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) out[t][e] = 0.0;
+
+  for (unsigned kb = 0; kb < KT; kb += kFlush) {
+    const unsigned ke = min(KT, kb + kFlush);
+    v16i acc[NG][NT];
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[g][t] = v16i{0};
+    for (unsigned kt = kb; kt < ke; ++kt) {
+      v4i a[NA], b[NB][NT];
+#pragma unroll
+      for (int i = 0; i < NA; ++i) a[i] = ap[i * sA + (size_t)kt * 64];
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) b[j][t] = bp[j * sB + ((size_t)t * KT + kt) * 64];
+#pragma unroll
+      for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+            acc[j - i - DMIN][t] =
+                __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j][t], acc[j - i - DMIN][t], 0, 0, 0);
+    }
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      const double sc = ldexp(1.0, 7 * (g + DMIN) - 6);
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) out[t][e] = fma((double)acc[g][t][e], sc, out[t][e]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const unsigned col = (ng * NT + t) * 32 + (lane & 31);
+    if (col >= ncols) continue;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const unsigned row = mt * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+      if (row < nq) S[(size_t)col * nq + row] = out[t][e];
+    }
+  }
+}
+
+template <int NT>
+static void launch_gemm(int nb, dim3 grid, hipStream_t st, const v4i *As, const v4i *Bs, unsigned MT,
+                        unsigned NTn, unsigned KT, unsigned nq, unsigned ncols, double *S) {
+  switch (nb) {
+    case 1:
+      hipLaunchKernelGGL((pairsum_mfma_kernel<kNA, 1, NT>), grid, dim3(256), 0, st, As, Bs, MT, NTn, KT,
+                         nq, ncols, S);
+      break;
+    case 2:
+      hipLaunchKernelGGL((pairsum_mfma_kernel<kNA, 2, NT>), grid, dim3(256), 0, st, As, Bs, MT, NTn, KT,
+                         nq, ncols, S);
+      break;
+    case 3:
+      hipLaunchKernelGGL((pairsum_mfma_kernel<kNA, 3, NT>), grid, dim3(256), 0, st, As, Bs, MT, NTn, KT,
+                         nq, ncols, S);
+      break;
+    default:
+      if constexpr (NT == 1) {
+        if (nb == 4)
+          hipLaunchKernelGGL((pairsum_mfma_kernel<kNA, 4, 1>), grid, dim3(256), 0, st, As, Bs, MT, NTn,
+                             KT, nq, ncols, S);
+        else
+          hipLaunchKernelGGL((pairsum_mfma_kernel<kNA, 5, 1>), grid, dim3(256), 0, st, As, Bs, MT, NTn,
+                             KT, nq, ncols, S);
+      }
+      break;
+  }
+}
+
+}  // namespace mm
+
+int pairsum_mfma(xs_ctx *ctx, const unsigned long long *hist, unsigned ncols, unsigned nhist,
+                 double bin, const double *q, unsigned nq, double *S) {
+  using namespace mm;
+  const unsigned MT = (nq + 127) / 128 * 4;
+  const unsigned KT = (nhist + 31) / 32;
+  const unsigned NTmax = (ncols + 63) / 64 * 2;  // column tiles for either NT
+  const size_t aBytes = (size_t)kNA * MT * KT * 1024, bBytes = (size_t)kMaxNB * NTmax * KT * 1024;
+  XS_TRY(ensure_matrix(ctx, Carver::need({aBytes, bBytes, 8})));
+  XS_TRY(ensure_pinned(ctx, 64));
+  Carver cv(ctx->mat);
+  v4i *As = cv.take<v4i>(aBytes / 16);
+  v4i *Bs = cv.take<v4i>(bBytes / 16);
+  unsigned long long *mx = cv.take<unsigned long long>(1);
+  // largest count -> number of count slices (one read back)
+  const size_t hn = (size_t)ncols * nhist;
+  hipLaunchKernelGGL(zero_u64, dim3(1), dim3(1), 0, ctx->stream, mx);
+  hipLaunchKernelGGL(max_u64, dim3((unsigned)std::min<size_t>((hn + 255) / 256, 2048)), dim3(256), 0,
+                     ctx->stream, hist, hn, mx);
+  XS_HIP_CHECK(hipGetLastError());
+  XS_HIP_CHECK(hipMemcpyAsync(ctx->pin, mx, 8, hipMemcpyDeviceToHost, ctx->stream));
+  XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  const unsigned long long cmax = *static_cast<unsigned long long *>(ctx->pin);
+  int nb = 1;
+  while (nb < kMaxNB && (cmax >> (7 * nb)) != 0ull) ++nb;
+  if ((cmax >> (7 * nb)) != 0ull)
+    return fail(XS_ERR_RANGE, "xs_hist_pairsum: a bin count exceeds 2^35 (matrix-core path)");
+  // two column tiles per wave unless the accumulators would not fit
+  const int NT = (ncols > 32 && nb <= 3) ? 2 : 1;
+  const unsigned NTn = (ncols + 32 * NT - 1) / (32 * NT) * NT;
+
+  hipEvent_t ev;
+  XS_TRY(prof_begin(ctx, "pairsum_slices", &ev));
+  const size_t ta = (size_t)MT * KT * 64, tb = (size_t)NTn * KT * 64;
+  hipLaunchKernelGGL(slice_sinc, dim3((unsigned)((ta + 255) / 256)), dim3(256), 0, ctx->stream, q, nq,
+                     nhist, bin, MT, KT, As);
+  hipLaunchKernelGGL(slice_counts, dim3((unsigned)((tb + 255) / 256)), dim3(256), 0, ctx->stream, hist,
+                     ncols, nhist, NTn, KT, nb, Bs);
+  XS_HIP_CHECK(hipGetLastError());
+  XS_TRY(prof_end(ctx, "pairsum_slices", ev));
+  XS_TRY(prof_begin(ctx, "pairsum_mfma", &ev));
+  const dim3 grid((MT / 4) * (NTn / NT));
+  if (NT == 2)
+    launch_gemm<2>(nb, grid, ctx->stream, As, Bs, MT, NTn, KT, nq, ncols, S);
+  else
+    launch_gemm<1>(nb, grid, ctx->stream, As, Bs, MT, NTn, KT, nq, ncols, S);
+  XS_HIP_CHECK(hipGetLastError());
+  XS_TRY(prof_end(ctx, "pairsum_mfma", ev));
+  return XS_OK;
+}
+
+}  // namespace xs

@@ -13,6 +13,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import subprocess
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -25,6 +26,8 @@ XS_OK = 0
 XS_DEVICE_PTRS = 0x1
 XS_F64_BINNING = 0x2
 XS_PDF_CPU_RADIUS = 0x4
+XS_BATCH_GRAPH = 0x8
+XS_PAIRSUM_MFMA = 0x10
 PDF_TYPES = {"RDF": 0, "PDF": 1, "rPDF": 2}
 
 _lib = None
@@ -60,15 +63,14 @@ def lib():
         L.xs_hist.argtypes = [vp, vp, sz, vp, u, d, u, vp, vp, u]
         L.xs_hist_pairsum.argtypes = [vp, vp, u, u, d, vp, u, vp, u]
         L.xs_pdf.argtypes = [vp, vp, u, vp, u, d, i, d, vp, vp, u]
-        for name in ("xs_debye_pairsum", "xs_partial_debye_pairsum", "xs_amp2d"):
-            if hasattr(L, name):
-                getattr(L, name).restype = i
-        if hasattr(L, "xs_debye_pairsum"):
-            L.xs_debye_pairsum.argtypes = [vp, vp, sz, vp, u, vp, u, vp, u]
-        if hasattr(L, "xs_partial_debye_pairsum"):
-            L.xs_partial_debye_pairsum.argtypes = [vp, vp, sz, vp, u, vp, u, vp, u, vp, u]
-        if hasattr(L, "xs_amp2d"):
-            L.xs_amp2d.argtypes = [vp, vp, sz, vp, u, vp, sz, vp, u, vp, vp, u]
+        L.xs_debye_pairsum.argtypes = [vp, vp, sz, vp, u, vp, u, vp, u]
+        L.xs_partial_debye_pairsum.argtypes = [vp, vp, sz, vp, u, vp, u, vp, u, vp, u]
+        L.xs_amp2d.argtypes = [vp, vp, sz, vp, u, vp, sz, vp, u, vp, vp, u]
+        L.xs_batch_plan.argtypes = [vp, u, vp, vp, vp, d, vp, u, C.POINTER(vp)]
+        L.xs_batch_info.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.xs_batch_run.argtypes = [vp, vp, vp, u, vp, i, vp, vp, vp, u]
+        L.xs_batch_overflow.argtypes = [vp, vp]
+        L.xs_batch_free.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -102,9 +104,13 @@ class Context:
 
     def __init__(self, device: int = 0):
         self.h = C.c_void_p()
+        self._batches = weakref.WeakSet()
         _check(lib().xs_init(int(device), C.byref(self.h)))
 
     def close(self):
+        """Free the context; batch plans made on it are freed first."""
+        for b in list(self._batches):
+            b.close()
         if self.h:
             lib().xs_free(self.h)
             self.h = C.c_void_p()
@@ -158,18 +164,20 @@ class Context:
                              C.c_void_p(ovf_ptr) if ovf_ptr else None, flags))
 
     # ---------------------------------------------------------------- H3
-    def hist_pairsum(self, hist, bin_: float, q):
+    def hist_pairsum(self, hist, bin_: float, q, mfma: bool = False):
+        """mfma=True: the int8 matrix-core product (many histograms)."""
         hist = np.ascontiguousarray(hist, dtype=np.uint64)
         q = np.ascontiguousarray(q, dtype=np.float64)
         nt, nh = hist.shape
         S = np.zeros(nt * q.size)
-        _check(lib().xs_hist_pairsum(self.h, _p(hist), nt, nh, float(bin_), _p(q), q.size, _p(S), 0))
+        _check(lib().xs_hist_pairsum(self.h, _p(hist), nt, nh, float(bin_), _p(q), q.size, _p(S),
+                                     XS_PAIRSUM_MFMA if mfma else 0))
         return S.reshape(nt, q.size)
 
-    def hist_pairsum_dev(self, hist_ptr, ntypes_, nhist, bin_, q_ptr, nq, S_ptr):
+    def hist_pairsum_dev(self, hist_ptr, ntypes_, nhist, bin_, q_ptr, nq, S_ptr, mfma=False):
         _check(lib().xs_hist_pairsum(self.h, C.c_void_p(hist_ptr), int(ntypes_), int(nhist),
                                      float(bin_), C.c_void_p(q_ptr), int(nq), C.c_void_p(S_ptr),
-                                     XS_DEVICE_PTRS))
+                                     XS_DEVICE_PTRS | (XS_PAIRSUM_MFMA if mfma else 0)))
 
     # ---------------------------------------------------------------- H4
     def pdf(self, hist, el_count, bin_, pdf_type, rho0, sl, cpu_radius=False):
@@ -233,6 +241,86 @@ class Context:
         _check(lib().xs_amp2d(self.h, C.c_void_p(xyz_ptr), int(n), _p(ec), len(ec),
                               C.c_void_p(qvec_ptr), int(nqp), C.c_void_p(rot_ptr), int(nrot),
                               C.c_void_p(ff_ptr), C.c_void_p(I_ptr), XS_DEVICE_PTRS))
+
+
+class Batch:
+    """A batch plan (xs_batch_plan) over an ensemble of structures.
+
+    ``structures`` is a list of (xyz (3, N) array, el_count) pairs, each
+    element-major.  ``f64`` bins in double as the reference CPU path."""
+
+    def __init__(self, ctx: Context, structures, bin_: float, nhist=None, f64: bool = True):
+        self.ctx = ctx
+        self.bin = float(bin_)
+        n_el = _u32([len(ec) for _, ec in structures])
+        ecs = _u32(np.concatenate([np.asarray(ec, dtype=np.uint32) for _, ec in structures]))
+        self._xyz = [np.ascontiguousarray(x, dtype=np.float64) for x, _ in structures]
+        ptrs = (C.c_void_p * len(structures))(*[x.ctypes.data for x in self._xyz])
+        nh = None if nhist is None else _u32(nhist)
+        self.h = C.c_void_p()
+        _check(lib().xs_batch_plan(ctx.h, len(structures), _p(n_el), _p(ecs), ptrs, self.bin,
+                                   _p(nh), XS_F64_BINNING if f64 else 0, C.byref(self.h)))
+        self._xyz = None
+        ctx._batches.add(self)
+        self.nstruct = len(structures)
+        self.n_el = [int(v) for v in n_el]
+        self.el_count = ecs
+        ns = self.nstruct
+        self.nhist = np.zeros(ns, dtype=np.uint32)
+        self.hist_off = np.zeros(ns, dtype=np.uint64)
+        self.pdf_off = np.zeros(ns, dtype=np.uint64)
+        self.type_first = np.zeros(ns, dtype=np.uint64)
+        tot = np.zeros(3, dtype=np.uint64)
+        _check(lib().xs_batch_info(self.h, _p(self.nhist), _p(self.hist_off), _p(self.pdf_off),
+                                   _p(self.type_first), _p(tot)))
+        self.hist_len, self.pdf_len, self.ntypes_total = (int(v) for v in tot)
+
+    def close(self):
+        if self.h:
+            lib().xs_batch_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, q=None, pdf_type=None, rho0=None, sl=None, cpu_radius=True, graph=False):
+        """Host arrays in and out: (hist, S or None, pdf or None) as flat arrays."""
+        hist = np.zeros(self.hist_len, dtype=np.uint64)
+        S = pdf = None
+        qa = None
+        if q is not None:
+            qa = np.ascontiguousarray(q, dtype=np.float64)
+            S = np.zeros(self.ntypes_total * qa.size)
+        t = -1
+        rho = sla = None
+        if pdf_type is not None:
+            t = PDF_TYPES.get(pdf_type, pdf_type)
+            pdf = np.zeros(self.pdf_len)
+            sla = np.ascontiguousarray(sl, dtype=np.float64)
+            rho = None if rho0 is None else np.ascontiguousarray(rho0, dtype=np.float64)
+        flags = (XS_PDF_CPU_RADIUS if cpu_radius else 0) | (XS_BATCH_GRAPH if graph else 0)
+        _check(lib().xs_batch_run(self.h, _p(hist), _p(qa), 0 if qa is None else qa.size, _p(S),
+                                  int(t), _p(rho), _p(sla), _p(pdf), flags))
+        return hist, S, pdf
+
+    def run_dev(self, hist_ptr, q_ptr=None, nq=0, S_ptr=None, pdf_type=None, rho0=None, sl=None,
+                pdf_ptr=None, cpu_radius=True, graph=False):
+        t = -1 if pdf_type is None else PDF_TYPES.get(pdf_type, pdf_type)
+        sla = None if sl is None else np.ascontiguousarray(sl, dtype=np.float64)
+        rho = None if rho0 is None else np.ascontiguousarray(rho0, dtype=np.float64)
+        flags = XS_DEVICE_PTRS | (XS_PDF_CPU_RADIUS if cpu_radius else 0) | \
+            (XS_BATCH_GRAPH if graph else 0)
+        vp = lambda v: C.c_void_p(v) if v else None  # noqa: E731
+        _check(lib().xs_batch_run(self.h, vp(hist_ptr), vp(q_ptr), int(nq), vp(S_ptr), int(t),
+                                  _p(rho), _p(sla), vp(pdf_ptr), flags))
+
+    def overflow(self):
+        o = np.zeros(self.nstruct, dtype=np.uint64)
+        _check(lib().xs_batch_overflow(self.h, _p(o)))
+        return o
 
 
 def device_count() -> int:

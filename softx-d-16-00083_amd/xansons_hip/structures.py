@@ -176,3 +176,72 @@ def random_cluster(n_per_el: Sequence[int], box: float, min_dist: float,
         o += c
     names = list(names) if names else [f"E{e}" for e in range(len(n_per_el))]
     return _concat(parts, names)
+
+
+def particle(n_target: int, n_el: int, shape: str, a: float, sigma: float,
+             rng: np.random.Generator) -> Ensemble:
+    """One nanoparticle of about n_target atoms.
+
+    shape: "fcc" / "bcc" (sphere cut from the lattice), "cube" (fcc cube) or
+    "core-shell" (fcc sphere, element 0 inside 60% of the radius, the other
+    elements drawn at random outside it).  Species of the non-core shapes are
+    drawn with Dirichlet(1) fractions.  Gaussian jitter sigma per coordinate."""
+    if shape == "bcc":
+        r = (3.0 * n_target * a ** 3 / (8.0 * np.pi)) ** (1.0 / 3.0)
+        s = bcc_sites(a, int(np.ceil(r / a)) + 1)
+        s = s[np.einsum("ij,ij->i", s, s) <= r * r]
+    elif shape == "cube":
+        half = 0.5 * (n_target * a ** 3 / 4.0) ** (1.0 / 3.0)
+        s = fcc_sites(a, int(np.ceil(half / a)) + 1)
+        s = s[np.all(np.abs(s) <= half, axis=1)]
+    else:
+        r = (3.0 * n_target * a ** 3 / (16.0 * np.pi)) ** (1.0 / 3.0)
+        s = fcc_sites(a, int(np.ceil(r / a)) + 1)
+        s = s[np.einsum("ij,ij->i", s, s) <= r * r]
+    s = s + rng.normal(0.0, sigma, size=s.shape)
+    if shape == "core-shell" and n_el > 1:
+        rr = np.sqrt(np.einsum("ij,ij->i", s, s))
+        core = rr <= 0.6 * rr.max()
+        species = np.where(core, 0, 1 + rng.integers(0, n_el - 1, size=s.shape[0]))
+    else:
+        frac = rng.dirichlet(np.ones(n_el))
+        species = np.minimum(np.searchsorted(np.cumsum(frac), rng.random(s.shape[0]), side="right"),
+                             n_el - 1)
+    parts = [s[species == e] for e in range(n_el)]
+    if any(p.shape[0] == 0 for p in parts):     # keep every element present
+        order = rng.permutation(s.shape[0])
+        species = np.arange(s.shape[0])[order] % n_el
+        parts = [s[species == e] for e in range(n_el)]
+    return _concat(parts, [f"E{e}" for e in range(n_el)])
+
+
+def particle_ensemble(count: int = 1000, seed: int = 9, nmin: int = 500,
+                      nmax: int = 200000) -> List[Ensemble]:
+    """An ensemble of nanoparticles: N log-uniform in [nmin, nmax], 1-4
+    elements, shapes fcc/bcc sphere, fcc cube and core-shell, sigma = 0.05 A
+    (workload: 1,000 particles of 500-200,000 atoms)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    shapes = ("fcc", "bcc", "cube", "core-shell")
+    out = []
+    for _ in range(count):
+        n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        n_el = int(rng.integers(1, 5))
+        shape = shapes[int(rng.integers(0, 4))]
+        a = float(rng.uniform(2.8, 3.4) if shape == "bcc" else rng.uniform(3.5, 4.2))
+        out.append(particle(n, n_el, shape, a, 0.05, rng))
+    return out
+
+
+def c4_particles(count: int = 256, seed: int = 11, nmin: int = 500,
+                 nmax: int = 8000) -> List[Ensemble]:
+    """Two-element fcc and core-shell spheres, N log-uniform in [nmin, nmax],
+    a in [3.6, 4.1] A, so every interatomic distance stays below 65.5 A
+    (workload: 256 particles x 3 pair types on one 16,384-bin grid at
+    0.004 A)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(count):
+        n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        shape = ("fcc", "core-shell")[int(rng.integers(0, 2))]
+        out.append(particle(n, 2, shape, float(rng.uniform(3.6, 4.1)), 0.05, rng))
+    return out

@@ -35,3 +35,16 @@ def test_bad_device_id_rejected():
     h = C.c_void_p()
     rc = xh.lib().xs_init(10_000, C.byref(h))
     assert rc == -4
+
+
+def test_batch_entry_points_reject_null_handles():
+    L = xh.lib()
+    plan = C.c_void_p()
+    one = (C.c_uint * 1)(1)
+    assert L.xs_batch_plan(None, 1, one, one, None, 0.01, None, 0, C.byref(plan)) == -1
+    assert not plan
+    assert L.xs_batch_run(None, None, None, 0, None, -1, None, None, None, 0) == -1
+    assert b"null plan" in L.xs_last_error()
+    assert L.xs_batch_info(None, None, None, None, None, None) == -1
+    assert L.xs_batch_overflow(None, None) == -1
+    assert L.xs_batch_free(None) == 0

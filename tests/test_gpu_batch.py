@@ -1,0 +1,142 @@
+"""Ensemble runs (xs_batch_plan / xs_batch_run): every structure's histograms
+equal the oracle's bit for bit (FP64 and FP32 binning), PDFs equal the
+oracle's bit for bit, intensities from the pair sums lie within 1e-5 of the
+FP64 oracle, and graph replays reproduce the eager run."""
+import numpy as np
+import pytest
+
+import xs_oracle as O
+from conftest import rel_err
+from xansons_hip import structures as st
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+BIN = 0.01
+
+
+@pytest.fixture(scope="module")
+def ensemble():
+    # 9 particles of 300-6000 atoms (1-4 elements; both histogram window
+    # classes: nhist below and above 8192) plus a two-atom structure
+    ens = st.particle_ensemble(9, seed=21, nmin=300, nmax=6000)
+    two = st.Ensemble(np.array([[0.0, 1.234], [0.0, 0.0], [0.0, 0.0]], dtype=np.float32), [1, 1],
+                      ["A", "B"])
+    return ens + [two]
+
+
+@pytest.fixture(scope="module")
+def bctx():
+    import xansons_hip as xh
+    c = xh.Context(0)
+    yield c
+    c.close()
+
+
+def sl_for(ens):
+    return np.concatenate([np.linspace(2.0, 9.0, e.n_el) for e in ens])
+
+
+def rho_for(ens):
+    return np.array([O.atomic_density(e.xyz64()) for e in ens])
+
+
+def split(b, ens, hist, S, pdf, nq):
+    out = []
+    for s, e in enumerate(ens):
+        P = e.n_el * (e.n_el + 1) // 2
+        nh = int(b.nhist[s])
+        h = hist[int(b.hist_off[s]):int(b.hist_off[s]) + P * nh].reshape(P, nh)
+        Ss = None if S is None else S[int(b.type_first[s]) * nq:(int(b.type_first[s]) + P) * nq]
+        ps = None if pdf is None else pdf[int(b.pdf_off[s]):int(b.pdf_off[s]) + (1 + P) * nh]
+        out.append((h, None if Ss is None else Ss.reshape(P, nq),
+                    None if ps is None else ps.reshape(1 + P, nh)))
+    return out
+
+
+def test_batch_f64_matches_oracle(bctx, ensemble):
+    import xansons_hip as xh
+    q = st.q_mesh(0.5, 20.0, 600)
+    sl, rho = sl_for(ensemble), rho_for(ensemble)
+    b = xh.Batch(bctx, [(e.xyz64(), e.el_count) for e in ensemble], BIN, f64=True)
+    assert max(b.nhist) > 8192 >= min(b.nhist)
+    hist, S, pdf = b.run(q=q, pdf_type="rPDF", rho0=rho, sl=sl)
+    assert not b.overflow().any()
+    so = 0
+    for s, (e, (h, Ss, ps)) in enumerate(zip(ensemble, split(b, ensemble, hist, S, pdf, q.size))):
+        x64 = e.xyz64()
+        nh = O.hist_size(x64, BIN)
+        assert int(b.nhist[s]) == nh
+        ho, ovf = O.hist_f64(x64, e.el_count, BIN, nh)
+        assert ovf == 0
+        mism = int(np.count_nonzero(h != ho))
+        print(f"struct {s}: N={e.n} n_el={e.n_el} nhist={nh} mismatches={mism}", flush=True)
+        assert mism == 0
+        sle = sl[so:so + e.n_el]
+        so += e.n_el
+        po = O.pdf(ho, e.el_count, BIN, xh.PDF_TYPES["rPDF"], rho[s], sle, cpu_radius_offset=True)
+        assert np.array_equal(ps, po)
+        Io = O.intensity_1d(O.hist_pairsum(ho, BIN, q), e.el_count, q, sl=sle)
+        Ig = O.intensity_1d(Ss, e.el_count, q, sl=sle)
+        pw, l2 = rel_err(Ig, Io)
+        assert pw <= TOL and l2 <= TOL, (s, pw, l2)
+    b.close()
+
+
+def test_batch_f32_matches_oracle(bctx, ensemble):
+    import xansons_hip as xh
+    b = xh.Batch(bctx, [(e.xyz, e.el_count) for e in ensemble], BIN, f64=False)
+    hist, _, _ = b.run()
+    for s, (e, (h, _, _)) in enumerate(zip(ensemble, split(b, ensemble, hist, None, None, 0))):
+        ho, _ = O.hist_f32(e.xyz, e.el_count, BIN, int(b.nhist[s]))
+        assert int(np.count_nonzero(h != ho)) == 0, s
+    b.close()
+
+
+def test_batch_given_nhist_and_overflow(bctx, ensemble):
+    import xansons_hip as xh
+    sub = ensemble[:4]
+    nh = [max(50, O.hist_size(e.xyz64(), BIN) // 3) for e in sub]
+    b = xh.Batch(bctx, [(e.xyz64(), e.el_count) for e in sub], BIN, nhist=nh, f64=True)
+    assert list(b.nhist) == nh
+    hist, _, _ = b.run()
+    ov = b.overflow()
+    for s, (e, (h, _, _)) in enumerate(zip(sub, split(b, sub, hist, None, None, 0))):
+        ho, ovo = O.hist_f64(e.xyz64(), e.el_count, BIN, nh[s])
+        assert np.array_equal(h, ho)
+        assert int(ov[s]) == ovo > 0
+    b.close()
+
+
+def test_batch_graph_replays_match_eager(bctx, ensemble):
+    import torch
+    import xansons_hip as xh
+    dev = torch.device("cuda", 0)
+    q = st.q_mesh(0.5, 20.0, 512)
+    sl, rho = sl_for(ensemble), rho_for(ensemble)
+    b = xh.Batch(bctx, [(e.xyz64(), e.el_count) for e in ensemble], BIN, f64=True)
+    h_ref, S_ref, p_ref = b.run(q=q, pdf_type="PDF", rho0=rho, sl=sl)
+    qd = torch.from_numpy(q).to(dev)
+    hist = torch.empty(b.hist_len, dtype=torch.int64, device=dev)
+    S = torch.empty(b.ntypes_total * q.size, dtype=torch.float64, device=dev)
+    pdf = torch.empty(b.pdf_len, dtype=torch.float64, device=dev)
+    h2 = torch.empty_like(hist)
+    for rep in range(4):
+        hist.fill_(-1)
+        S.fill_(float("nan"))
+        pdf.fill_(-1.0)
+        torch.cuda.synchronize()
+        b.run_dev(hist.data_ptr(), qd.data_ptr(), q.size, S.data_ptr(), "PDF", rho, sl, pdf.data_ptr(),
+                  graph=True)
+        bctx.synchronize()
+        hh = hist.cpu().numpy().view(np.uint64)
+        assert np.array_equal(hh, h_ref), rep
+        assert np.array_equal(pdf.cpu().numpy(), p_ref, equal_nan=True), rep  # rho0 = 0: 0/0
+        Sg = S.cpu().numpy()
+        assert np.allclose(Sg, S_ref, rtol=1e-12, atol=1e-9 * np.abs(S_ref).max()), rep
+        # eager work of the program between replays: another (non-graph) run
+        b.run_dev(h2.data_ptr())
+        bctx.synchronize()
+        assert np.array_equal(h2.cpu().numpy().view(np.uint64), h_ref)
+        print(f"replay {rep}: ok", flush=True)
+    b.close()
