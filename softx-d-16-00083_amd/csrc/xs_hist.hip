@@ -287,6 +287,96 @@ __device__ __forceinline__ uint32_t bin_index<float>(float s, float bin, double 
   return t < (float)nhist ? (uint32_t)t : nhist;
 }
 
+// Fast part of bin_index: the estimate's floor and whether it may differ
+// from the IEEE result (then exact_bin decides).
+template <class T>
+__device__ __forceinline__ uint32_t fast_bin(T s, T bin, double inv_bin, double band,
+                                             uint32_t nhist, bool &bad);
+
+template <>
+__device__ __forceinline__ uint32_t fast_bin<double>(double s, double bin, double inv_bin,
+                                                     double band, uint32_t nhist, bool &bad) {
+  (void)bin;
+  (void)nhist;
+  const double y = __builtin_amdgcn_rsq(s);
+  double g = s * y;
+  g = fma(g, fma(-g, 0.5 * y, 0.5), g);
+  const double kd = g * inv_bin;
+  bad = !(fabs(__builtin_amdgcn_fract(kd) - 0.5) < band);
+  return (uint32_t)kd;
+}
+
+template <>
+__device__ __forceinline__ uint32_t fast_bin<float>(float s, float bin, double inv_bin, double band,
+                                                    uint32_t nhist, bool &bad) {
+  bad = false;
+  return bin_index<float>(s, bin, inv_bin, band, nhist);
+}
+
+template <class T>
+__device__ __forceinline__ uint32_t exact_bin(T s, T bin, uint32_t nhist) {
+  const T t = (T)__builtin_sqrt((double)s) / bin;
+  return t < (T)nhist ? (uint32_t)t : nhist;
+}
+
+template <>
+__device__ __forceinline__ uint32_t exact_bin<float>(float s, float bin, uint32_t nhist) {
+  const float t = __builtin_sqrtf(s) / bin;
+  return t < (float)nhist ? (uint32_t)t : nhist;
+}
+
+// Groups of 8 J atoms (not the like-element diagonal tile): 8 independent
+// distance chains, then one wave-uniform test each for the rare exact-binning
+// and out-of-window cases, so the common path has no per-pair branch.
+template <class T, int WCAP>
+__device__ __forceinline__ void pair_group8(T xi, T yi, T zi, const T *sx, const T *sy,
+                                            const T *sz, uint32_t ng, T bin, double inv_bin,
+                                            double band, uint32_t nhist, uint32_t B, uint32_t wlim,
+                                            uint32_t *win, unsigned long long *__restrict__ hp,
+                                            unsigned long long *__restrict__ ovf) {
+#pragma clang fp contract(off)
+  for (uint32_t j0 = 0; j0 < ng; j0 += 8) {
+    T s[8];
+    uint32_t k[8];
+    bool bad = false;
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      const T dx = xi - sx[j0 + p];
+      const T dy = yi - sy[j0 + p];
+      const T dz = zi - sz[j0 + p];
+      T t = dx * dx + dy * dy;
+      s[p] = t + dz * dz;
+    }
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      bool b;
+      k[p] = fast_bin<T>(s[p], bin, inv_bin, band, nhist, b);
+      bad |= b;
+    }
+    if (__builtin_expect(bad, 0)) {
+#pragma unroll
+      for (int p = 0; p < 8; ++p) k[p] = exact_bin<T>(s[p], bin, nhist);
+    }
+    bool out = false;
+#pragma unroll
+    for (int p = 0; p < 8; ++p) out |= (k[p] - B >= wlim);
+    if (__builtin_expect(!out, 1)) {
+#pragma unroll
+      for (int p = 0; p < 8; ++p) atomicAdd(&win[k[p] & (WCAP - 1)], 1u);
+    } else {
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        if (k[p] - B < wlim)
+          atomicAdd(&win[k[p] & (WCAP - 1)], 1u);
+        else if (k[p] < nhist)
+          atomicAdd(&hp[k[p]], 1ull);
+        else
+          atomicAdd(ovf, 1ull);
+      }
+    }
+  }
+}
+
 template <class T, int WCAP, bool DIAG>
 __device__ __forceinline__ void pair_block(T xi, T yi, T zi, uint32_t il, const T *sx,
                                            const T *sy, const T *sz, uint32_t cn,
@@ -399,12 +489,16 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
       }
       __syncthreads();
       if (ivalid) {
-        if (diag)
+        if (diag) {
           pair_block<T, WCAP, true>(xi, yi, zi, tid, sx, sy, sz, cn, s0, bin, td.inv_bin, td.band,
                                     nhist, B, wlim, win, hp, ovf);
-        else
-          pair_block<T, WCAP, false>(xi, yi, zi, tid, sx, sy, sz, cn, s0, bin, td.inv_bin, td.band,
-                                     nhist, B, wlim, win, hp, ovf);
+        } else {
+          const uint32_t ng = cn & ~7u;
+          pair_group8<T, WCAP>(xi, yi, zi, sx, sy, sz, ng, bin, td.inv_bin, td.band, nhist, B, wlim,
+                               win, hp, ovf);
+          pair_block<T, WCAP, false>(xi, yi, zi, tid, sx + ng, sy + ng, sz + ng, cn - ng, s0 + ng, bin,
+                                     td.inv_bin, td.band, nhist, B, wlim, win, hp, ovf);
+        }
       }
       __syncthreads();
     }
@@ -743,6 +837,7 @@ static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_coun
   order_items(items, types, tiles);
 
   const size_t nt = std::max<size_t>(tiles.size(), 1), ni = std::max<size_t>(items.size(), 1);
+  XS_TRY(ensure_pinned(ctx, 64));
   XS_TRY(ensure_workspace(ctx, Carver::need({3 * n * sizeof(T), nt * sizeof(Tile), nt * 48,
                                              ntypes * sizeof(TypeDesc), ni * sizeof(Item),
                                              hsize * 8, 8})));
