@@ -353,6 +353,73 @@ def pairsum_matrix_leg(ctx, steps):
     }
 
 
+def amp2d_leg(ctx, nrot=16):
+    """2D pattern throughput: the 50,000-atom three-element disordered solid,
+    a 512 x 512 Cartesian q plane (|q_x|, |q_y| <= 8 A^-1, q_z = 0) and
+    `nrot` of the 256 seeded uniform rotations; complex-exponential terms per
+    second.  Checks: Friedel symmetry I(q) = I(-q) over the plane, and a
+    sample of 64 q points x 2 rotations against an FP64 numpy sum."""
+    import torch
+    ens = st.disordered_cubic()
+    x64 = ens.xyz64()
+    qv = st.q_plane(8.0, 512)
+    nqp = qv.shape[0]
+    rot = st.uniform_rotations(256, seed=8)[:nrot].reshape(nrot, 9)
+    ff = st.gaussian_ff(np.linalg.norm(qv, axis=1), st.gaussian_ff_coeffs(3, seed=9))
+    dev = torch.device("cuda", 0)
+    xd = torch.from_numpy(x64).to(dev)
+    qd = torch.from_numpy(qv).to(dev)
+    rd = torch.from_numpy(np.ascontiguousarray(rot)).to(dev)
+    fd = torch.from_numpy(np.ascontiguousarray(ff)).to(dev)
+    Id = torch.empty(nqp, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def run():
+        ctx.amp2d_dev(xd.data_ptr(), ens.n, ens.el_count, qd.data_ptr(), nqp, rd.data_ptr(), nrot,
+                      fd.data_ptr(), Id.data_ptr())
+
+    run()
+    ctx.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    run()
+    ctx.synchronize()
+    t = time.perf_counter() - t0
+    kms = ctx.profile_read("amp2d")[0]
+    ctx.profile(False)
+    I = Id.cpu().numpy()
+    fried = float(np.max(np.abs(I - I[::-1]) / np.maximum(np.abs(I), 1e-300)))
+    # FP64 numpy evaluation of 64 q points x 2 rotations
+    idx = np.random.Generator(np.random.PCG64(1)).choice(nqp, 64, replace=False)
+    Ie = np.zeros(64)
+    for r in range(2):
+        qr = qv[idx] @ rot[r].reshape(3, 3)          # (R^T q) . x = q . (R x)
+        amp = np.zeros(64, dtype=np.complex128)
+        off = 0
+        for e, c in enumerate(ens.el_count):
+            ph = qr @ x64[:, off:off + c]
+            amp += ff[e, idx] * np.exp(1j * ph).sum(axis=1)
+            off += c
+        Ie += np.abs(amp) ** 2
+    Ig = torch.empty(64, dtype=torch.float64, device=dev)
+    qs = torch.from_numpy(np.ascontiguousarray(qv[idx])).to(dev)
+    fs = torch.from_numpy(np.ascontiguousarray(ff[:, idx])).to(dev)
+    ctx.amp2d_dev(xd.data_ptr(), ens.n, ens.el_count, qs.data_ptr(), 64, rd.data_ptr(), 2,
+                  fs.data_ptr(), Ig.data_ptr())
+    ctx.synchronize()
+    err = float(np.max(np.abs(Ig.cpu().numpy() - Ie) / Ie))
+    terms = ens.n * nqp * nrot
+    return {
+        "workload": f"{ens.n}-atom 3-element disordered solid, 512 x 512 q plane (|q| components <= 8), "
+                    f"{nrot} of 256 seeded uniform rotations",
+        "terms": terms, "ms": round(t * 1e3, 2), "kernel_ms": round(kms, 2),
+        "gterms_per_s": round(terms / (kms * 1e-3) / 1e9, 1),
+        "full_256_rotations_s_extrapolated": round(kms * 1e-3 * 256 / nrot, 3),
+        "checks": {"friedel_max_rel": fried, "fp64_sample_max_rel": err,
+                   "fp64_sample": "64 q points x 2 rotations, FP64 numpy sum"},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -361,6 +428,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ensemble", type=int, default=1000,
                     help="particles in the batched-ensemble leg (0 skips it)")
+    ap.add_argument("--no-2d", action="store_true", help="skip the 2D pattern leg")
     ap.add_argument("--no-matrix", action="store_true",
                     help="skip the many-histogram pair-sum leg (FP64 vector vs int8 matrix cores)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -378,7 +446,7 @@ def main():
     import xansons_hip as xh
 
     ens = st.rocksalt_cube()
-    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    # bin count as GetHistSize (CalcFunctions.cpp:224-243): bounding-box diagonal / bin + 1
     nh = int((np.linalg.norm(ens.xyz64().max(axis=1) - ens.xyz64().min(axis=1)) / BIN)) + 1
     na, nb = ens.el_count
     pairs = ens.n * (ens.n - 1) // 2
@@ -429,6 +497,8 @@ def main():
     }
     if not args.no_matrix:
         line["pairsum_matrix"] = pairsum_matrix_leg(ctx, max(1, args.steps))
+    if not args.no_2d:
+        line["pattern_2d"] = amp2d_leg(ctx)
     if args.ensemble > 0:
         del hist, ovf
         torch.cuda.empty_cache()

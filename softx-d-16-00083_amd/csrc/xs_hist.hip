@@ -218,8 +218,11 @@ __global__ void fill_u64(unsigned long long *__restrict__ p, size_t n) {
 // ---------------------------------------------------------------- histogram kernel
 
 // Conservative bin range [kl, ku] of all pairs between two boxes.
+// Conservative bin range [kl, ku] of all pairs between two boxes (one bin of
+// margin each side).  fits: no pair can reach nhist (ku was not clamped).
 __device__ __forceinline__ void box_bins(const double *__restrict__ bi, const double *__restrict__ bj,
-                                         double bin, uint32_t nhist, uint32_t &kl, uint32_t &ku) {
+                                         double bin, uint32_t nhist, uint32_t &kl, uint32_t &ku,
+                                         bool &fits) {
   double lo2 = 0.0, hi2 = 0.0;
   for (int a = 0; a < 3; ++a) {
     const double gap = fmax(fmax(bj[a] - bi[3 + a], bi[a] - bj[3 + a]), 0.0);
@@ -230,6 +233,7 @@ __device__ __forceinline__ void box_bins(const double *__restrict__ bi, const do
   const double top = (double)(nhist - 1);
   double fl = floor(sqrt(lo2) / bin) - 1.0;
   double fh = floor(sqrt(hi2) / bin) + 1.0;
+  fits = fh <= top;
   fl = fl < 0.0 ? 0.0 : (fl > top ? top : fl);
   fh = fh > top ? top : (fh < fl ? fl : fh);
   kl = (uint32_t)fl;
@@ -326,41 +330,55 @@ __device__ __forceinline__ uint32_t exact_bin<float>(float s, float bin, uint32_
 }
 
 // Groups of 8 J atoms (not the like-element diagonal tile): 8 independent
-// distance chains, then one wave-uniform test each for the rare exact-binning
-// and out-of-window cases, so the common path has no per-pair branch.
-template <class T, int WCAP>
+// distance chains and no per-pair branch on the common path.
+//  * FP64: the estimate takes s with fused multiply-adds (it only has to lie
+//    within the band of the IEEE value); a pair whose estimate is too close
+//    to a bin edge recomputes s and its bin exactly as the reference does.
+//  * SAFE: the J tile's bin range lies inside the window and below nhist
+//    (kl >= B, ku < B + WCAP, ku < nhist from the tile boxes), so every pair
+//    goes to the window unchecked; otherwise each pair is tested.
+template <class T, int WCAP, bool SAFE>
 __device__ __forceinline__ void pair_group8(T xi, T yi, T zi, const T *sx, const T *sy,
                                             const T *sz, uint32_t ng, T bin, double inv_bin,
                                             double band, uint32_t nhist, uint32_t B, uint32_t wlim,
                                             uint32_t *win, unsigned long long *__restrict__ hp,
                                             unsigned long long *__restrict__ ovf) {
 #pragma clang fp contract(off)
+  constexpr bool F64 = sizeof(T) == 8;
   for (uint32_t j0 = 0; j0 < ng; j0 += 8) {
-    T s[8];
     uint32_t k[8];
-    bool bad = false;
+    bool bad[8], any = false;
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
       const T dx = xi - sx[j0 + p];
       const T dy = yi - sy[j0 + p];
       const T dz = zi - sz[j0 + p];
-      T t = dx * dx + dy * dy;
-      s[p] = t + dz * dz;
+      T s;
+      if constexpr (F64) {
+        s = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+      } else {
+        s = dx * dx + dy * dy;
+        s = s + dz * dz;
+      }
+      k[p] = fast_bin<T>(s, bin, inv_bin, band, nhist, bad[p]);
+      any |= bad[p];
     }
+    if constexpr (F64) {
+      if (__builtin_expect(any, 0)) {
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
-      bool b;
-      k[p] = fast_bin<T>(s[p], bin, inv_bin, band, nhist, b);
-      bad |= b;
+        for (int p = 0; p < 8; ++p) {
+          if (bad[p]) {
+            const T dx = xi - sx[j0 + p];
+            const T dy = yi - sy[j0 + p];
+            const T dz = zi - sz[j0 + p];
+            T s = dx * dx + dy * dy;
+            s = s + dz * dz;
+            k[p] = exact_bin<T>(s, bin, nhist);
+          }
+        }
+      }
     }
-    if (__builtin_expect(bad, 0)) {
-#pragma unroll
-      for (int p = 0; p < 8; ++p) k[p] = exact_bin<T>(s[p], bin, nhist);
-    }
-    bool out = false;
-#pragma unroll
-    for (int p = 0; p < 8; ++p) out |= (k[p] - B >= wlim);
-    if (__builtin_expect(!out, 1)) {
+    if constexpr (SAFE) {
 #pragma unroll
       for (int p = 0; p < 8; ++p) atomicAdd(&win[k[p] & (WCAP - 1)], 1u);
     } else {
@@ -437,7 +455,8 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
     if (m < n) {
       const uint32_t tj = (it.jstart + m) % td.b_n;
       uint32_t kl, ku;
-      box_bins(bi, A.boxes + 6 * (size_t)(td.b_base + tj), td.bin, nhist, kl, ku);
+      bool fits;
+      box_bins(bi, A.boxes + 6 * (size_t)(td.b_base + tj), td.bin, nhist, kl, ku, fits);
       key = ((unsigned long long)kl << 32) | m;
     }
     keys[m] = key;
@@ -467,9 +486,11 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
     const uint32_t tj = (it.jstart + m) % td.b_n;
     const uint32_t tjg = td.b_base + tj;
     uint32_t kl, ku;
-    box_bins(bi, A.boxes + 6 * (size_t)tjg, td.bin, nhist, kl, ku);
+    bool fits;
+    box_bins(bi, A.boxes + 6 * (size_t)tjg, td.bin, nhist, kl, ku, fits);
     uint32_t wlim = 0;
-    if (ku - kl + 1 <= (uint32_t)WCAP) {
+    const bool span_ok = ku - kl + 1 <= (uint32_t)WCAP;
+    if (span_ok) {
       const uint32_t nb = ku + 1 > (uint32_t)WCAP ? ku + 1 - WCAP : 0u;
       if (nb > B) {
         flush_window<WCAP>(win, B, min(min(nb, B + (uint32_t)WCAP), nhist), hp);
@@ -494,8 +515,12 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
                                     nhist, B, wlim, win, hp, ovf);
         } else {
           const uint32_t ng = cn & ~7u;
-          pair_group8<T, WCAP>(xi, yi, zi, sx, sy, sz, ng, bin, td.inv_bin, td.band, nhist, B, wlim,
-                               win, hp, ovf);
+          if (span_ok && fits)
+            pair_group8<T, WCAP, true>(xi, yi, zi, sx, sy, sz, ng, bin, td.inv_bin, td.band, nhist, B,
+                                       wlim, win, hp, ovf);
+          else
+            pair_group8<T, WCAP, false>(xi, yi, zi, sx, sy, sz, ng, bin, td.inv_bin, td.band, nhist, B,
+                                        wlim, win, hp, ovf);
           pair_block<T, WCAP, false>(xi, yi, zi, tid, sx + ng, sy + ng, sz + ng, cn - ng, s0 + ng, bin,
                                      td.inv_bin, td.band, nhist, B, wlim, win, hp, ovf);
         }

@@ -245,3 +245,32 @@ def c4_particles(count: int = 256, seed: int = 11, nmin: int = 500,
         shape = ("fcc", "core-shell")[int(rng.integers(0, 2))]
         out.append(particle(n, 2, shape, float(rng.uniform(3.6, 4.1)), 0.05, rng))
     return out
+
+
+def uniform_rotations(n: int, seed: int = 8) -> np.ndarray:
+    """n rotation matrices (n, 3, 3) uniform on SO(3), from uniform unit
+    quaternions (Shoemake's subgroup algorithm)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    a, b = np.sqrt(1.0 - u1), np.sqrt(u1)
+    w, x = a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2)
+    y, z = b * np.sin(2 * np.pi * u3), b * np.cos(2 * np.pi * u3)
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z)
+    R[:, 0, 1] = 2 * (x * y - z * w)
+    R[:, 0, 2] = 2 * (x * z + y * w)
+    R[:, 1, 0] = 2 * (x * y + z * w)
+    R[:, 1, 1] = 1 - 2 * (x * x + z * z)
+    R[:, 1, 2] = 2 * (y * z - x * w)
+    R[:, 2, 0] = 2 * (x * z - y * w)
+    R[:, 2, 1] = 2 * (y * z + x * w)
+    R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def q_plane(qmax: float = 8.0, n: int = 512) -> np.ndarray:
+    """(n*n, 3) Cartesian q vectors, qx, qy on n points from -qmax to qmax
+    (exactly symmetric about 0), qz = 0."""
+    g = (np.arange(n) - (n - 1) / 2.0) * (2.0 * qmax / (n - 1))
+    QX, QY = np.meshgrid(g, g, indexing="ij")
+    return np.stack([QX.ravel(), QY.ravel(), np.zeros(QX.size)], axis=1)
