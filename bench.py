@@ -57,7 +57,7 @@ FETCH_SCALE, WRITE_SCALE = 1048576 / 524308, 1.0
 HIST_TRAFFIC_BYTES = int((HIST_FETCH_KIB * FETCH_SCALE + HIST_WRITE_KIB * WRITE_SCALE) * 1024)
 
 
-def gpu_step_setup(ctx, ens, nh, q, f64):
+def gpu_step_setup(ctx, ens, nh, q, f64, screen=False):
     import torch
     dev = torch.device("cuda", 0)
     xyz = torch.from_numpy(np.ascontiguousarray(ens.xyz64() if f64 else ens.xyz)).to(dev)
@@ -72,7 +72,7 @@ def gpu_step_setup(ctx, ens, nh, q, f64):
 
     def step():
         ctx.hist_dev(xyz.data_ptr(), ens.n, ens.el_count, BIN, nh, hist.data_ptr(), ovf.data_ptr(),
-                     f64=f64)
+                     f64=f64, screen=screen)
         ctx.hist_pairsum_dev(hist.data_ptr(), P, nh, BIN, qd.data_ptr(), q.size, S.data_ptr())
         ctx.pdf_dev(hist.data_ptr(), ens.el_count, nh, BIN, "rPDF", rho0, sl, pdf.data_ptr())
 
@@ -152,7 +152,7 @@ def copy_probe(nbytes=1 << 30):
     print(json.dumps({"copy_probe_bytes": nbytes}))
 
 
-def ensemble_leg(ctx, count, steps, warmup):
+def ensemble_leg(ctx, count, steps, warmup, screen=False):
     """Batched (one plan, graph replays) vs structure-by-structure runs of the
     same kernels on `count` nanoparticles: histograms (FP64 binning), pair
     sums on 2,048 q and rPDF per particle.  Both write the same layout, which
@@ -176,7 +176,7 @@ def ensemble_leg(ctx, count, steps, warmup):
     sl = np.concatenate([np.linspace(2.0, 9.0, p.n_el) for p in parts])
     bctx = xh.Context(0)          # own non-default stream (graph capture)
     t0 = time.perf_counter()
-    b = xh.Batch(bctx, [(p.xyz64(), p.el_count) for p in parts], bin_, f64=True)
+    b = xh.Batch(bctx, [(p.xyz64(), p.el_count) for p in parts], bin_, f64=True, screen=screen)
     t_plan = time.perf_counter() - t0
     dev = torch.device("cuda", 0)
     qd = torch.from_numpy(q).to(dev)
@@ -220,7 +220,8 @@ def ensemble_leg(ctx, count, steps, warmup):
             P = p.n_el * (p.n_el + 1) // 2
             nh = int(b.nhist[s])
             hp = hist2.data_ptr() + 8 * int(b.hist_off[s])
-            bctx.hist_dev(xyz_d[s].data_ptr(), p.n, p.el_count, bin_, nh, hp, None, f64=True)
+            bctx.hist_dev(xyz_d[s].data_ptr(), p.n, p.el_count, bin_, nh, hp, None, f64=True,
+                          screen=screen)
             bctx.hist_pairsum_dev(hp, P, nh, bin_, qd.data_ptr(), nq,
                                   S2.data_ptr() + 8 * int(b.type_first[s]) * nq)
             bctx.pdf_dev(hp, p.el_count, nh, bin_, "rPDF", rho[s], sl[e0:e0 + p.n_el],
@@ -433,6 +434,46 @@ def amp2d_leg(ctx, nrot=16):
     }
 
 
+def debye_leg(ctx):
+    """Direct Debye pair sums (every pair, every q, FP64 terms): the 1,985-atom
+    fcc sphere on the reference q mesh (1,024 q over 0.5-10 A^-1; 2.0e9
+    pair-q) and the 50,000-atom three-element disordered solid on the same
+    1,024 q (1.28e12 pair-q).  Gpair-q/s from the kernel's HIP events; the small case is
+    checked against an FP64 numpy sum over all pairs."""
+    import torch
+    dev = torch.device("cuda", 0)
+    out = {}
+    for name, ens, q in (("fcc_1985", st.fcc_sphere(), st.q_mesh(0.5, 10.0, 1024)),
+                         ("disordered_50000", st.disordered_cubic(), st.q_mesh(0.5, 10.0, 1024))):
+        x64 = ens.xyz64()
+        xd = torch.from_numpy(x64).to(dev)
+        qd = torch.from_numpy(q).to(dev)
+        P = ens.n_el * (ens.n_el + 1) // 2
+        S = torch.empty(P * q.size, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        ctx.debye_pairsum_dev(xd.data_ptr(), ens.n, ens.el_count, qd.data_ptr(), q.size, S.data_ptr())
+        ctx.synchronize()
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        ctx.debye_pairsum_dev(xd.data_ptr(), ens.n, ens.el_count, qd.data_ptr(), q.size, S.data_ptr())
+        ctx.synchronize()
+        t = time.perf_counter() - t0
+        kms = ctx.profile_read("debye")[0]
+        ctx.profile(False)
+        pairs = ens.n * (ens.n - 1) // 2
+        rec = {"atoms": ens.n, "nq": int(q.size), "pair_q": pairs * int(q.size), "ms": round(t * 1e3, 3),
+               "kernel_ms": round(kms, 3), "gpair_q_per_s": round(pairs * q.size / (kms * 1e-3) / 1e9, 1)}
+        if ens.n <= 2000:
+            d = np.sqrt(((x64[:, :, None] - x64[:, None, :]) ** 2).sum(axis=0))
+            iu = np.triu_indices(ens.n, 1)
+            r = d[iu]
+            Se = np.array([np.sum(np.sin(qq * r) / (qq * r + 1e-8)) for qq in q])
+            Sg = S.cpu().numpy()
+            rec["max_rel_err_vs_fp64"] = float(np.max(np.abs(Sg - Se)) / np.max(np.abs(Se)))
+        out[name] = rec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -442,10 +483,15 @@ def main():
     ap.add_argument("--ensemble", type=int, default=1000,
                     help="particles in the batched-ensemble leg (0 skips it)")
     ap.add_argument("--no-2d", action="store_true", help="skip the 2D pattern leg")
+    ap.add_argument("--no-debye", action="store_true", help="skip the direct Debye leg")
     ap.add_argument("--no-matrix", action="store_true",
                     help="skip the many-histogram pair-sum leg (FP64 vector vs int8 matrix cores)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
+    ap.add_argument("--screen", action="store_true",
+                    help="headline with the FP32-screened FP64 histogram kernel (same counts)")
+    ap.add_argument("--only", choices=["matrix", "2d", "ensemble", "debye"],
+                    help="run one extra leg alone and print its object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported: "
@@ -453,6 +499,17 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
+        return
+    if args.only:
+        import xansons_hip as xh
+        ctx = xh.Context(0)
+        leg = {"matrix": lambda: pairsum_matrix_leg(ctx, max(1, args.steps)),
+               "2d": lambda: amp2d_leg(ctx),
+               "debye": lambda: debye_leg(ctx),
+               "ensemble": lambda: ensemble_leg(ctx, args.ensemble, max(1, args.steps // 2), 1,
+                                                screen=args.screen)}
+        print(json.dumps({args.only: leg[args.only]()}))
+        ctx.close()
         return
 
     import torch
@@ -468,7 +525,7 @@ def main():
     ctx = xh.Context(0)
     ctx.set_stream(torch.cuda.current_stream().cuda_stream)
     # headline: FP64 binning, the reference CPU path's expression (bit-exact counts)
-    step, hist, ovf = gpu_step_setup(ctx, ens, nh, q, f64=True)
+    step, hist, ovf = gpu_step_setup(ctx, ens, nh, q, f64=True, screen=args.screen)
     ms, kern = time_steps(ctx, step, args.steps, args.warmup)
     h = hist.view(3, nh).cpu().numpy()
     totals_ok = [int(v) for v in h.sum(axis=1)] == [na * (na - 1) // 2, na * nb, nb * (nb - 1) // 2]
@@ -518,6 +575,8 @@ def main():
         line["pairsum_matrix"] = pairsum_matrix_leg(ctx, max(1, args.steps))
     if not args.no_2d:
         line["pattern_2d"] = amp2d_leg(ctx)
+    if not args.no_debye:
+        line["debye"] = debye_leg(ctx)
     if args.ensemble > 0:
         del hist, ovf
         torch.cuda.empty_cache()
