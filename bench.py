@@ -57,7 +57,7 @@ FETCH_SCALE, WRITE_SCALE = 1048576 / 524308, 1.0
 HIST_TRAFFIC_BYTES = int((HIST_FETCH_KIB * FETCH_SCALE + HIST_WRITE_KIB * WRITE_SCALE) * 1024)
 
 
-def gpu_step_setup(ctx, ens, nh, q, f64, screen=False):
+def gpu_step_setup(ctx, ens, nh, q, f64, screen=False, wide=False):
     import torch
     dev = torch.device("cuda", 0)
     xyz = torch.from_numpy(np.ascontiguousarray(ens.xyz64() if f64 else ens.xyz)).to(dev)
@@ -72,7 +72,7 @@ def gpu_step_setup(ctx, ens, nh, q, f64, screen=False):
 
     def step():
         ctx.hist_dev(xyz.data_ptr(), ens.n, ens.el_count, BIN, nh, hist.data_ptr(), ovf.data_ptr(),
-                     f64=f64, screen=screen)
+                     f64=f64, screen=screen, wide=wide)
         ctx.hist_pairsum_dev(hist.data_ptr(), P, nh, BIN, qd.data_ptr(), q.size, S.data_ptr())
         ctx.pdf_dev(hist.data_ptr(), ens.el_count, nh, BIN, "rPDF", rho0, sl, pdf.data_ptr())
 
@@ -488,6 +488,8 @@ def main():
                     help="skip the many-histogram pair-sum leg (FP64 vector vs int8 matrix cores)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
+    ap.add_argument("--wide", action="store_true",
+                    help="histogram kernels with two I atoms per thread (same counts)")
     ap.add_argument("--screen", action="store_true",
                     help="headline with the FP32-screened FP64 histogram kernel (same counts)")
     ap.add_argument("--only", choices=["matrix", "2d", "ensemble", "debye"],
@@ -501,7 +503,9 @@ def main():
         copy_probe()
         return
     if args.only:
+        import torch
         import xansons_hip as xh
+        torch.cuda.init()          # the legs allocate through torch
         ctx = xh.Context(0)
         leg = {"matrix": lambda: pairsum_matrix_leg(ctx, max(1, args.steps)),
                "2d": lambda: amp2d_leg(ctx),
@@ -525,12 +529,12 @@ def main():
     ctx = xh.Context(0)
     ctx.set_stream(torch.cuda.current_stream().cuda_stream)
     # headline: FP64 binning, the reference CPU path's expression (bit-exact counts)
-    step, hist, ovf = gpu_step_setup(ctx, ens, nh, q, f64=True, screen=args.screen)
+    step, hist, ovf = gpu_step_setup(ctx, ens, nh, q, f64=True, screen=args.screen, wide=args.wide)
     ms, kern = time_steps(ctx, step, args.steps, args.warmup)
     h = hist.view(3, nh).cpu().numpy()
     totals_ok = [int(v) for v in h.sum(axis=1)] == [na * (na - 1) // 2, na * nb, nb * (nb - 1) // 2]
     del step
-    step32, hist32, _ = gpu_step_setup(ctx, ens, nh, q, f64=False)
+    step32, hist32, _ = gpu_step_setup(ctx, ens, nh, q, f64=False, wide=args.wide)
     ms32, kern32 = time_steps(ctx, step32, max(1, args.steps // 2), 1)
     del step32, hist32
 

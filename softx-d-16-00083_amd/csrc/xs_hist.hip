@@ -45,7 +45,7 @@ constexpr int kStage = 256;      // J atoms staged in LDS per step
 constexpr int kSortCap = 2048;   // J tiles sorted per work item
 constexpr int kMaxEl = 16;
 constexpr int kBoxBlocks = 512;  // partial bounding boxes
-constexpr double kFill = 800.0;  // target atoms per cell of the largest element
+constexpr double kFill = 0.94 * kTile;  // target atoms in the fullest cell
 constexpr uint32_t kMaxCellsPerAxis = 256;
 
 struct Tile {
@@ -70,7 +70,7 @@ struct Item {
 
 struct Grid {
   double lo[3];
-  double inv_cell;
+  double inv_cell[3];  // per axis (0 for a flat axis)
   uint32_t g[3];
 };
 
@@ -138,44 +138,37 @@ __global__ void hist_bbox_reduce(const double *__restrict__ part, int nparts,
   if (threadIdx.x < 6) bbox[threadIdx.x] = s[threadIdx.x][0];
 }
 
-struct ElemRanges {
-  uint32_t off[kMaxEl], cnt[kMaxEl];
-};
-
+// Cell of every atom of one element (keys) and atoms per cell (count).
 template <class T>
 __global__ void hist_cell_count(const T *__restrict__ x, const T *__restrict__ y,
-                                const T *__restrict__ z, ElemRanges g, Grid grid,
+                                const T *__restrict__ z, uint32_t n, Grid grid,
                                 uint32_t *__restrict__ keys, uint32_t *__restrict__ count) {
-  const int e = blockIdx.y;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.cnt[e]) return;
-  const size_t gi = (size_t)g.off[e] + i;
-  const double v[3] = {(double)x[gi], (double)y[gi], (double)z[gi]};
+  if (i >= n) return;
+  const double v[3] = {(double)x[i], (double)y[i], (double)z[i]};
   uint32_t c[3];
   for (int a = 0; a < 3; ++a) {
-    const double f = (v[a] - grid.lo[a]) * grid.inv_cell;
+    const double f = (v[a] - grid.lo[a]) * grid.inv_cell[a];
     const int ci = isfinite(f) ? (int)f : 0;
     c[a] = (uint32_t)min(max(ci, 0), (int)grid.g[a] - 1);
   }
-  const uint32_t m = grid.g[0] * grid.g[1] * grid.g[2];
   const uint32_t key = (c[0] * grid.g[1] + c[1]) * grid.g[2] + c[2];
-  keys[gi] = key;
-  atomicAdd(&count[(size_t)e * m + key], 1u);
+  keys[i] = key;
+  atomicAdd(&count[key], 1u);
 }
 
+// Counting-sort scatter of one element: start[cell] is the next free slot.
 template <class T>
 __global__ void hist_scatter(const T *__restrict__ x, const T *__restrict__ y,
-                             const T *__restrict__ z, ElemRanges g, uint32_t m,
-                             const uint32_t *__restrict__ keys, uint32_t *__restrict__ start,
-                             T *__restrict__ sx, T *__restrict__ sy, T *__restrict__ sz) {
-  const int e = blockIdx.y;
+                             const T *__restrict__ z, uint32_t n, const uint32_t *__restrict__ keys,
+                             uint32_t *__restrict__ start, T *__restrict__ sx, T *__restrict__ sy,
+                             T *__restrict__ sz) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.cnt[e]) return;
-  const size_t gi = (size_t)g.off[e] + i;
-  const uint32_t pos = atomicAdd(&start[(size_t)e * m + keys[gi]], 1u);
-  sx[pos] = x[gi];
-  sy[pos] = y[gi];
-  sz[pos] = z[gi];
+  if (i >= n) return;
+  const uint32_t pos = atomicAdd(&start[keys[i]], 1u);
+  sx[pos] = x[i];
+  sy[pos] = y[i];
+  sz[pos] = z[i];
 }
 
 template <class T>
@@ -240,10 +233,10 @@ __device__ __forceinline__ void box_bins(const double *__restrict__ bi, const do
   ku = (uint32_t)fh;
 }
 
-template <int WCAP>
+template <int WCAP, uint32_t NTH = kTile>
 __device__ __forceinline__ void flush_window(uint32_t *win, uint32_t lo, uint32_t hi,
                                              unsigned long long *__restrict__ hp) {
-  for (uint32_t k = lo + threadIdx.x; k < hi; k += kTile) {
+  for (uint32_t k = lo + threadIdx.x; k < hi; k += NTH) {
     const uint32_t slot = k & (WCAP - 1);
     const uint32_t v = win[slot];
     if (v) {
@@ -532,6 +525,214 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
     }
   }
   flush_window<WCAP>(win, B, min(B + (uint32_t)WCAP, nhist), hp);
+}
+
+// ---------------------------------------------------------------- two I atoms per thread
+//
+// Same items, window, J order and binning as hist_kernel, with 512 threads
+// each holding two I atoms (tid and tid + 512): every J atom read from LDS
+// serves two pairs, and a group of 8 J atoms gives 16 independent chains.
+// Waves whose lanes do not all hold two atoms (at most two per item) take the
+// one-atom loops.
+
+template <class T, int WCAP, bool SAFE>
+__device__ __forceinline__ void pair_group8x2(const T *xi, const T *yi, const T *zi, const T *sx,
+                                              const T *sy, const T *sz, uint32_t ng, T bin,
+                                              double inv_bin, double band, uint32_t nhist,
+                                              uint32_t B, uint32_t wlim, uint32_t *win,
+                                              unsigned long long *__restrict__ hp,
+                                              unsigned long long *__restrict__ ovf) {
+#pragma clang fp contract(off)
+  constexpr bool F64 = sizeof(T) == 8;
+  for (uint32_t j0 = 0; j0 < ng; j0 += 8) {
+    uint32_t k[2][8];
+    bool bad[2][8], any = false;
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      const T xj = sx[j0 + p], yj = sy[j0 + p], zj = sz[j0 + p];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const T dx = xi[r] - xj;
+        const T dy = yi[r] - yj;
+        const T dz = zi[r] - zj;
+        T s;
+        if constexpr (F64) {
+          s = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+        } else {
+          s = dx * dx + dy * dy;
+          s = s + dz * dz;
+        }
+        k[r][p] = fast_bin<T>(s, bin, inv_bin, band, nhist, bad[r][p]);
+        any |= bad[r][p];
+      }
+    }
+    if constexpr (F64) {
+      if (__builtin_expect(any, 0)) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+          for (int p = 0; p < 8; ++p) {
+            if (bad[r][p]) {
+              const T dx = xi[r] - sx[j0 + p];
+              const T dy = yi[r] - sy[j0 + p];
+              const T dz = zi[r] - sz[j0 + p];
+              T s = dx * dx + dy * dy;
+              s = s + dz * dz;
+              k[r][p] = exact_bin<T>(s, bin, nhist);
+            }
+          }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        if (SAFE || k[r][p] - B < wlim)
+          atomicAdd(&win[k[r][p] & (WCAP - 1)], 1u);
+        else if (k[r][p] < nhist)
+          atomicAdd(&hp[k[r][p]], 1ull);
+        else
+          atomicAdd(ovf, 1ull);
+      }
+  }
+}
+
+template <class T, int WCAP>
+__global__ __launch_bounds__(kTile / 2) void hist_kernel2(Args<T> A) {
+  constexpr uint32_t NTH = kTile / 2;
+  __shared__ uint32_t win[WCAP];
+  __shared__ unsigned long long keys[kSortCap];
+  __shared__ T sx[kStage], sy[kStage], sz[kStage];
+
+  const Item it = A.items[blockIdx.x];
+  const TypeDesc td = A.types[it.type];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t tiI = td.a_base + it.ti;
+  const Tile I = A.tiles[tiI];
+  T xi[2] = {0, 0}, yi[2] = {0, 0}, zi[2] = {0, 0};
+  bool iv[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const uint32_t il = tid + r * NTH;
+    iv[r] = il < I.cnt;
+    if (iv[r]) {
+      const size_t gi = (size_t)I.off + il;
+      xi[r] = A.x[gi];
+      yi[r] = A.y[gi];
+      zi[r] = A.z[gi];
+    }
+  }
+  // wave-uniform: both atoms valid on every lane / only the first on every lane
+  const uint32_t wbase = tid & ~63u;
+  const bool both = wbase + 64 + NTH <= I.cnt;
+  const bool first_only = wbase + NTH >= I.cnt && wbase + 64 <= I.cnt;
+  const T bin = (T)td.bin;
+  const uint32_t nhist = td.nhist;
+  const double *bi = A.boxes + 6 * (size_t)tiI;
+  const uint32_t n = it.jcount;
+  uint32_t npow = 1;
+  while (npow < n) npow <<= 1;
+  unsigned long long *hp = A.hist + td.hist_off;
+  unsigned long long *ovf = A.overflow + td.ovf;
+
+  for (uint32_t m = tid; m < npow; m += NTH) {
+    unsigned long long key = ~0ull;
+    if (m < n) {
+      const uint32_t tj = (it.jstart + m) % td.b_n;
+      uint32_t kl, ku;
+      bool fits;
+      box_bins(bi, A.boxes + 6 * (size_t)(td.b_base + tj), td.bin, nhist, kl, ku, fits);
+      key = ((unsigned long long)kl << 32) | m;
+    }
+    keys[m] = key;
+  }
+  for (uint32_t w = tid; w < WCAP; w += NTH) win[w] = 0u;
+  __syncthreads();
+  for (uint32_t k = 2; k <= npow; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = tid; i < npow; i += NTH) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const unsigned long long u = keys[i], v = keys[ixj];
+          const bool up = (i & k) == 0;
+          if ((u > v) == up) {
+            keys[i] = v;
+            keys[ixj] = u;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  uint32_t B = (uint32_t)(keys[0] >> 32);
+  for (uint32_t s = 0; s < n; ++s) {
+    const uint32_t m = (uint32_t)(keys[s] & 0xffffffffu);
+    const uint32_t tj = (it.jstart + m) % td.b_n;
+    const uint32_t tjg = td.b_base + tj;
+    uint32_t kl, ku;
+    bool fits;
+    box_bins(bi, A.boxes + 6 * (size_t)tjg, td.bin, nhist, kl, ku, fits);
+    uint32_t wlim = 0;
+    const bool span_ok = ku - kl + 1 <= (uint32_t)WCAP;
+    if (span_ok) {
+      const uint32_t nb = ku + 1 > (uint32_t)WCAP ? ku + 1 - WCAP : 0u;
+      if (nb > B) {
+        flush_window<WCAP, NTH>(win, B, min(min(nb, B + (uint32_t)WCAP), nhist), hp);
+        B = nb;
+        __syncthreads();
+      }
+      wlim = min((uint32_t)WCAP, nhist - B);
+    }
+    const Tile J = A.tiles[tjg];
+    const bool diag = td.same && tj == it.ti;
+    for (uint32_t s0 = 0; s0 < J.cnt; s0 += kStage) {
+      const uint32_t cn = min((uint32_t)kStage, J.cnt - s0);
+      if (tid < cn) {
+        sx[tid] = A.x[(size_t)J.off + s0 + tid];
+        sy[tid] = A.y[(size_t)J.off + s0 + tid];
+        sz[tid] = A.z[(size_t)J.off + s0 + tid];
+      }
+      __syncthreads();
+      const uint32_t ng = cn & ~7u;
+      if (!diag && both) {
+        if (span_ok && fits)
+          pair_group8x2<T, WCAP, true>(xi, yi, zi, sx, sy, sz, ng, bin, td.inv_bin, td.band, nhist, B,
+                                       wlim, win, hp, ovf);
+        else
+          pair_group8x2<T, WCAP, false>(xi, yi, zi, sx, sy, sz, ng, bin, td.inv_bin, td.band, nhist,
+                                        B, wlim, win, hp, ovf);
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+          pair_block<T, WCAP, false>(xi[r], yi[r], zi[r], tid + r * NTH, sx + ng, sy + ng, sz + ng,
+                                     cn - ng, s0 + ng, bin, td.inv_bin, td.band, nhist, B, wlim, win,
+                                     hp, ovf);
+      } else if (!diag && first_only) {
+        if (span_ok && fits)
+          pair_group8<T, WCAP, true>(xi[0], yi[0], zi[0], sx, sy, sz, ng, bin, td.inv_bin, td.band,
+                                     nhist, B, wlim, win, hp, ovf);
+        else
+          pair_group8<T, WCAP, false>(xi[0], yi[0], zi[0], sx, sy, sz, ng, bin, td.inv_bin, td.band,
+                                      nhist, B, wlim, win, hp, ovf);
+        pair_block<T, WCAP, false>(xi[0], yi[0], zi[0], tid, sx + ng, sy + ng, sz + ng, cn - ng,
+                                   s0 + ng, bin, td.inv_bin, td.band, nhist, B, wlim, win, hp, ovf);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          if (iv[r]) {
+            if (diag)
+              pair_block<T, WCAP, true>(xi[r], yi[r], zi[r], tid + r * NTH, sx, sy, sz, cn, s0, bin,
+                                        td.inv_bin, td.band, nhist, B, wlim, win, hp, ovf);
+            else
+              pair_block<T, WCAP, false>(xi[r], yi[r], zi[r], tid + r * NTH, sx, sy, sz, cn, s0, bin,
+                                         td.inv_bin, td.band, nhist, B, wlim, win, hp, ovf);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  flush_window<WCAP, NTH>(win, B, min(B + (uint32_t)WCAP, nhist), hp);
 }
 
 // ---------------------------------------------------------------- FP32-screened FP64 kernel
@@ -836,26 +1037,65 @@ __global__ __launch_bounds__(kTile) void hist_kernel_q(Args<double> A) {
 
 // ---------------------------------------------------------------- host planning
 
-static Grid make_grid(const double bbox[6], unsigned max_count, double bin, uint32_t nhist,
-                      uint32_t wcap) {
+// Cell sizing.  Lanes of a work group hold the atoms of one I tile, so a
+// tile should be nearly full: cells are sized for kFill atoms in the fullest
+// cell.  The first guess assumes uniform density over the bounding box; one
+// refinement rescales the edge by cbrt(kFill / fullest count) after counting
+// (spheres are ~2x denser than their box).  When the histogram exceeds the
+// window, the edge is capped at (WCAP - 8) bin / (2 sqrt 3) so that the bin
+// span between any two cells fits the window.
+static Grid grid_with_cell(const double bbox[6], double cell, double cap) {
   Grid gr{};
-  double ext[3], vol = 1.0;
+  double ext[3];
   for (int a = 0; a < 3; ++a) {
     gr.lo[a] = bbox[a];
     ext[a] = std::max(bbox[3 + a] - bbox[a], 0.0);
   }
   const double span = std::max({ext[0], ext[1], ext[2], 1e-6});
-  const double spacing = span / std::max(1.0, std::cbrt((double)std::max(max_count, 1u)));
-  for (int a = 0; a < 3; ++a) vol *= std::max(ext[a], spacing);
-  double cell = std::cbrt(kFill * vol / std::max(max_count, 1u));
-  if (nhist > wcap) cell = std::min(cell, (wcap - 8) * bin / (2.0 * std::sqrt(3.0)));
+  cell = std::min(cell, cap);
   cell = std::max(cell, span / kMaxCellsPerAxis);
   cell = std::max(cell, 1e-9);
-  gr.inv_cell = 1.0 / cell;
-  for (int a = 0; a < 3; ++a)
+  for (int a = 0; a < 3; ++a) {
     gr.g[a] = (uint32_t)std::min<double>(kMaxCellsPerAxis,
-                                         std::max(1.0, std::ceil(ext[a] / cell + 1e-9)));
+                                         std::max(1.0, std::ceil(ext[a] / cell - 1e-9)));
+    gr.inv_cell[a] = ext[a] > 0.0 ? gr.g[a] / ext[a] : 0.0;
+  }
   return gr;
+}
+
+static double cell_cap(double bin, uint32_t nhist, uint32_t wcap) {
+  return nhist > wcap ? (wcap - 8) * bin / (2.0 * std::sqrt(3.0)) : INFINITY;
+}
+
+static double first_cell(const double bbox[6], unsigned max_count, double cap) {
+  double ext[3], vol = 1.0;
+  for (int a = 0; a < 3; ++a) ext[a] = std::max(bbox[3 + a] - bbox[a], 0.0);
+  const double span = std::max({ext[0], ext[1], ext[2], 1e-6});
+  const double spacing = span / std::max(1.0, std::cbrt((double)std::max(max_count, 1u)));
+  for (int a = 0; a < 3; ++a) vol *= std::max(ext[a], spacing);
+  return std::min(std::cbrt(kFill * vol / std::max(max_count, 1u)), cap);
+}
+
+// New edge from the fullest cell's count, or 0 when the current one is good.
+static double refined_cell(double cell, uint32_t fullest, double cap) {
+  if (fullest == 0) return 0.0;
+  const bool over = fullest > (uint32_t)kTile;
+  const bool under = fullest < 0.75 * kFill && cell < cap;
+  if (!over && !under) return 0.0;
+  // aim the fullest cell just below a full tile when shrinking, at kFill when growing
+  const double target = over ? 0.99 * kTile : kFill;
+  return std::min(cell * std::cbrt(target / fullest), cap);
+}
+
+// Tiles of one cell: ceil(c / kTile) chunks of near-equal size.
+static void cell_tiles(uint32_t pos, uint32_t c, std::vector<Tile> &tiles) {
+  if (!c) return;
+  const uint32_t nt = (c + kTile - 1) / kTile;
+  for (uint32_t t = 0, o = 0; t < nt; ++t) {
+    const uint32_t len = (c - o) / (nt - t);
+    tiles.push_back(Tile{pos + o, len});
+    o += len;
+  }
 }
 
 static uint32_t wcap_for(uint32_t nhist) { return nhist <= 8192 ? 8192u : 32768u; }
@@ -958,44 +1198,55 @@ static void sort_structure_host(const T *xyz, size_t n, const unsigned *el_count
     }
   if (n == 0)
     for (int a = 0; a < 6; ++a) bbox[a] = 0.0;
-  unsigned max_count = 0;
-  for (unsigned e = 0; e < n_el; ++e) max_count = std::max(max_count, el_count[e]);
-  const Grid gr = make_grid(bbox, max_count, bin, nhist, wcap_for(nhist));
-  const uint32_t m = gr.g[0] * gr.g[1] * gr.g[2];
-  std::vector<uint32_t> key(n), cnt(m), pos(m);
+  // each element on its own grid, sized for full tiles of that element
+  const double cap = cell_cap(bin, nhist, wcap_for(nhist));
+  std::vector<uint32_t> key, cnt;
   size_t off = 0;
   for (unsigned e = 0; e < n_el; ++e) {
-    std::fill(cnt.begin(), cnt.end(), 0u);
-    for (size_t i = off; i < off + el_count[e]; ++i) {
-      uint32_t c[3];
-      for (int a = 0; a < 3; ++a) {
-        const double f = ((double)xyz[a * n + i] - gr.lo[a]) * gr.inv_cell;
-        const int ci = std::isfinite(f) ? (int)f : 0;
-        c[a] = (uint32_t)std::min(std::max(ci, 0), (int)gr.g[a] - 1);
+    const size_t ne = el_count[e];
+    double cell = first_cell(bbox, (unsigned)ne, cap);
+    Grid gr;
+    uint32_t m;
+    key.assign(ne, 0u);
+    for (int pass = 0;; ++pass) {
+      gr = grid_with_cell(bbox, cell, cap);
+      m = gr.g[0] * gr.g[1] * gr.g[2];
+      cnt.assign(m, 0u);
+      for (size_t i = 0; i < ne; ++i) {
+        uint32_t c[3];
+        for (int a = 0; a < 3; ++a) {
+          const double f = ((double)xyz[a * n + off + i] - gr.lo[a]) * gr.inv_cell[a];
+          const int ci = std::isfinite(f) ? (int)f : 0;
+          c[a] = (uint32_t)std::min(std::max(ci, 0), (int)gr.g[a] - 1);
+        }
+        key[i] = (c[0] * gr.g[1] + c[1]) * gr.g[2] + c[2];
+        ++cnt[key[i]];
       }
-      key[i] = (c[0] * gr.g[1] + c[1]) * gr.g[2] + c[2];
-      ++cnt[key[i]];
+      const uint32_t fullest = ne ? *std::max_element(cnt.begin(), cnt.end()) : 0u;
+      const double nc = pass == 0 ? refined_cell(cell, fullest, cap) : 0.0;
+      if (nc == 0.0) break;
+      cell = nc;
     }
     const size_t base = sx.size();
     uint32_t p = (uint32_t)base;
     ebase[e] = (uint32_t)tiles.size();
     for (uint32_t c = 0; c < m; ++c) {
-      pos[c] = p;
-      for (uint32_t t = 0; t < cnt[c]; t += kTile)
-        tiles.push_back(Tile{p + t, std::min<uint32_t>(kTile, cnt[c] - t)});
-      p += cnt[c];
+      const uint32_t k = cnt[c];
+      cnt[c] = p;  // becomes the next free slot of the cell
+      cell_tiles(p, k, tiles);
+      p += k;
     }
     en[e] = (uint32_t)tiles.size() - ebase[e];
-    sx.resize(base + el_count[e]);
-    sy.resize(base + el_count[e]);
-    sz.resize(base + el_count[e]);
-    for (size_t i = off; i < off + el_count[e]; ++i) {
-      const uint32_t d = pos[key[i]]++;
-      sx[d] = xyz[i];
-      sy[d] = xyz[n + i];
-      sz[d] = xyz[2 * n + i];
+    sx.resize(base + ne);
+    sy.resize(base + ne);
+    sz.resize(base + ne);
+    for (size_t i = 0; i < ne; ++i) {
+      const uint32_t d = cnt[key[i]]++;
+      sx[d] = xyz[off + i];
+      sy[d] = xyz[n + off + i];
+      sz[d] = xyz[2 * n + off + i];
     }
-    off += el_count[e];
+    off += ne;
   }
   for (size_t t = boxes.size() / 6; t < tiles.size(); ++t) {
     double b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
@@ -1012,7 +1263,7 @@ static void sort_structure_host(const T *xyz, size_t n, const unsigned *el_count
 
 template <class T>
 static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wcap, bool prof,
-                       bool screen) {
+                       bool screen, bool wide) {
   if (!nitems) return XS_OK;
   hipEvent_t ev = nullptr;
   if (prof) XS_TRY(prof_begin(ctx, "hist", &ev));
@@ -1028,10 +1279,17 @@ static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wc
       return XS_OK;
     }
   }
-  if (wcap == 8192)
+  if (wide) {
+    const dim3 block2(kTile / 2);
+    if (wcap == 8192)
+      hipLaunchKernelGGL((hist_kernel2<T, 8192>), grid, block2, 0, ctx->stream, A);
+    else
+      hipLaunchKernelGGL((hist_kernel2<T, 32768>), grid, block2, 0, ctx->stream, A);
+  } else if (wcap == 8192) {
     hipLaunchKernelGGL((hist_kernel<T, 8192>), grid, block, 0, ctx->stream, A);
-  else
+  } else {
     hipLaunchKernelGGL((hist_kernel<T, 32768>), grid, block, 0, ctx->stream, A);
+  }
   XS_HIP_CHECK(hipGetLastError());
   if (prof) XS_TRY(prof_end(ctx, "hist", ev));
   return XS_OK;
@@ -1042,18 +1300,14 @@ static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wc
 template <class T>
 static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_count,
                       unsigned n_el, double bin, unsigned nhist, unsigned long long *hist_d,
-                      unsigned long long *ovf_user, bool screen) {
+                      unsigned long long *ovf_user, bool screen, bool wide) {
   const unsigned ntypes = ntypes_of(n_el);
   const size_t hsize = (size_t)ntypes * nhist;
   const uint32_t wcap = wcap_for(nhist);
-  ElemRanges g{};
-  unsigned max_count = 0;
-  uint32_t off = 0;
-  for (unsigned e = 0; e < n_el; ++e) {
-    g.off[e] = off;
-    g.cnt[e] = el_count[e];
-    off += el_count[e];
-    max_count = std::max(max_count, el_count[e]);
+  uint32_t eoff[kMaxEl];
+  for (unsigned e = 0, o = 0; e < n_el; ++e) {
+    eoff[e] = o;
+    o += el_count[e];
   }
   // 1. bounding box on the GPU, read back
   XS_TRY(ensure_pinned(ctx, 4096));
@@ -1074,87 +1328,107 @@ static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_co
     std::memcpy(bbox, ctx->pin, sizeof(bbox));
   }
   XS_TRY(check_span(bbox, bin));
-  const Grid grid = make_grid(bbox, max_count, bin, nhist, wcap);
-  const uint32_t m = grid.g[0] * grid.g[1] * grid.g[2];
 
-  // 2. workspace for the rest of the call.  A non-empty cell holds at least
-  //    one atom, so tiles <= n/kTile + min(n, n_el*m) + n_el.
-  const size_t max_tiles = n / kTile + std::min<size_t>(n, (size_t)n_el * m) + n_el;
-  const size_t cb = (size_t)n_el * m * sizeof(uint32_t);
-  XS_TRY(ensure_workspace(ctx, Carver::need({3 * n * sizeof(T), n * sizeof(uint32_t), cb, cb,
-                                             max_tiles * 48, sizeof(unsigned long long)})));
-  XS_TRY(ensure_pinned(ctx, cb));
+  // 2. workspace: sorted coordinates, cell keys, overflow counter
+  XS_TRY(ensure_workspace(ctx, Carver::need({3 * n * sizeof(T), n * sizeof(uint32_t),
+                                             sizeof(unsigned long long)})));
   Carver cv(ctx->ws);
   T *sorted = cv.take<T>(3 * n);
   uint32_t *keys = cv.take<uint32_t>(n);
-  uint32_t *count = cv.take<uint32_t>((size_t)n_el * m);
-  uint32_t *start_d = cv.take<uint32_t>((size_t)n_el * m);
-  double *boxes = cv.take<double>(max_tiles * 6);
   unsigned long long *ovf_own = cv.take<unsigned long long>(1);
   unsigned long long *ovf_d = ovf_user ? ovf_user : ovf_own;
+  T *sxp = sorted, *syp = sorted + n, *szp = sorted + 2 * n;
 
+  // 3. per element: cell grid sized for full tiles (a second count when the
+  //    fullest cell is far from kFill); counts read back, tiles on the host.
+  //    The counts go through the table buffer, which holds nothing else yet.
   hipEvent_t ev;
   XS_TRY(prof_begin(ctx, "hist_prep", &ev));
-  const unsigned zb = (unsigned)std::min<size_t>((hsize + 255) / 256, 4096);
-  hipLaunchKernelGGL(fill_u64, dim3(std::max(zb, 1u)), dim3(256), 0, ctx->stream, hist_d, hsize);
-  hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, ctx->stream, ovf_d, (size_t)1);
-  XS_HIP_CHECK(hipMemsetAsync(count, 0, cb, ctx->stream));
-  const dim3 eg((max_count + 255) / 256, n_el);
-  hipLaunchKernelGGL((hist_cell_count<T>), eg, dim3(256), 0, ctx->stream, xyz, xyz + n, xyz + 2 * n,
-                     g, grid, keys, count);
-  XS_HIP_CHECK(hipGetLastError());
-  // 3. read the cell counts back; tiles, types and items on the host
-  XS_HIP_CHECK(hipMemcpyAsync(ctx->pin, count, cb, hipMemcpyDeviceToHost, ctx->stream));
-  XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  const uint32_t *count_h = static_cast<const uint32_t *>(ctx->pin);
-  std::vector<uint32_t> start_h((size_t)n_el * m);
+  const double cap = cell_cap(bin, nhist, wcap);
+  std::vector<Grid> grids(n_el);
+  std::vector<uint32_t> m_of(n_el), soff(n_el + 1, 0u), start_h;
   std::vector<Tile> tiles;
   uint32_t ebase[kMaxEl], en[kMaxEl];
   for (unsigned e = 0; e < n_el; ++e) {
-    ebase[e] = (uint32_t)tiles.size();
-    uint32_t pos = g.off[e];
-    for (uint32_t c = 0; c < m; ++c) {
-      const uint32_t k = count_h[(size_t)e * m + c];
-      start_h[(size_t)e * m + c] = pos;
-      for (uint32_t t = 0; t < k; t += kTile)
-        tiles.push_back(Tile{pos + t, std::min<uint32_t>(kTile, k - t)});
-      pos += k;
+    const uint32_t ne = el_count[e];
+    double cell = first_cell(bbox, ne, cap);
+    for (int pass = 0;; ++pass) {
+      grids[e] = grid_with_cell(bbox, cell, cap);
+      const uint32_t m = grids[e].g[0] * grids[e].g[1] * grids[e].g[2];
+      m_of[e] = m;
+      XS_TRY(ensure_aux(ctx, (size_t)m * sizeof(uint32_t)));
+      XS_TRY(ensure_pinned(ctx, (size_t)m * sizeof(uint32_t)));
+      uint32_t *count = static_cast<uint32_t *>(ctx->aux);
+      XS_HIP_CHECK(hipMemsetAsync(count, 0, (size_t)m * sizeof(uint32_t), ctx->stream));
+      if (ne)
+        hipLaunchKernelGGL((hist_cell_count<T>), dim3((ne + 255) / 256), dim3(256), 0, ctx->stream,
+                           xyz + eoff[e], xyz + n + eoff[e], xyz + 2 * n + eoff[e], ne, grids[e],
+                           keys + eoff[e], count);
+      XS_HIP_CHECK(hipGetLastError());
+      XS_HIP_CHECK(hipMemcpyAsync(ctx->pin, count, (size_t)m * sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+      XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      const uint32_t *c_h = static_cast<const uint32_t *>(ctx->pin);
+      const uint32_t fullest = ne ? *std::max_element(c_h, c_h + m) : 0u;
+      const double nc = pass == 0 ? refined_cell(cell, fullest, cap) : 0.0;
+      if (nc == 0.0) {
+        soff[e + 1] = soff[e] + m;
+        start_h.resize(soff[e + 1]);
+        ebase[e] = (uint32_t)tiles.size();
+        uint32_t pos = eoff[e];
+        for (uint32_t c = 0; c < m; ++c) {
+          start_h[soff[e] + c] = pos;
+          cell_tiles(pos, c_h[c], tiles);
+          pos += c_h[c];
+        }
+        en[e] = (uint32_t)tiles.size() - ebase[e];
+        break;
+      }
+      cell = nc;
     }
-    en[e] = (uint32_t)tiles.size() - ebase[e];
   }
-  if (tiles.size() > max_tiles) return fail(XS_ERR_RANGE, "xs_hist: tile bound exceeded");
   std::vector<TypeDesc> types;
   add_types(n_el, ebase, en, nhist, bin, 0ull, 0u, types);
   std::vector<Item> items;
   add_items(types, 0, (uint32_t)types.size(), items);
   order_items(items, types, tiles);
-  // tables in their own buffer, sized exactly (the stream is idle here)
-  const size_t tb = tiles.size() * sizeof(Tile), yb = types.size() * sizeof(TypeDesc),
-               ib = items.size() * sizeof(Item);
-  XS_TRY(ensure_aux(ctx, Carver::need({tb, yb, ib})));
+
+  // 4. tables (exact sizes) in the table buffer; the stream is idle here
+  const size_t sb = start_h.size() * sizeof(uint32_t), tb = tiles.size() * sizeof(Tile),
+               yb = types.size() * sizeof(TypeDesc), ib = items.size() * sizeof(Item),
+               bb = tiles.size() * 6 * sizeof(double);
+  XS_TRY(ensure_aux(ctx, Carver::need({sb, tb, yb, ib, bb})));
   Carver ca(ctx->aux);
+  uint32_t *start_d = ca.take<uint32_t>(start_h.size());
   Tile *tiles_d = ca.take<Tile>(tiles.size());
   TypeDesc *types_d = ca.take<TypeDesc>(types.size());
   Item *items_d = ca.take<Item>(items.size());
-  XS_HIP_CHECK(hipMemcpy(start_d, start_h.data(), cb, hipMemcpyHostToDevice));
+  double *boxes = ca.take<double>(tiles.size() * 6);
+  if (sb) XS_HIP_CHECK(hipMemcpy(start_d, start_h.data(), sb, hipMemcpyHostToDevice));
   if (tb) XS_HIP_CHECK(hipMemcpy(tiles_d, tiles.data(), tb, hipMemcpyHostToDevice));
   XS_HIP_CHECK(hipMemcpy(types_d, types.data(), yb, hipMemcpyHostToDevice));
   if (ib) XS_HIP_CHECK(hipMemcpy(items_d, items.data(), ib, hipMemcpyHostToDevice));
-  T *sxp = sorted, *syp = sorted + n, *szp = sorted + 2 * n;
-  hipLaunchKernelGGL((hist_scatter<T>), eg, dim3(256), 0, ctx->stream, xyz, xyz + n, xyz + 2 * n, g, m,
-                     keys, start_d, sxp, syp, szp);
+  const unsigned zb = (unsigned)std::min<size_t>((hsize + 255) / 256, 4096);
+  hipLaunchKernelGGL(fill_u64, dim3(std::max(zb, 1u)), dim3(256), 0, ctx->stream, hist_d, hsize);
+  hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, ctx->stream, ovf_d, (size_t)1);
+  for (unsigned e = 0; e < n_el; ++e)
+    if (el_count[e])
+      hipLaunchKernelGGL((hist_scatter<T>), dim3((el_count[e] + 255) / 256), dim3(256), 0, ctx->stream,
+                         xyz + eoff[e], xyz + n + eoff[e], xyz + 2 * n + eoff[e], el_count[e],
+                         keys + eoff[e], start_d + soff[e], sxp, syp, szp);
   if (!tiles.empty())
     hipLaunchKernelGGL((hist_tile_boxes<T>), dim3((unsigned)tiles.size()), dim3(256), 0, ctx->stream,
                        sxp, syp, szp, tiles_d, boxes);
   XS_HIP_CHECK(hipGetLastError());
   XS_TRY(prof_end(ctx, "hist_prep", ev));
   Args<T> A{sxp, syp, szp, tiles_d, boxes, types_d, items_d, hist_d, ovf_d};
-  return launch_hist<T>(ctx, A, items.size(), wcap, true, screen);
+  return launch_hist<T>(ctx, A, items.size(), wcap, true, screen, wide);
 }
 
 template <class T>
 static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_count, unsigned n_el,
-                    double bin, unsigned nhist, uint64_t *hist_out, uint64_t *ovf_out, bool screen) {
+                    double bin, unsigned nhist, uint64_t *hist_out, uint64_t *ovf_out, bool screen,
+                    bool wide) {
   const unsigned ntypes = ntypes_of(n_el);
   const size_t hsize = (size_t)ntypes * nhist;
   {
@@ -1206,7 +1480,7 @@ static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_coun
   hipLaunchKernelGGL(fill_u64, dim3(std::max(zb, 1u)), dim3(256), 0, ctx->stream, hist_d, hsize);
   hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, ctx->stream, ovf_d, (size_t)1);
   Args<T> A{sorted, sorted + n, sorted + 2 * n, tiles_d, boxes_d, types_d, items_d, hist_d, ovf_d};
-  XS_TRY(launch_hist<T>(ctx, A, items.size(), wcap_for(nhist), true, screen));
+  XS_TRY(launch_hist<T>(ctx, A, items.size(), wcap_for(nhist), true, screen, wide));
   XS_HIP_CHECK(hipMemcpyAsync(hist_out, hist_d, hsize * 8, hipMemcpyDeviceToHost, ctx->stream));
   XS_HIP_CHECK(hipMemcpyAsync(ctx->pin, ovf_d, 8, hipMemcpyDeviceToHost, ctx->stream));
   XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -1234,11 +1508,12 @@ static void free_plan(xs_batch *b) {
 template <class T>
 static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const unsigned *el_count,
                       const double *const *xyz, double bin, const unsigned *nhist_in, bool f64,
-                      bool screen, xs_batch **out) {
+                      bool screen, bool wide, xs_batch **out) {
   auto *b = new xs_batch();
   b->ctx = ctx;
   b->f64 = f64;
   b->screen = screen;
+  b->wide = wide;
   b->bin = bin;
   b->nstruct = nstruct;
   std::vector<T> sx, sy, sz, tmp;
@@ -1374,9 +1649,9 @@ static int plan_hist(xs_batch *b, unsigned long long *hist_d, bool prof) {
             items,
             hist_d,
             b->ovf};
-  XS_TRY(launch_hist<T>(ctx, A, b->items8192, 8192, prof, b->screen));
+  XS_TRY(launch_hist<T>(ctx, A, b->items8192, 8192, prof, b->screen, b->wide));
   A.items = items + b->items8192;
-  XS_TRY(launch_hist<T>(ctx, A, b->items32768, 32768, prof, b->screen));
+  XS_TRY(launch_hist<T>(ctx, A, b->items32768, 32768, prof, b->screen, b->wide));
   return XS_OK;
 }
 
@@ -1410,6 +1685,7 @@ extern "C" int xs_hist(xs_ctx *ctx, const void *xyz, size_t n_atoms, const unsig
     return fail(XS_ERR_RANGE, "xs_hist: nhist > 2^24 needs XS_F64_BINNING");
   const bool device = flags & XS_DEVICE_PTRS;
   const bool screen = (flags & XS_HIST_SCREEN) != 0;
+  const bool wide = (flags & XS_HIST_WIDE) != 0;
   if (n_atoms == 0) {
     const size_t hsize = (size_t)ntypes_of(n_el) * nhist;
     if (device) {
@@ -1426,15 +1702,15 @@ extern "C" int xs_hist(xs_ctx *ctx, const void *xyz, size_t n_atoms, const unsig
   if (device) {
     if (f64)
       return hist::run_device<double>(ctx, static_cast<const double *>(xyz), n_atoms, el_count, n_el,
-                                      bin, nhist, h, o, screen);
+                                      bin, nhist, h, o, screen, wide);
     return hist::run_device<float>(ctx, static_cast<const float *>(xyz), n_atoms, el_count, n_el, bin,
-                                   nhist, h, o, screen);
+                                   nhist, h, o, screen, wide);
   }
   if (f64)
     return hist::run_host<double>(ctx, static_cast<const double *>(xyz), n_atoms, el_count, n_el, bin,
-                                  nhist, hist, overflow, screen);
+                                  nhist, hist, overflow, screen, wide);
   return hist::run_host<float>(ctx, static_cast<const float *>(xyz), n_atoms, el_count, n_el, bin,
-                               nhist, hist, overflow, screen);
+                               nhist, hist, overflow, screen, wide);
 }
 
 extern "C" int xs_batch_plan(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el,
@@ -1448,9 +1724,12 @@ extern "C" int xs_batch_plan(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el
   for (unsigned s = 0; s < nstruct; ++s)
     if (!xyz[s]) return fail(XS_ERR_ARG, "xs_batch_plan: null structure");
   const bool screen = (flags & XS_HIST_SCREEN) != 0;
+  const bool wide = (flags & XS_HIST_WIDE) != 0;
   if (flags & XS_F64_BINNING)
-    return hist::plan_build<double>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, true, screen, plan);
-  return hist::plan_build<float>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, false, screen, plan);
+    return hist::plan_build<double>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, true, screen, wide,
+                                    plan);
+  return hist::plan_build<float>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, false, screen, wide,
+                                 plan);
 }
 
 extern "C" int xs_batch_free(xs_batch *plan) {
