@@ -42,11 +42,12 @@ NQ = 4096
 FP64_PEAK_TFLOPS = 78.6           # MI355X published FP64 vector peak (binning is FP64)
 VALU_ISSUE_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12   # non-packed lane-ops/s (39.3 T)
 OPS_PER_PAIR = 11                 # 3 sub, 3 mul, 2 add, sqrt, div, cvt
-# FP64-rate VALU issue slots per pair of the histogram kernel's common path
-# (gfx950 assembly: 3 sub, mul + 2 fma, v_rsq_f64 = 4 slots, Newton mul/mul/
-# fma/fma, mul by 1/bin, fract, add, cmp, cvt) and the measured issue rate of
-# v_fma_f64 on this part (8 independent chains per lane, 2,048 work groups).
-SLOTS_PER_PAIR = 19
+# FP64-rate VALU issue slots per pair of the histogram kernel, from rocprofv3
+# SQ counters on this workload: 19.3 VALU wave-instructions per 64 pairs, one
+# of them v_rsq_f64, which issues at a quarter of the v_fma_f64 rate (3 extra
+# slots); and the measured v_fma_f64 issue rate on this part (8 independent
+# chains per lane, 2,048 work groups).
+SLOTS_PER_PAIR = 22.4
 FMA_F64_MEASURED_TOPS = 29.57
 # HBM bytes per launch of hist_kernel<double,32768> on this workload, from
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch), scaled by
@@ -57,7 +58,7 @@ FETCH_SCALE, WRITE_SCALE = 1048576 / 524308, 1.0
 HIST_TRAFFIC_BYTES = int((HIST_FETCH_KIB * FETCH_SCALE + HIST_WRITE_KIB * WRITE_SCALE) * 1024)
 
 
-def gpu_step_setup(ctx, ens, nh, q, f64, screen=False, wide=False):
+def gpu_step_setup(ctx, ens, nh, q, f64, screen=False):
     import torch
     dev = torch.device("cuda", 0)
     xyz = torch.from_numpy(np.ascontiguousarray(ens.xyz64() if f64 else ens.xyz)).to(dev)
@@ -72,7 +73,7 @@ def gpu_step_setup(ctx, ens, nh, q, f64, screen=False, wide=False):
 
     def step():
         ctx.hist_dev(xyz.data_ptr(), ens.n, ens.el_count, BIN, nh, hist.data_ptr(), ovf.data_ptr(),
-                     f64=f64, screen=screen, wide=wide)
+                     f64=f64, screen=screen)
         ctx.hist_pairsum_dev(hist.data_ptr(), P, nh, BIN, qd.data_ptr(), q.size, S.data_ptr())
         ctx.pdf_dev(hist.data_ptr(), ens.el_count, nh, BIN, "rPDF", rho0, sl, pdf.data_ptr())
 
@@ -488,8 +489,6 @@ def main():
                     help="skip the many-histogram pair-sum leg (FP64 vector vs int8 matrix cores)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
-    ap.add_argument("--wide", action="store_true",
-                    help="histogram kernels with two I atoms per thread (same counts)")
     ap.add_argument("--screen", action="store_true",
                     help="headline with the FP32-screened FP64 histogram kernel (same counts)")
     ap.add_argument("--only", choices=["matrix", "2d", "ensemble", "debye"],
@@ -529,12 +528,12 @@ def main():
     ctx = xh.Context(0)
     ctx.set_stream(torch.cuda.current_stream().cuda_stream)
     # headline: FP64 binning, the reference CPU path's expression (bit-exact counts)
-    step, hist, ovf = gpu_step_setup(ctx, ens, nh, q, f64=True, screen=args.screen, wide=args.wide)
+    step, hist, ovf = gpu_step_setup(ctx, ens, nh, q, f64=True, screen=args.screen)
     ms, kern = time_steps(ctx, step, args.steps, args.warmup)
     h = hist.view(3, nh).cpu().numpy()
     totals_ok = [int(v) for v in h.sum(axis=1)] == [na * (na - 1) // 2, na * nb, nb * (nb - 1) // 2]
     del step
-    step32, hist32, _ = gpu_step_setup(ctx, ens, nh, q, f64=False, wide=args.wide)
+    step32, hist32, _ = gpu_step_setup(ctx, ens, nh, q, f64=False)
     ms32, kern32 = time_steps(ctx, step32, max(1, args.steps // 2), 1)
     del step32, hist32
 

@@ -51,8 +51,6 @@ extern "C" {
                                    pairs in FP32 and bin only uncertain ones in FP64
                                    (same counts; experimental, slower on large
                                    histograms) */
-#define XS_HIST_WIDE 0x40u      /* xs_hist, xs_batch_plan: 512-thread work groups, two I
-                                   atoms per thread (same counts) */
 
 /* PDF types, typedefs.h:54-56 */
 #define XS_PDF_RDF 0

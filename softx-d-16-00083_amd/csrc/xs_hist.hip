@@ -233,10 +233,10 @@ __device__ __forceinline__ void box_bins(const double *__restrict__ bi, const do
   ku = (uint32_t)fh;
 }
 
-template <int WCAP, uint32_t NTH = kTile>
+template <int WCAP>
 __device__ __forceinline__ void flush_window(uint32_t *win, uint32_t lo, uint32_t hi,
                                              unsigned long long *__restrict__ hp) {
-  for (uint32_t k = lo + threadIdx.x; k < hi; k += NTH) {
+  for (uint32_t k = lo + threadIdx.x; k < hi; k += kTile) {
     const uint32_t slot = k & (WCAP - 1);
     const uint32_t v = win[slot];
     if (v) {
@@ -525,214 +525,6 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
     }
   }
   flush_window<WCAP>(win, B, min(B + (uint32_t)WCAP, nhist), hp);
-}
-
-// ---------------------------------------------------------------- two I atoms per thread
-//
-// Same items, window, J order and binning as hist_kernel, with 512 threads
-// each holding two I atoms (tid and tid + 512): every J atom read from LDS
-// serves two pairs, and a group of 8 J atoms gives 16 independent chains.
-// Waves whose lanes do not all hold two atoms (at most two per item) take the
-// one-atom loops.
-
-template <class T, int WCAP, bool SAFE>
-__device__ __forceinline__ void pair_group8x2(const T *xi, const T *yi, const T *zi, const T *sx,
-                                              const T *sy, const T *sz, uint32_t ng, T bin,
-                                              double inv_bin, double band, uint32_t nhist,
-                                              uint32_t B, uint32_t wlim, uint32_t *win,
-                                              unsigned long long *__restrict__ hp,
-                                              unsigned long long *__restrict__ ovf) {
-#pragma clang fp contract(off)
-  constexpr bool F64 = sizeof(T) == 8;
-  for (uint32_t j0 = 0; j0 < ng; j0 += 8) {
-    uint32_t k[2][8];
-    bool bad[2][8], any = false;
-#pragma unroll
-    for (int p = 0; p < 8; ++p) {
-      const T xj = sx[j0 + p], yj = sy[j0 + p], zj = sz[j0 + p];
-#pragma unroll
-      for (int r = 0; r < 2; ++r) {
-        const T dx = xi[r] - xj;
-        const T dy = yi[r] - yj;
-        const T dz = zi[r] - zj;
-        T s;
-        if constexpr (F64) {
-          s = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-        } else {
-          s = dx * dx + dy * dy;
-          s = s + dz * dz;
-        }
-        k[r][p] = fast_bin<T>(s, bin, inv_bin, band, nhist, bad[r][p]);
-        any |= bad[r][p];
-      }
-    }
-    if constexpr (F64) {
-      if (__builtin_expect(any, 0)) {
-#pragma unroll
-        for (int r = 0; r < 2; ++r)
-#pragma unroll
-          for (int p = 0; p < 8; ++p) {
-            if (bad[r][p]) {
-              const T dx = xi[r] - sx[j0 + p];
-              const T dy = yi[r] - sy[j0 + p];
-              const T dz = zi[r] - sz[j0 + p];
-              T s = dx * dx + dy * dy;
-              s = s + dz * dz;
-              k[r][p] = exact_bin<T>(s, bin, nhist);
-            }
-          }
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-      for (int p = 0; p < 8; ++p) {
-        if (SAFE || k[r][p] - B < wlim)
-          atomicAdd(&win[k[r][p] & (WCAP - 1)], 1u);
-        else if (k[r][p] < nhist)
-          atomicAdd(&hp[k[r][p]], 1ull);
-        else
-          atomicAdd(ovf, 1ull);
-      }
-  }
-}
-
-template <class T, int WCAP>
-__global__ __launch_bounds__(kTile / 2) void hist_kernel2(Args<T> A) {
-  constexpr uint32_t NTH = kTile / 2;
-  __shared__ uint32_t win[WCAP];
-  __shared__ unsigned long long keys[kSortCap];
-  __shared__ T sx[kStage], sy[kStage], sz[kStage];
-
-  const Item it = A.items[blockIdx.x];
-  const TypeDesc td = A.types[it.type];
-  const uint32_t tid = threadIdx.x;
-  const uint32_t tiI = td.a_base + it.ti;
-  const Tile I = A.tiles[tiI];
-  T xi[2] = {0, 0}, yi[2] = {0, 0}, zi[2] = {0, 0};
-  bool iv[2];
-#pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const uint32_t il = tid + r * NTH;
-    iv[r] = il < I.cnt;
-    if (iv[r]) {
-      const size_t gi = (size_t)I.off + il;
-      xi[r] = A.x[gi];
-      yi[r] = A.y[gi];
-      zi[r] = A.z[gi];
-    }
-  }
-  // wave-uniform: both atoms valid on every lane / only the first on every lane
-  const uint32_t wbase = tid & ~63u;
-  const bool both = wbase + 64 + NTH <= I.cnt;
-  const bool first_only = wbase + NTH >= I.cnt && wbase + 64 <= I.cnt;
-  const T bin = (T)td.bin;
-  const uint32_t nhist = td.nhist;
-  const double *bi = A.boxes + 6 * (size_t)tiI;
-  const uint32_t n = it.jcount;
-  uint32_t npow = 1;
-  while (npow < n) npow <<= 1;
-  unsigned long long *hp = A.hist + td.hist_off;
-  unsigned long long *ovf = A.overflow + td.ovf;
-
-  for (uint32_t m = tid; m < npow; m += NTH) {
-    unsigned long long key = ~0ull;
-    if (m < n) {
-      const uint32_t tj = (it.jstart + m) % td.b_n;
-      uint32_t kl, ku;
-      bool fits;
-      box_bins(bi, A.boxes + 6 * (size_t)(td.b_base + tj), td.bin, nhist, kl, ku, fits);
-      key = ((unsigned long long)kl << 32) | m;
-    }
-    keys[m] = key;
-  }
-  for (uint32_t w = tid; w < WCAP; w += NTH) win[w] = 0u;
-  __syncthreads();
-  for (uint32_t k = 2; k <= npow; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = tid; i < npow; i += NTH) {
-        const uint32_t ixj = i ^ j;
-        if (ixj > i) {
-          const unsigned long long u = keys[i], v = keys[ixj];
-          const bool up = (i & k) == 0;
-          if ((u > v) == up) {
-            keys[i] = v;
-            keys[ixj] = u;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-
-  uint32_t B = (uint32_t)(keys[0] >> 32);
-  for (uint32_t s = 0; s < n; ++s) {
-    const uint32_t m = (uint32_t)(keys[s] & 0xffffffffu);
-    const uint32_t tj = (it.jstart + m) % td.b_n;
-    const uint32_t tjg = td.b_base + tj;
-    uint32_t kl, ku;
-    bool fits;
-    box_bins(bi, A.boxes + 6 * (size_t)tjg, td.bin, nhist, kl, ku, fits);
-    uint32_t wlim = 0;
-    const bool span_ok = ku - kl + 1 <= (uint32_t)WCAP;
-    if (span_ok) {
-      const uint32_t nb = ku + 1 > (uint32_t)WCAP ? ku + 1 - WCAP : 0u;
-      if (nb > B) {
-        flush_window<WCAP, NTH>(win, B, min(min(nb, B + (uint32_t)WCAP), nhist), hp);
-        B = nb;
-        __syncthreads();
-      }
-      wlim = min((uint32_t)WCAP, nhist - B);
-    }
-    const Tile J = A.tiles[tjg];
-    const bool diag = td.same && tj == it.ti;
-    for (uint32_t s0 = 0; s0 < J.cnt; s0 += kStage) {
-      const uint32_t cn = min((uint32_t)kStage, J.cnt - s0);
-      if (tid < cn) {
-        sx[tid] = A.x[(size_t)J.off + s0 + tid];
-        sy[tid] = A.y[(size_t)J.off + s0 + tid];
-        sz[tid] = A.z[(size_t)J.off + s0 + tid];
-      }
-      __syncthreads();
-      const uint32_t ng = cn & ~7u;
-      if (!diag && both) {
-        if (span_ok && fits)
-          pair_group8x2<T, WCAP, true>(xi, yi, zi, sx, sy, sz, ng, bin, td.inv_bin, td.band, nhist, B,
-                                       wlim, win, hp, ovf);
-        else
-          pair_group8x2<T, WCAP, false>(xi, yi, zi, sx, sy, sz, ng, bin, td.inv_bin, td.band, nhist,
-                                        B, wlim, win, hp, ovf);
-#pragma unroll
-        for (int r = 0; r < 2; ++r)
-          pair_block<T, WCAP, false>(xi[r], yi[r], zi[r], tid + r * NTH, sx + ng, sy + ng, sz + ng,
-                                     cn - ng, s0 + ng, bin, td.inv_bin, td.band, nhist, B, wlim, win,
-                                     hp, ovf);
-      } else if (!diag && first_only) {
-        if (span_ok && fits)
-          pair_group8<T, WCAP, true>(xi[0], yi[0], zi[0], sx, sy, sz, ng, bin, td.inv_bin, td.band,
-                                     nhist, B, wlim, win, hp, ovf);
-        else
-          pair_group8<T, WCAP, false>(xi[0], yi[0], zi[0], sx, sy, sz, ng, bin, td.inv_bin, td.band,
-                                      nhist, B, wlim, win, hp, ovf);
-        pair_block<T, WCAP, false>(xi[0], yi[0], zi[0], tid, sx + ng, sy + ng, sz + ng, cn - ng,
-                                   s0 + ng, bin, td.inv_bin, td.band, nhist, B, wlim, win, hp, ovf);
-      } else {
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-          if (iv[r]) {
-            if (diag)
-              pair_block<T, WCAP, true>(xi[r], yi[r], zi[r], tid + r * NTH, sx, sy, sz, cn, s0, bin,
-                                        td.inv_bin, td.band, nhist, B, wlim, win, hp, ovf);
-            else
-              pair_block<T, WCAP, false>(xi[r], yi[r], zi[r], tid + r * NTH, sx, sy, sz, cn, s0, bin,
-                                         td.inv_bin, td.band, nhist, B, wlim, win, hp, ovf);
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-  flush_window<WCAP, NTH>(win, B, min(B + (uint32_t)WCAP, nhist), hp);
 }
 
 // ---------------------------------------------------------------- FP32-screened FP64 kernel
@@ -1263,7 +1055,7 @@ static void sort_structure_host(const T *xyz, size_t n, const unsigned *el_count
 
 template <class T>
 static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wcap, bool prof,
-                       bool screen, bool wide) {
+                       bool screen) {
   if (!nitems) return XS_OK;
   hipEvent_t ev = nullptr;
   if (prof) XS_TRY(prof_begin(ctx, "hist", &ev));
@@ -1279,13 +1071,7 @@ static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wc
       return XS_OK;
     }
   }
-  if (wide) {
-    const dim3 block2(kTile / 2);
-    if (wcap == 8192)
-      hipLaunchKernelGGL((hist_kernel2<T, 8192>), grid, block2, 0, ctx->stream, A);
-    else
-      hipLaunchKernelGGL((hist_kernel2<T, 32768>), grid, block2, 0, ctx->stream, A);
-  } else if (wcap == 8192) {
+  if (wcap == 8192) {
     hipLaunchKernelGGL((hist_kernel<T, 8192>), grid, block, 0, ctx->stream, A);
   } else {
     hipLaunchKernelGGL((hist_kernel<T, 32768>), grid, block, 0, ctx->stream, A);
@@ -1300,7 +1086,7 @@ static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wc
 template <class T>
 static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_count,
                       unsigned n_el, double bin, unsigned nhist, unsigned long long *hist_d,
-                      unsigned long long *ovf_user, bool screen, bool wide) {
+                      unsigned long long *ovf_user, bool screen) {
   const unsigned ntypes = ntypes_of(n_el);
   const size_t hsize = (size_t)ntypes * nhist;
   const uint32_t wcap = wcap_for(nhist);
@@ -1422,13 +1208,12 @@ static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_co
   XS_HIP_CHECK(hipGetLastError());
   XS_TRY(prof_end(ctx, "hist_prep", ev));
   Args<T> A{sxp, syp, szp, tiles_d, boxes, types_d, items_d, hist_d, ovf_d};
-  return launch_hist<T>(ctx, A, items.size(), wcap, true, screen, wide);
+  return launch_hist<T>(ctx, A, items.size(), wcap, true, screen);
 }
 
 template <class T>
 static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_count, unsigned n_el,
-                    double bin, unsigned nhist, uint64_t *hist_out, uint64_t *ovf_out, bool screen,
-                    bool wide) {
+                    double bin, unsigned nhist, uint64_t *hist_out, uint64_t *ovf_out, bool screen) {
   const unsigned ntypes = ntypes_of(n_el);
   const size_t hsize = (size_t)ntypes * nhist;
   {
@@ -1480,7 +1265,7 @@ static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_coun
   hipLaunchKernelGGL(fill_u64, dim3(std::max(zb, 1u)), dim3(256), 0, ctx->stream, hist_d, hsize);
   hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, ctx->stream, ovf_d, (size_t)1);
   Args<T> A{sorted, sorted + n, sorted + 2 * n, tiles_d, boxes_d, types_d, items_d, hist_d, ovf_d};
-  XS_TRY(launch_hist<T>(ctx, A, items.size(), wcap_for(nhist), true, screen, wide));
+  XS_TRY(launch_hist<T>(ctx, A, items.size(), wcap_for(nhist), true, screen));
   XS_HIP_CHECK(hipMemcpyAsync(hist_out, hist_d, hsize * 8, hipMemcpyDeviceToHost, ctx->stream));
   XS_HIP_CHECK(hipMemcpyAsync(ctx->pin, ovf_d, 8, hipMemcpyDeviceToHost, ctx->stream));
   XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -1508,12 +1293,11 @@ static void free_plan(xs_batch *b) {
 template <class T>
 static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const unsigned *el_count,
                       const double *const *xyz, double bin, const unsigned *nhist_in, bool f64,
-                      bool screen, bool wide, xs_batch **out) {
+                      bool screen, xs_batch **out) {
   auto *b = new xs_batch();
   b->ctx = ctx;
   b->f64 = f64;
   b->screen = screen;
-  b->wide = wide;
   b->bin = bin;
   b->nstruct = nstruct;
   std::vector<T> sx, sy, sz, tmp;
@@ -1649,9 +1433,9 @@ static int plan_hist(xs_batch *b, unsigned long long *hist_d, bool prof) {
             items,
             hist_d,
             b->ovf};
-  XS_TRY(launch_hist<T>(ctx, A, b->items8192, 8192, prof, b->screen, b->wide));
+  XS_TRY(launch_hist<T>(ctx, A, b->items8192, 8192, prof, b->screen));
   A.items = items + b->items8192;
-  XS_TRY(launch_hist<T>(ctx, A, b->items32768, 32768, prof, b->screen, b->wide));
+  XS_TRY(launch_hist<T>(ctx, A, b->items32768, 32768, prof, b->screen));
   return XS_OK;
 }
 
@@ -1685,7 +1469,6 @@ extern "C" int xs_hist(xs_ctx *ctx, const void *xyz, size_t n_atoms, const unsig
     return fail(XS_ERR_RANGE, "xs_hist: nhist > 2^24 needs XS_F64_BINNING");
   const bool device = flags & XS_DEVICE_PTRS;
   const bool screen = (flags & XS_HIST_SCREEN) != 0;
-  const bool wide = (flags & XS_HIST_WIDE) != 0;
   if (n_atoms == 0) {
     const size_t hsize = (size_t)ntypes_of(n_el) * nhist;
     if (device) {
@@ -1702,15 +1485,15 @@ extern "C" int xs_hist(xs_ctx *ctx, const void *xyz, size_t n_atoms, const unsig
   if (device) {
     if (f64)
       return hist::run_device<double>(ctx, static_cast<const double *>(xyz), n_atoms, el_count, n_el,
-                                      bin, nhist, h, o, screen, wide);
+                                      bin, nhist, h, o, screen);
     return hist::run_device<float>(ctx, static_cast<const float *>(xyz), n_atoms, el_count, n_el, bin,
-                                   nhist, h, o, screen, wide);
+                                   nhist, h, o, screen);
   }
   if (f64)
     return hist::run_host<double>(ctx, static_cast<const double *>(xyz), n_atoms, el_count, n_el, bin,
-                                  nhist, hist, overflow, screen, wide);
+                                  nhist, hist, overflow, screen);
   return hist::run_host<float>(ctx, static_cast<const float *>(xyz), n_atoms, el_count, n_el, bin,
-                               nhist, hist, overflow, screen, wide);
+                               nhist, hist, overflow, screen);
 }
 
 extern "C" int xs_batch_plan(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el,
@@ -1724,12 +1507,9 @@ extern "C" int xs_batch_plan(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el
   for (unsigned s = 0; s < nstruct; ++s)
     if (!xyz[s]) return fail(XS_ERR_ARG, "xs_batch_plan: null structure");
   const bool screen = (flags & XS_HIST_SCREEN) != 0;
-  const bool wide = (flags & XS_HIST_WIDE) != 0;
   if (flags & XS_F64_BINNING)
-    return hist::plan_build<double>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, true, screen, wide,
-                                    plan);
-  return hist::plan_build<float>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, false, screen, wide,
-                                 plan);
+    return hist::plan_build<double>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, true, screen, plan);
+  return hist::plan_build<float>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, false, screen, plan);
 }
 
 extern "C" int xs_batch_free(xs_batch *plan) {

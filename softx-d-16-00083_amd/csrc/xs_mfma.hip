@@ -159,14 +159,23 @@ __global__ __launch_bounds__(256) void pairsum_mfma_kernel(const v4i *__restrict
     for (int g = 0; g < NG; ++g)
 #pragma unroll
       for (int t = 0; t < NT; ++t) acc[g][t] = v16i{0};
-    for (unsigned kt = kb; kt < ke; ++kt) {
-      v4i a[NA], b[NB][NT];
+    // operands of k tile kt + 1 are loaded while the products of kt run
+    v4i a[NA], b[NB][NT];
 #pragma unroll
-      for (int i = 0; i < NA; ++i) a[i] = ap[i * sA + (size_t)kt * 64];
+    for (int i = 0; i < NA; ++i) a[i] = ap[i * sA + (size_t)kb * 64];
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) b[j][t] = bp[j * sB + ((size_t)t * KT + kb) * 64];
+    for (unsigned kt = kb; kt < ke; ++kt) {
+      v4i an[NA], bn[NB][NT];
+      const unsigned kn = kt + 1 < ke ? kt + 1 : kt;
+#pragma unroll
+      for (int i = 0; i < NA; ++i) an[i] = ap[i * sA + (size_t)kn * 64];
 #pragma unroll
       for (int j = 0; j < NB; ++j)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) b[j][t] = bp[j * sB + ((size_t)t * KT + kt) * 64];
+        for (int t = 0; t < NT; ++t) bn[j][t] = bp[j * sB + ((size_t)t * KT + kn) * 64];
 #pragma unroll
       for (int i = 0; i < NA; ++i)
 #pragma unroll
@@ -175,6 +184,12 @@ __global__ __launch_bounds__(256) void pairsum_mfma_kernel(const v4i *__restrict
           for (int t = 0; t < NT; ++t)
             acc[j - i - DMIN][t] =
                 __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j][t], acc[j - i - DMIN][t], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < NA; ++i) a[i] = an[i];
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) b[j][t] = bn[j][t];
     }
 #pragma unroll
     for (int g = 0; g < NG; ++g) {

@@ -83,10 +83,9 @@ def test_batch_f64_matches_oracle(bctx, ensemble):
     b.close()
 
 
-@pytest.mark.parametrize("wide", [False, True])
-def test_batch_f32_matches_oracle(bctx, ensemble, wide):
+def test_batch_f32_matches_oracle(bctx, ensemble):
     import xansons_hip as xh
-    b = xh.Batch(bctx, [(e.xyz, e.el_count) for e in ensemble], BIN, f64=False, wide=wide)
+    b = xh.Batch(bctx, [(e.xyz, e.el_count) for e in ensemble], BIN, f64=False)
     hist, _, _ = b.run()
     for s, (e, (h, _, _)) in enumerate(zip(ensemble, split(b, ensemble, hist, None, None, 0))):
         ho, _ = O.hist_f32(e.xyz, e.el_count, BIN, int(b.nhist[s]))
