@@ -58,7 +58,7 @@ FETCH_SCALE, WRITE_SCALE = 1048576 / 524308, 1.0
 HIST_TRAFFIC_BYTES = int((HIST_FETCH_KIB * FETCH_SCALE + HIST_WRITE_KIB * WRITE_SCALE) * 1024)
 
 
-def gpu_step_setup(ctx, ens, nh, q, f64, screen=False):
+def gpu_step_setup(ctx, ens, nh, q, f64):
     import torch
     dev = torch.device("cuda", 0)
     xyz = torch.from_numpy(np.ascontiguousarray(ens.xyz64() if f64 else ens.xyz)).to(dev)
@@ -73,7 +73,7 @@ def gpu_step_setup(ctx, ens, nh, q, f64, screen=False):
 
     def step():
         ctx.hist_dev(xyz.data_ptr(), ens.n, ens.el_count, BIN, nh, hist.data_ptr(), ovf.data_ptr(),
-                     f64=f64, screen=screen)
+                     f64=f64)
         ctx.hist_pairsum_dev(hist.data_ptr(), P, nh, BIN, qd.data_ptr(), q.size, S.data_ptr())
         ctx.pdf_dev(hist.data_ptr(), ens.el_count, nh, BIN, "rPDF", rho0, sl, pdf.data_ptr())
 
@@ -153,7 +153,7 @@ def copy_probe(nbytes=1 << 30):
     print(json.dumps({"copy_probe_bytes": nbytes}))
 
 
-def ensemble_leg(ctx, count, steps, warmup, screen=False):
+def ensemble_leg(ctx, count, steps, warmup):
     """Batched (one plan, graph replays) vs structure-by-structure runs of the
     same kernels on `count` nanoparticles: histograms (FP64 binning), pair
     sums on 2,048 q and rPDF per particle.  Both write the same layout, which
@@ -177,7 +177,7 @@ def ensemble_leg(ctx, count, steps, warmup, screen=False):
     sl = np.concatenate([np.linspace(2.0, 9.0, p.n_el) for p in parts])
     bctx = xh.Context(0)          # own non-default stream (graph capture)
     t0 = time.perf_counter()
-    b = xh.Batch(bctx, [(p.xyz64(), p.el_count) for p in parts], bin_, f64=True, screen=screen)
+    b = xh.Batch(bctx, [(p.xyz64(), p.el_count) for p in parts], bin_, f64=True)
     t_plan = time.perf_counter() - t0
     dev = torch.device("cuda", 0)
     qd = torch.from_numpy(q).to(dev)
@@ -221,8 +221,7 @@ def ensemble_leg(ctx, count, steps, warmup, screen=False):
             P = p.n_el * (p.n_el + 1) // 2
             nh = int(b.nhist[s])
             hp = hist2.data_ptr() + 8 * int(b.hist_off[s])
-            bctx.hist_dev(xyz_d[s].data_ptr(), p.n, p.el_count, bin_, nh, hp, None, f64=True,
-                          screen=screen)
+            bctx.hist_dev(xyz_d[s].data_ptr(), p.n, p.el_count, bin_, nh, hp, None, f64=True)
             bctx.hist_pairsum_dev(hp, P, nh, bin_, qd.data_ptr(), nq,
                                   S2.data_ptr() + 8 * int(b.type_first[s]) * nq)
             bctx.pdf_dev(hp, p.el_count, nh, bin_, "rPDF", rho[s], sl[e0:e0 + p.n_el],
@@ -489,8 +488,6 @@ def main():
                     help="skip the many-histogram pair-sum leg (FP64 vector vs int8 matrix cores)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
-    ap.add_argument("--screen", action="store_true",
-                    help="headline with the FP32-screened FP64 histogram kernel (same counts)")
     ap.add_argument("--only", choices=["matrix", "2d", "ensemble", "debye"],
                     help="run one extra leg alone and print its object (for profiling)")
     args = ap.parse_args()
@@ -509,8 +506,7 @@ def main():
         leg = {"matrix": lambda: pairsum_matrix_leg(ctx, max(1, args.steps)),
                "2d": lambda: amp2d_leg(ctx),
                "debye": lambda: debye_leg(ctx),
-               "ensemble": lambda: ensemble_leg(ctx, args.ensemble, max(1, args.steps // 2), 1,
-                                                screen=args.screen)}
+               "ensemble": lambda: ensemble_leg(ctx, args.ensemble, max(1, args.steps // 2), 1)}
         print(json.dumps({args.only: leg[args.only]()}))
         ctx.close()
         return
@@ -528,7 +524,7 @@ def main():
     ctx = xh.Context(0)
     ctx.set_stream(torch.cuda.current_stream().cuda_stream)
     # headline: FP64 binning, the reference CPU path's expression (bit-exact counts)
-    step, hist, ovf = gpu_step_setup(ctx, ens, nh, q, f64=True, screen=args.screen)
+    step, hist, ovf = gpu_step_setup(ctx, ens, nh, q, f64=True)
     ms, kern = time_steps(ctx, step, args.steps, args.warmup)
     h = hist.view(3, nh).cpu().numpy()
     totals_ok = [int(v) for v in h.sum(axis=1)] == [na * (na - 1) // 2, na * nb, nb * (nb - 1) // 2]

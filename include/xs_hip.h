@@ -47,10 +47,6 @@ extern "C" {
 #define XS_PDF_CPU_RADIUS 0x4u  /* xs_pdf: radius offset of the CPU path for p >= 1 */
 #define XS_BATCH_GRAPH 0x8u     /* xs_batch_run: capture once, replay as a hipGraph */
 #define XS_PAIRSUM_MFMA 0x10u   /* xs_hist_pairsum: int8 matrix-core product */
-#define XS_HIST_SCREEN 0x20u    /* xs_hist, xs_batch_plan with XS_F64_BINNING: screen
-                                   pairs in FP32 and bin only uncertain ones in FP64
-                                   (same counts; experimental, slower on large
-                                   histograms) */
 
 /* PDF types, typedefs.h:54-56 */
 #define XS_PDF_RDF 0

@@ -262,6 +262,16 @@ __device__ __forceinline__ void flush_window(uint32_t *win, uint32_t lo, uint32_
 //   bounding-box diagonal spans 2^31 bins.
 // float: the IEEE single-precision expression of the reference GPU kernel,
 //   evaluated directly (correctly rounded sqrtf and division).
+// sqrt(s)/bin from v_rsq_f64 and one Newton step with 1/bin folded in:
+// y = rsq(s), g = s y, t = g y, estimate = g (1.5 - 0.5 t) / bin
+// = g * fma(t, -0.5/bin, 1.5/bin) (four FP64 operations after the rsq).
+__device__ __forceinline__ double sqrt_over_bin(double s, double inv_bin) {
+  const double y = __builtin_amdgcn_rsq(s);
+  const double g = s * y;
+  const double t = g * y;
+  return g * fma(t, -0.5 * inv_bin, 1.5 * inv_bin);
+}
+
 template <class T>
 __device__ __forceinline__ uint32_t bin_index(T s, T bin, double inv_bin, double band,
                                               uint32_t nhist);
@@ -269,10 +279,7 @@ __device__ __forceinline__ uint32_t bin_index(T s, T bin, double inv_bin, double
 template <>
 __device__ __forceinline__ uint32_t bin_index<double>(double s, double bin, double inv_bin,
                                                       double band, uint32_t nhist) {
-  const double y = __builtin_amdgcn_rsq(s);
-  double g = s * y;
-  g = fma(g, fma(-g, 0.5 * y, 0.5), g);
-  const double kd = g * inv_bin;
+  const double kd = sqrt_over_bin(s, inv_bin);
   if (__builtin_expect(fabs(__builtin_amdgcn_fract(kd) - 0.5) < band, 1)) return (uint32_t)kd;
   const double t = __builtin_sqrt(s) / bin;
   return t < (double)nhist ? (uint32_t)t : nhist;
@@ -298,10 +305,7 @@ __device__ __forceinline__ uint32_t fast_bin<double>(double s, double bin, doubl
                                                      double band, uint32_t nhist, bool &bad) {
   (void)bin;
   (void)nhist;
-  const double y = __builtin_amdgcn_rsq(s);
-  double g = s * y;
-  g = fma(g, fma(-g, 0.5 * y, 0.5), g);
-  const double kd = g * inv_bin;
+  const double kd = sqrt_over_bin(s, inv_bin);
   bad = !(fabs(__builtin_amdgcn_fract(kd) - 0.5) < band);
   return (uint32_t)kd;
 }
@@ -520,306 +524,6 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
           pair_block<T, WCAP, false>(xi, yi, zi, tid, sx + ng, sy + ng, sz + ng, cn - ng, s0 + ng, bin,
                                      td.inv_bin, td.band, nhist, B, wlim, win, hp, ovf);
         }
-      }
-      __syncthreads();
-    }
-  }
-  flush_window<WCAP>(win, B, min(B + (uint32_t)WCAP, nhist), hp);
-}
-
-// ---------------------------------------------------------------- FP32-screened FP64 kernel
-//
-// Same work items, window and J order as hist_kernel<double>, with a cheaper
-// common path.  Per J tile the pair is moved into a frame centred between the
-// two tile boxes and scaled to bin units, X = (x - o) / bin, rounded to
-// float.  Each pair's bin is estimated in FP32, t = sqrt(|Xi - Xj|^2)
-// (v_sqrt_f32), and accepted when floor(t) is certain: the distance of t to
-// the nearest integer exceeds the bound E(t) below.  Uncertain pairs (a few
-// per cent) go to a per-wave queue in LDS and are binned 64 at a time by the
-// exact FP64 path of bin_index<double>, with full lanes.
-//
-// Bound.  With R = (r_I + r_J)/bin (box half-diagonals) and e_s the relative
-// error of v_sqrt_f32, rounding the frame coordinates moves t by at most
-// (t + 2R) 2^-24, the subtraction by t 2^-24, the fused sum of squares by
-// 1.5 t 2^-24 and the square root by t e_s.  Measured e_s <= 2^-23 (1 ulp),
-// so |t - t_true| < 5.5 t 2^-24 + 2R 2^-24; E(t) = rho t + sigma doubles both
-// terms and adds 2^-16: rho = 11 * 2^-24, sigma = 4R 2^-24 + 2^-16.  And
-// t_true = sqrt(s)/bin differs from the reference's IEEE quotient by less than
-// 2^-50 t, inside the same margin.  A NaN or zero distance is uncertain.
-
-constexpr int kWaves = kTile / 64;
-constexpr int kQueue = 128;  // entries per wave (a ring; < 64 + 64 in use)
-constexpr float kRho = 11.0f / 16777216.0f;
-
-struct QCtx {
-  const double *sx, *sy, *sz;  // staged J atoms (FP64, for the exact path)
-  uint32_t *qw;                // this wave's queue
-  double bin, inv_bin, band;
-  uint32_t nhist, B, wlim;
-  uint32_t *win;
-  unsigned long long *hp, *ovf;
-};
-
-template <int WCAP, bool SAFE>
-__device__ __forceinline__ void count_pair(const QCtx &c, uint32_t k) {
-  if (SAFE) {
-    atomicAdd(&c.win[k & (WCAP - 1)], 1u);
-  } else if (k - c.B < c.wlim) {
-    atomicAdd(&c.win[k & (WCAP - 1)], 1u);
-  } else if (k < c.nhist) {
-    atomicAdd(&c.hp[k], 1ull);
-  } else {
-    atomicAdd(c.ovf, 1ull);
-  }
-}
-
-// Bin `cnt` (<= 64) queued pairs starting at ring position `head`: lane l
-// takes entry head + l, fetches its I atom from the owning lane and its J
-// atom from LDS, and bins it exactly.  Every lane of the wave takes part.
-template <int WCAP, bool SAFE>
-__device__ __forceinline__ void drain_queue(const QCtx &c, uint32_t head, uint32_t cnt, double xi,
-                                            double yi, double zi) {
-#pragma clang fp contract(off)
-  const uint32_t lane = threadIdx.x & 63;
-  const bool act = lane < cnt;
-  const uint32_t e = act ? c.qw[(head + lane) & (kQueue - 1)] : 0u;
-  const int src = (int)(e >> 8);
-  const double xs = __shfl(xi, src), ys = __shfl(yi, src), zs = __shfl(zi, src);
-  if (act) {
-    const uint32_t j = e & 255u;
-    const double dx = xs - c.sx[j];
-    const double dy = ys - c.sy[j];
-    const double dz = zs - c.sz[j];
-    double s = dx * dx + dy * dy;
-    s = s + dz * dz;
-    count_pair<WCAP, SAFE>(c, bin_index<double>(s, c.bin, c.inv_bin, c.band, c.nhist));
-  }
-}
-
-// J atoms [0, ng) of the staged block, ng a multiple of 8.  FULL: every lane
-// of the wave holds a valid I atom.
-template <int WCAP, bool SAFE, bool FULL>
-__device__ __forceinline__ void pair_group8_q(const QCtx &c, bool valid, float xf, float yf, float zf,
-                                              const float *fx, const float *fy, const float *fz,
-                                              uint32_t ng, float sigma, double xi, double yi,
-                                              double zi, uint32_t &head, uint32_t &cnt) {
-  const uint32_t lane = threadIdx.x & 63;
-  for (uint32_t j0 = 0; j0 < ng; j0 += 8) {
-    float t[8];
-    bool sure[8], unc[8], any = false;
-#pragma unroll
-    for (int p = 0; p < 8; ++p) {
-      const float dx = xf - fx[j0 + p];
-      const float dy = yf - fy[j0 + p];
-      const float dz = zf - fz[j0 + p];
-      t[p] = __builtin_amdgcn_sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)));
-      const float f = __builtin_amdgcn_fractf(t[p]);
-      // certain: |f - 1/2| + E(t) < 1/2 (false for NaN)
-      const bool ok = __builtin_fabsf(f - 0.5f) + __builtin_fmaf(t[p], kRho, sigma) < 0.5f;
-      sure[p] = FULL ? ok : (ok && valid);
-      unc[p] = FULL ? !ok : (valid && !ok);
-      any |= unc[p];
-    }
-#pragma unroll
-    for (int p = 0; p < 8; ++p)
-      if (sure[p]) count_pair<WCAP, SAFE>(c, (uint32_t)t[p]);
-    if (__ballot(any)) {
-#pragma unroll
-      for (int p = 0; p < 8; ++p) {
-        const unsigned long long m = __ballot(unc[p]);
-        if (m) {
-          if (unc[p]) {
-            const uint32_t pos = __builtin_amdgcn_mbcnt_hi(
-                (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            c.qw[(head + cnt + pos) & (kQueue - 1)] = (lane << 8) | (j0 + p);
-          }
-          cnt += (uint32_t)__popcll(m);
-          if (cnt >= 64) {
-            __builtin_amdgcn_wave_barrier();
-            drain_queue<WCAP, SAFE>(c, head, 64, xi, yi, zi);
-            __builtin_amdgcn_wave_barrier();
-            head += 64;
-            cnt -= 64;
-          }
-        }
-      }
-    }
-  }
-}
-
-template <int WCAP>
-__global__ __launch_bounds__(kTile) void hist_kernel_q(Args<double> A) {
-  __shared__ uint32_t win[WCAP];
-  // keys while sorting; afterwards the J order, the FP32 stage and the queues
-  __shared__ __align__(16) unsigned long long scratch[kSortCap];
-  __shared__ double sx[kStage], sy[kStage], sz[kStage];
-  unsigned long long *keys = scratch;
-  uint16_t *order = reinterpret_cast<uint16_t *>(scratch);
-  float *fx = reinterpret_cast<float *>(reinterpret_cast<char *>(scratch) + 4096);
-  float *fy = fx + kStage, *fz = fy + kStage;
-  uint32_t *queues = reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(scratch) + 4096 +
-                                                  3 * kStage * sizeof(float));
-  static_assert(4096 + 3 * kStage * sizeof(float) + kWaves * kQueue * sizeof(uint32_t) <=
-                    kSortCap * sizeof(unsigned long long),
-                "scratch layout");
-
-  const Item it = A.items[blockIdx.x];
-  const TypeDesc td = A.types[it.type];
-  const uint32_t tid = threadIdx.x;
-  const uint32_t tiI = td.a_base + it.ti;
-  const Tile I = A.tiles[tiI];
-  const bool ivalid = tid < I.cnt;
-  double xi = 0, yi = 0, zi = 0;
-  if (ivalid) {
-    const size_t gi = (size_t)I.off + tid;
-    xi = A.x[gi];
-    yi = A.y[gi];
-    zi = A.z[gi];
-  }
-  const uint32_t nhist = td.nhist;
-  const double *bi = A.boxes + 6 * (size_t)tiI;
-  const uint32_t n = it.jcount;
-  uint32_t npow = 1;
-  while (npow < n) npow <<= 1;
-  unsigned long long *hp = A.hist + td.hist_off;
-  unsigned long long *ovf = A.overflow + td.ovf;
-  // lanes of this wave with an I atom: all, some or none
-  const uint32_t wbase = tid & ~63u;
-  const bool wfull = wbase + 64 <= I.cnt, wnone = wbase >= I.cnt;
-
-  for (uint32_t m = tid; m < npow; m += kTile) {
-    unsigned long long key = ~0ull;
-    if (m < n) {
-      const uint32_t tj = (it.jstart + m) % td.b_n;
-      uint32_t kl, ku;
-      bool fits;
-      box_bins(bi, A.boxes + 6 * (size_t)(td.b_base + tj), td.bin, nhist, kl, ku, fits);
-      key = ((unsigned long long)kl << 32) | m;
-    }
-    keys[m] = key;
-  }
-  for (uint32_t w = tid; w < WCAP; w += kTile) win[w] = 0u;
-  __syncthreads();
-  for (uint32_t k = 2; k <= npow; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = tid; i < npow; i += kTile) {
-        const uint32_t ixj = i ^ j;
-        if (ixj > i) {
-          const unsigned long long u = keys[i], v = keys[ixj];
-          const bool up = (i & k) == 0;
-          if ((u > v) == up) {
-            keys[i] = v;
-            keys[ixj] = u;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-  uint32_t B = (uint32_t)(keys[0] >> 32);
-  // compact the sorted order to 16-bit indices (frees the rest of scratch)
-  uint32_t om[kSortCap / kTile];
-#pragma unroll
-  for (int r = 0; r < kSortCap / kTile; ++r) {
-    const uint32_t i = tid + r * kTile;
-    om[r] = i < n ? (uint32_t)(keys[i] & 0xffffu) : 0u;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < kSortCap / kTile; ++r) order[tid + r * kTile] = (uint16_t)om[r];
-  __syncthreads();
-
-  QCtx c;
-  c.sx = sx;
-  c.sy = sy;
-  c.sz = sz;
-  c.qw = queues + (tid >> 6) * kQueue;
-  c.bin = td.bin;
-  c.inv_bin = td.inv_bin;
-  c.band = td.band;
-  c.nhist = nhist;
-  c.win = win;
-  c.hp = hp;
-  c.ovf = ovf;
-  const double rI = 0.5 * sqrt((bi[3] - bi[0]) * (bi[3] - bi[0]) + (bi[4] - bi[1]) * (bi[4] - bi[1]) +
-                               (bi[5] - bi[2]) * (bi[5] - bi[2]));
-
-  for (uint32_t s = 0; s < n; ++s) {
-    const uint32_t m = order[s];
-    const uint32_t tj = (it.jstart + m) % td.b_n;
-    const uint32_t tjg = td.b_base + tj;
-    const double *bj = A.boxes + 6 * (size_t)tjg;
-    uint32_t kl, ku;
-    bool fits;
-    box_bins(bi, bj, td.bin, nhist, kl, ku, fits);
-    uint32_t wlim = 0;
-    const bool span_ok = ku - kl + 1 <= (uint32_t)WCAP;
-    if (span_ok) {
-      const uint32_t nb = ku + 1 > (uint32_t)WCAP ? ku + 1 - WCAP : 0u;
-      if (nb > B) {
-        flush_window<WCAP>(win, B, min(min(nb, B + (uint32_t)WCAP), nhist), hp);
-        B = nb;
-        __syncthreads();
-      }
-      wlim = min((uint32_t)WCAP, nhist - B);
-    }
-    c.B = B;
-    c.wlim = wlim;
-    const bool safe = span_ok && fits;
-    const Tile J = A.tiles[tjg];
-    const bool diag = td.same && tj == it.ti;
-    // frame of this tile pair: origin between the box centres, bin units
-    const double ox = 0.25 * (bi[0] + bi[3] + bj[0] + bj[3]);
-    const double oy = 0.25 * (bi[1] + bi[4] + bj[1] + bj[4]);
-    const double oz = 0.25 * (bi[2] + bi[5] + bj[2] + bj[5]);
-    const double rJ = 0.5 * sqrt((bj[3] - bj[0]) * (bj[3] - bj[0]) + (bj[4] - bj[1]) * (bj[4] - bj[1]) +
-                                 (bj[5] - bj[2]) * (bj[5] - bj[2]));
-    const float sigma = (float)(4.0 * (rI + rJ) * td.inv_bin * 0x1p-24 + 0x1p-16);
-    const float xf = (float)((xi - ox) * td.inv_bin);
-    const float yf = (float)((yi - oy) * td.inv_bin);
-    const float zf = (float)((zi - oz) * td.inv_bin);
-    for (uint32_t s0 = 0; s0 < J.cnt; s0 += kStage) {
-      const uint32_t cn = min((uint32_t)kStage, J.cnt - s0);
-      if (tid < cn) {
-        const size_t g = (size_t)J.off + s0 + tid;
-        const double x = A.x[g], y = A.y[g], z = A.z[g];
-        sx[tid] = x;
-        sy[tid] = y;
-        sz[tid] = z;
-        fx[tid] = (float)((x - ox) * td.inv_bin);
-        fy[tid] = (float)((y - oy) * td.inv_bin);
-        fz[tid] = (float)((z - oz) * td.inv_bin);
-      }
-      __syncthreads();
-      if (diag) {
-        if (ivalid)
-          pair_block<double, WCAP, true>(xi, yi, zi, tid, sx, sy, sz, cn, s0, td.bin, td.inv_bin,
-                                         td.band, nhist, B, wlim, win, hp, ovf);
-      } else if (!wnone) {
-        const uint32_t ng = cn & ~7u;
-        uint32_t head = 0, cnt = 0;
-        if (safe) {
-          if (wfull)
-            pair_group8_q<WCAP, true, true>(c, ivalid, xf, yf, zf, fx, fy, fz, ng, sigma, xi, yi, zi,
-                                            head, cnt);
-          else
-            pair_group8_q<WCAP, true, false>(c, ivalid, xf, yf, zf, fx, fy, fz, ng, sigma, xi, yi, zi,
-                                             head, cnt);
-          __builtin_amdgcn_wave_barrier();
-          if (cnt) drain_queue<WCAP, true>(c, head, cnt, xi, yi, zi);
-        } else {
-          if (wfull)
-            pair_group8_q<WCAP, false, true>(c, ivalid, xf, yf, zf, fx, fy, fz, ng, sigma, xi, yi, zi,
-                                             head, cnt);
-          else
-            pair_group8_q<WCAP, false, false>(c, ivalid, xf, yf, zf, fx, fy, fz, ng, sigma, xi, yi,
-                                              zi, head, cnt);
-          __builtin_amdgcn_wave_barrier();
-          if (cnt) drain_queue<WCAP, false>(c, head, cnt, xi, yi, zi);
-        }
-        if (ivalid)
-          pair_block<double, WCAP, false>(xi, yi, zi, tid, sx + ng, sy + ng, sz + ng, cn - ng, s0 + ng,
-                                          td.bin, td.inv_bin, td.band, nhist, B, wlim, win, hp, ovf);
       }
       __syncthreads();
     }
@@ -1054,23 +758,11 @@ static void sort_structure_host(const T *xyz, size_t n, const unsigned *el_count
 }
 
 template <class T>
-static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wcap, bool prof,
-                       bool screen) {
+static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wcap, bool prof) {
   if (!nitems) return XS_OK;
   hipEvent_t ev = nullptr;
   if (prof) XS_TRY(prof_begin(ctx, "hist", &ev));
   const dim3 grid((unsigned)nitems), block(kTile);
-  if constexpr (sizeof(T) == 8) {
-    if (screen) {
-      if (wcap == 8192)
-        hipLaunchKernelGGL((hist_kernel_q<8192>), grid, block, 0, ctx->stream, A);
-      else
-        hipLaunchKernelGGL((hist_kernel_q<32768>), grid, block, 0, ctx->stream, A);
-      XS_HIP_CHECK(hipGetLastError());
-      if (prof) XS_TRY(prof_end(ctx, "hist", ev));
-      return XS_OK;
-    }
-  }
   if (wcap == 8192) {
     hipLaunchKernelGGL((hist_kernel<T, 8192>), grid, block, 0, ctx->stream, A);
   } else {
@@ -1086,7 +778,7 @@ static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wc
 template <class T>
 static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_count,
                       unsigned n_el, double bin, unsigned nhist, unsigned long long *hist_d,
-                      unsigned long long *ovf_user, bool screen) {
+                      unsigned long long *ovf_user) {
   const unsigned ntypes = ntypes_of(n_el);
   const size_t hsize = (size_t)ntypes * nhist;
   const uint32_t wcap = wcap_for(nhist);
@@ -1208,12 +900,12 @@ static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_co
   XS_HIP_CHECK(hipGetLastError());
   XS_TRY(prof_end(ctx, "hist_prep", ev));
   Args<T> A{sxp, syp, szp, tiles_d, boxes, types_d, items_d, hist_d, ovf_d};
-  return launch_hist<T>(ctx, A, items.size(), wcap, true, screen);
+  return launch_hist<T>(ctx, A, items.size(), wcap, true);
 }
 
 template <class T>
 static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_count, unsigned n_el,
-                    double bin, unsigned nhist, uint64_t *hist_out, uint64_t *ovf_out, bool screen) {
+                    double bin, unsigned nhist, uint64_t *hist_out, uint64_t *ovf_out) {
   const unsigned ntypes = ntypes_of(n_el);
   const size_t hsize = (size_t)ntypes * nhist;
   {
@@ -1265,7 +957,7 @@ static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_coun
   hipLaunchKernelGGL(fill_u64, dim3(std::max(zb, 1u)), dim3(256), 0, ctx->stream, hist_d, hsize);
   hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, ctx->stream, ovf_d, (size_t)1);
   Args<T> A{sorted, sorted + n, sorted + 2 * n, tiles_d, boxes_d, types_d, items_d, hist_d, ovf_d};
-  XS_TRY(launch_hist<T>(ctx, A, items.size(), wcap_for(nhist), true, screen));
+  XS_TRY(launch_hist<T>(ctx, A, items.size(), wcap_for(nhist), true));
   XS_HIP_CHECK(hipMemcpyAsync(hist_out, hist_d, hsize * 8, hipMemcpyDeviceToHost, ctx->stream));
   XS_HIP_CHECK(hipMemcpyAsync(ctx->pin, ovf_d, 8, hipMemcpyDeviceToHost, ctx->stream));
   XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -1293,11 +985,10 @@ static void free_plan(xs_batch *b) {
 template <class T>
 static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const unsigned *el_count,
                       const double *const *xyz, double bin, const unsigned *nhist_in, bool f64,
-                      bool screen, xs_batch **out) {
+                      xs_batch **out) {
   auto *b = new xs_batch();
   b->ctx = ctx;
   b->f64 = f64;
-  b->screen = screen;
   b->bin = bin;
   b->nstruct = nstruct;
   std::vector<T> sx, sy, sz, tmp;
@@ -1433,9 +1124,9 @@ static int plan_hist(xs_batch *b, unsigned long long *hist_d, bool prof) {
             items,
             hist_d,
             b->ovf};
-  XS_TRY(launch_hist<T>(ctx, A, b->items8192, 8192, prof, b->screen));
+  XS_TRY(launch_hist<T>(ctx, A, b->items8192, 8192, prof));
   A.items = items + b->items8192;
-  XS_TRY(launch_hist<T>(ctx, A, b->items32768, 32768, prof, b->screen));
+  XS_TRY(launch_hist<T>(ctx, A, b->items32768, 32768, prof));
   return XS_OK;
 }
 
@@ -1468,7 +1159,6 @@ extern "C" int xs_hist(xs_ctx *ctx, const void *xyz, size_t n_atoms, const unsig
   if (!f64 && nhist > (1u << 24))
     return fail(XS_ERR_RANGE, "xs_hist: nhist > 2^24 needs XS_F64_BINNING");
   const bool device = flags & XS_DEVICE_PTRS;
-  const bool screen = (flags & XS_HIST_SCREEN) != 0;
   if (n_atoms == 0) {
     const size_t hsize = (size_t)ntypes_of(n_el) * nhist;
     if (device) {
@@ -1485,15 +1175,15 @@ extern "C" int xs_hist(xs_ctx *ctx, const void *xyz, size_t n_atoms, const unsig
   if (device) {
     if (f64)
       return hist::run_device<double>(ctx, static_cast<const double *>(xyz), n_atoms, el_count, n_el,
-                                      bin, nhist, h, o, screen);
+                                      bin, nhist, h, o);
     return hist::run_device<float>(ctx, static_cast<const float *>(xyz), n_atoms, el_count, n_el, bin,
-                                   nhist, h, o, screen);
+                                   nhist, h, o);
   }
   if (f64)
     return hist::run_host<double>(ctx, static_cast<const double *>(xyz), n_atoms, el_count, n_el, bin,
-                                  nhist, hist, overflow, screen);
+                                  nhist, hist, overflow);
   return hist::run_host<float>(ctx, static_cast<const float *>(xyz), n_atoms, el_count, n_el, bin,
-                               nhist, hist, overflow, screen);
+                               nhist, hist, overflow);
 }
 
 extern "C" int xs_batch_plan(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el,
@@ -1506,10 +1196,9 @@ extern "C" int xs_batch_plan(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el
   if (!(bin > 0.0) || !std::isfinite(bin)) return fail(XS_ERR_ARG, "xs_batch_plan: bin must be > 0");
   for (unsigned s = 0; s < nstruct; ++s)
     if (!xyz[s]) return fail(XS_ERR_ARG, "xs_batch_plan: null structure");
-  const bool screen = (flags & XS_HIST_SCREEN) != 0;
   if (flags & XS_F64_BINNING)
-    return hist::plan_build<double>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, true, screen, plan);
-  return hist::plan_build<float>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, false, screen, plan);
+    return hist::plan_build<double>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, true, plan);
+  return hist::plan_build<float>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, false, plan);
 }
 
 extern "C" int xs_batch_free(xs_batch *plan) {

@@ -61,7 +61,6 @@ struct BatchRunState {
 struct xs_batch {
   xs_ctx *ctx = nullptr;
   bool f64 = true;
-  bool screen = false;  // FP64 binning: FP32 screening with exact fallback
   double bin = 0;
   unsigned nstruct = 0;
   std::vector<unsigned> n_el, el_count, nhist, ntot;
