@@ -43,17 +43,17 @@ FP64_PEAK_TFLOPS = 78.6           # MI355X published FP64 vector peak (binning i
 VALU_ISSUE_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12   # non-packed lane-ops/s (39.3 T)
 OPS_PER_PAIR = 11                 # 3 sub, 3 mul, 2 add, sqrt, div, cvt
 # FP64-rate VALU issue slots per pair of the histogram kernel, from rocprofv3
-# SQ counters on this workload: 19.3 VALU wave-instructions per 64 pairs, one
+# SQ counters on this workload: 18.2 VALU wave-instructions per 64 pairs, one
 # of them v_rsq_f64, which issues at a quarter of the v_fma_f64 rate (3 extra
 # slots); and the measured v_fma_f64 issue rate on this part (8 independent
 # chains per lane, 2,048 work groups).
-SLOTS_PER_PAIR = 22.4
+SLOTS_PER_PAIR = 21.4
 FMA_F64_MEASURED_TOPS = 29.57
 # HBM bytes per launch of hist_kernel<double,32768> on this workload, from
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch), scaled by
 # the counters' reading for a 1 GiB device-to-device copy (FETCH_SIZE reads
 # 524,308 KiB for the 1,048,576 KiB copied, WRITE_SIZE 1,048,576 KiB).
-HIST_FETCH_KIB, HIST_WRITE_KIB = 5502739, 1665438
+HIST_FETCH_KIB, HIST_WRITE_KIB = 4533620, 1302084
 FETCH_SCALE, WRITE_SCALE = 1048576 / 524308, 1.0
 HIST_TRAFFIC_BYTES = int((HIST_FETCH_KIB * FETCH_SCALE + HIST_WRITE_KIB * WRITE_SCALE) * 1024)
 
@@ -562,7 +562,7 @@ def main():
                      "valu_slot_frac_of_measured_fma_rate": round(
                          SLOTS_PER_PAIR * pairs / (hist_ms * 1e-3) / 1e12 / FMA_F64_MEASURED_TOPS, 4),
                      "traffic_note": "HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE, copy-calibrated); "
-                                     "the J tiles are re-read per I tile from HBM/MALL: ~11 GB at 27 GB/s, "
+                                     "the J tiles are re-read per work item from HBM/MALL: ~9 GB at 26 GB/s, "
                                      "0.3% of HBM bandwidth"},
         "kernels_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in kern.items()},
         "f32_binning": {"ms_per_step": round(ms32, 3),
