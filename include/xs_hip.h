@@ -55,6 +55,7 @@ extern "C" {
 
 typedef struct xs_ctx xs_ctx;
 typedef struct xs_batch xs_batch;
+typedef struct xs_atoms xs_atoms;
 
 /* Open a context on a device and create its stream.
  * Replaces SetDeviceCuda (CalcFunctionsCUDA.cu:1485-1519), which falls back
@@ -204,6 +205,59 @@ int xs_batch_overflow(xs_batch *plan, uint64_t *overflow);
 /* Free a plan.  xs_free frees the plans of its context that are still
  * alive; a plan must not be used or freed after its context is freed. */
 int xs_batch_free(xs_batch *plan);
+
+/* Ensemble builder.  One structural block as block::calcAtoms receives it
+ * after ReadConfig (ReadXML.cpp:254-323): cell atoms made absolute
+ * (redefAtoms, CalcFunctions.cpp:116-127) and sorted element-major
+ * (block::sortAtoms, 158-196), symmetry translations made absolute
+ * (redefSymm, 128-133), copies' rotation rows from calcRotMatrix.
+ * 3-vectors are row-major: r_atom[a*3+k]; p_symm[s*9 + row*3 + col] holds
+ * rows R1, R2, R3 of SymmEqPos s (s = 0 is the identity); cell = a, b, c;
+ * cf[c*9 + row*3 + col] the rotation rows of copy c.  el[a] is the ensemble
+ * element of cell atom a (block::id of its type).  rcut > 0 keeps atoms with
+ * |r - r_mean|^2 <= rcut^2 (CutOff Rcut), rcut_copies > 0 those within
+ * rcut_copies of the mean copy position (CutOff RcutCopies); dev[a] and
+ * dev_mol are standard deviations (sqrt of Uiso, mol_Uiso). */
+typedef struct xs_block_desc {
+  unsigned natom, nsymm, ncopy;
+  unsigned ncell[3];
+  const unsigned *el;
+  const double *r_atom, *occ, *dev, *p_symm, *r_symm, *cell, *r_copy, *cf;
+  double r_mean[3];
+  double rcut, rcut_copies, dev_mol;
+  int mol_rotation;
+} xs_block_desc;
+
+/* Build the atoms of an ensemble of blocks on the device.  Replaces
+ * CalcAndPrintAtoms' loop over block::calcAtoms (CalcFunctions.cpp:398-404,
+ * 251-397): symmetry images of the cell atoms; images closer than 0.5 A
+ * under translations of up to +-2 cells share one position and their
+ * occupancies add up across elements; then, for every copy and cell, one
+ * occupancy draw per covering site places at most one atom, with the atom
+ * and molecule displacements, molecule rotations and cut-offs of the
+ * reference.  Positions are computed in FP64 with the reference's operation
+ * order, so without random terms they equal its atoms bit for bit.  The
+ * clock-seeded default_random_engine of the reference is replaced by
+ * Philox4x64-10 counters keyed by (seed, block), so one seed gives one
+ * ensemble whatever the launch order.  Blocks' descriptions are host memory
+ * and are not kept.  The result stays on the device until xs_atoms_free. */
+int xs_atoms_build(xs_ctx *ctx, const xs_block_desc *blocks, unsigned nblocks, unsigned n_el,
+                   uint64_t seed, xs_atoms **atoms);
+
+/* Counts of the built ensemble: el_count[n_el] (NatomEl of
+ * CalcAndPrintAtoms, 405-408), block_el_count[nblocks*n_el] (NatomElAll of
+ * each block), *n_atoms (Ntot).  Any output may be NULL. */
+int xs_atoms_info(const xs_atoms *atoms, unsigned *el_count, unsigned *block_el_count,
+                  uint64_t *n_atoms);
+
+/* Copy the atoms as element-major SoA xyz[3*n_atoms] (within an element:
+ * block, copy, cell, site order -- the order of the reference's ra[el]).
+ * XS_DEVICE_PTRS: xyz is device memory (enqueued on the context stream),
+ * else host memory (synchronous). */
+int xs_atoms_copy(xs_atoms *atoms, double *xyz, unsigned flags);
+
+/* Free a built ensemble.  xs_free frees those of its context still alive. */
+int xs_atoms_free(xs_atoms *atoms);
 
 #ifdef __cplusplus
 }

@@ -517,3 +517,202 @@ void or_calc_int2d(const double *xyz, const unsigned *el_count, unsigned n_el,
   free(w);
   free(qvec);
 }
+
+/* ---- Ensemble builder --------------------------------------------------- */
+
+void or_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]) {
+  uint64_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint64_t k0 = key[0], k1 = key[1];
+  for (int round = 0; round < 10; ++round) {
+    if (round) {
+      k0 += 0x9E3779B97F4A7C15ULL;
+      k1 += 0xBB67AE8584CAA73BULL;
+    }
+    const unsigned __int128 p0 = (unsigned __int128)0xD2E7470EE14C6C93ULL * c0;
+    const unsigned __int128 p1 = (unsigned __int128)0xCA5A826395121157ULL * c2;
+    const uint64_t hi0 = (uint64_t)(p0 >> 64), lo0 = (uint64_t)p0;
+    const uint64_t hi1 = (uint64_t)(p1 >> 64), lo1 = (uint64_t)p1;
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static double unit_draw(uint64_t x) { return (double)(x >> 11) * 0x1.0p-53; }
+
+static void draws(uint64_t a, uint64_t b, uint64_t c, const uint64_t key[2], uint64_t out[4]) {
+  const uint64_t ctr[4] = {a, b, c, 0};
+  or_philox4x64(ctr, key, out);
+}
+
+/* |N(0, sd)| from two uniforms (Box-Muller). */
+static double half_normal(double sd, uint64_t ur, uint64_t ua) {
+  const double z = sqrt(-2.0 * log(1.0 - unit_draw(ur))) * cos(2.0 * XS_PI * unit_draw(ua));
+  const double v = sd * z;
+  return v < 0 ? -v : v;
+}
+
+/* (x, y, z) = R (sin t cos f, sin t sin f, cos t), CalcFunctions.cpp:334, 375-378 */
+static void spherical(double R, double fi, double theta, double v[3]) {
+  v[0] = R * sin(theta) * cos(fi);
+  v[1] = R * sin(theta) * sin(fi);
+  v[2] = R * cos(theta);
+}
+
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+typedef struct {
+  double occ;
+  unsigned atom, symm, el;
+} or_site_entry;
+
+long long or_calc_atoms(const or_block *blocks, unsigned nblocks, uint64_t seed, long long cap,
+                        unsigned *el_out, unsigned *block_out, double *xyz_out) {
+  long long n = 0;
+  for (unsigned b = 0; b < nblocks; ++b) {
+    const or_block *B = &blocks[b];
+    const uint64_t key[2] = {seed, b};
+    const unsigned M = B->natom * B->nsymm;
+    const double *e = B->cell;
+    /* sites rAtomCell (CalcFunctions.cpp:268-274) */
+    double *rc = (double *)malloc(sizeof(double) * 3 * (M ? M : 1));
+    for (unsigned a = 0; a < B->natom; ++a)
+      for (unsigned s = 0; s < B->nsymm; ++s) {
+        const double *P = B->p_symm + 9 * s, *r = B->r_atom + 3 * a;
+        double *o = rc + 3 * ((size_t)a * B->nsymm + s);
+        for (int k = 0; k < 3; ++k) o[k] = dot3(P + 3 * k, r) + B->r_symm[3 * s + k];
+      }
+    /* occupancy lists occNum (276-319): site i, then every later site j
+     * within 0.5 A under a translation of up to +-2 cells along each axis
+     * with more than one cell, occupancy summed when the element changes */
+    size_t *off = (size_t *)calloc((size_t)M + 1, sizeof(size_t));
+    size_t cap_e = (size_t)M + 16, ne = 0;
+    or_site_entry *ent = (or_site_entry *)malloc(sizeof(or_site_entry) * cap_e);
+    int lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = B->ncell[k] == 1 ? 0 : -2;
+      hi[k] = B->ncell[k] == 1 ? 1 : 3;
+    }
+    for (unsigned i = 0; i < M; ++i) {
+      off[i] = ne;
+      const unsigned ai = i / B->nsymm;
+      if (ne == cap_e) { cap_e *= 2; ent = (or_site_entry *)realloc(ent, sizeof(or_site_entry) * cap_e); }
+      ent[ne++] = (or_site_entry){B->occ[ai], ai, i % B->nsymm, B->el[ai]};
+      for (unsigned j = i + 1; j < M; ++j) {
+        const unsigned aj = j / B->nsymm;
+        for (int X = lo[0]; X < hi[0]; ++X)
+          for (int Y = lo[1]; Y < hi[1]; ++Y)
+            for (int Z = lo[2]; Z < hi[2]; ++Z) {
+              double d[3];
+              for (int k = 0; k < 3; ++k) {
+                const double ct = e[k] * (double)X + e[3 + k] * (double)Y + e[6 + k] * (double)Z;
+                d[k] = rc[3 * i + k] - rc[3 * j + k] + ct;
+              }
+              if (d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < 0.25) {
+                const or_site_entry last = ent[ne - 1];
+                if (ne == cap_e) { cap_e *= 2; ent = (or_site_entry *)realloc(ent, sizeof(or_site_entry) * cap_e); }
+                const double occ = B->el[aj] != last.el ? last.occ + B->occ[aj] : last.occ;
+                ent[ne++] = (or_site_entry){occ, aj, j % B->nsymm, B->el[aj]};
+              }
+            }
+      }
+    }
+    off[M] = ne;
+    /* copies' centre for RcutCopies (263-266) */
+    double rmc[3] = {0, 0, 0};
+    if (B->rcut_copies > 0) {
+      for (unsigned c = 0; c < B->ncopy; ++c)
+        for (int k = 0; k < 3; ++k) rmc[k] += B->r_copy[3 * c + k];
+      for (int k = 0; k < 3; ++k) rmc[k] = rmc[k] / (double)B->ncopy;
+    }
+    /* covering sites: those not listed by an earlier covering site (342-344),
+     * the same in every cell */
+    unsigned char *covered = (unsigned char *)calloc(M ? M : 1, 1);
+    unsigned R = 0;
+    for (unsigned i = 0; i < M; ++i)
+      if (!covered[i]) {
+        ++R;
+        for (size_t k = off[i]; k < off[i + 1]; ++k) covered[(size_t)ent[k].atom * B->nsymm + ent[k].symm] = 1;
+      }
+    const double rcut2 = B->rcut * B->rcut, rcc2 = B->rcut_copies * B->rcut_copies;
+    uint64_t C = 0;
+    for (unsigned c = 0; c < B->ncopy; ++c)
+      for (unsigned X = 0; X < B->ncell[0]; ++X)
+        for (unsigned Y = 0; Y < B->ncell[1]; ++Y)
+          for (unsigned Z = 0; Z < B->ncell[2]; ++Z, ++C) {
+            double ct[3];
+            for (int k = 0; k < 3; ++k) ct[k] = e[k] * (double)X + e[3 + k] * (double)Y + e[6 + k] * (double)Z;
+            memset(covered, 0, M ? M : 1);
+            unsigned r = 0;
+            for (unsigned i = 0; i < M; ++i) {
+              if (covered[i]) continue;
+              const uint64_t L = C * R + r++;
+              uint64_t u[4];
+              draws(L, 0, 0, key, u);
+              const double randval = unit_draw(u[0]);
+              for (size_t k = off[i]; k < off[i + 1]; ++k) covered[(size_t)ent[k].atom * B->nsymm + ent[k].symm] = 1;
+              for (size_t k = off[i]; k < off[i + 1]; ++k) {
+                if (!(randval <= ent[k].occ)) continue;
+                const unsigned ia = ent[k].atom, is = ent[k].symm;
+                const double *P = B->p_symm + 9 * is, *ra = B->r_atom + 3 * ia;
+                double ra0[3], ra1[3], md[3] = {0, 0, 0};
+                for (int q = 0; q < 3; ++q) ra0[q] = dot3(P + 3 * q, ra);
+                if (B->mol_rotation) { /* 331-339, 352 */
+                  uint64_t w[4];
+                  draws(C, 4, is, key, w);
+                  const double al = 2.0 * XS_PI * unit_draw(w[0]), ga = XS_PI * unit_draw(w[1]),
+                               be = 2.0 * XS_PI * unit_draw(w[2]);
+                  const double ca = cos(al), sa = sin(al), cb = cos(be), sb = sin(be), cg = cos(ga), sg = sin(ga);
+                  /* rows of Rz(al) Rx(be) Rz(ga) */
+                  const double Rm[9] = {ca * cg - sa * cb * sg, -ca * sg - sa * cb * cg, sa * sb,
+                                        sa * cg + ca * cb * sg, -sa * sg + ca * cb * cg, -ca * sb,
+                                        sb * sg, sb * cg, cb};
+                  for (int q = 0; q < 3; ++q) ra1[q] = dot3(ra0, Rm + 3 * q);
+                } else {
+                  for (int q = 0; q < 3; ++q) ra1[q] = ra0[q];
+                }
+                if (B->dev_mol >= 1.e-7) { /* 322-329 */
+                  uint64_t w[4], v[4];
+                  draws(C, 2, is, key, w);
+                  draws(C, 3, is, key, v);
+                  spherical(half_normal(B->dev_mol, w[2], v[0]), 2.0 * XS_PI * unit_draw(w[0]),
+                            XS_PI * unit_draw(w[1]), md);
+                }
+                for (int q = 0; q < 3; ++q) ra1[q] += B->r_symm[3 * is + q] + ct[q] + md[q];
+                double u0[3];
+                for (int q = 0; q < 3; ++q) u0[q] = ra1[q] - B->r_mean[q];
+                if (!(B->rcut > 0) || dot3(u0, u0) <= rcut2) { /* 356 */
+                  for (int q = 0; q < 3; ++q) ra1[q] = B->r_copy[3 * c + q] + dot3(u0, B->cf + 9 * c + 3 * q);
+                  if (B->dev[ia] >= 1.e-7) { /* 359-365 */
+                    uint64_t v[4];
+                    double rd[3];
+                    draws(L, 1, 0, key, v);
+                    spherical(half_normal(B->dev[ia], u[3], v[0]), 2.0 * XS_PI * unit_draw(u[1]),
+                              XS_PI * unit_draw(u[2]), rd);
+                    for (int q = 0; q < 3; ++q) ra1[q] += rd[q];
+                  }
+                  double w0[3];
+                  for (int q = 0; q < 3; ++q) w0[q] = ra1[q] - rmc[q];
+                  if (!(B->rcut_copies > 0) || dot3(w0, w0) <= rcc2) { /* 366-369 */
+                    if (n >= cap) { n = -1; goto done; }
+                    el_out[n] = ent[k].el;
+                    block_out[n] = b;
+                    for (int q = 0; q < 3; ++q) xyz_out[3 * n + q] = ra1[q];
+                    ++n;
+                  }
+                }
+                break;
+              }
+            }
+          }
+  done:
+    free(covered);
+    free(ent);
+    free(off);
+    free(rc);
+    if (n < 0) return -1;
+  }
+  return n;
+}

@@ -145,6 +145,52 @@ void or_calc_int2d(const double *xyz, const unsigned *el_count, unsigned n_el,
                    int polar, const double *ff, const double *sl, double *I2D,
                    double *I1D, int nthreads);
 
+/* ---- Ensemble builder --------------------------------------------------- */
+
+/* Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11): ten rounds of the
+ * 4x64 bijection keyed by key[2], out = philox(ctr).  The random draws of
+ * or_calc_atoms (and of the GPU builder) are functions of counters, so a
+ * seed reproduces the same ensemble whatever the launch order. */
+void or_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]);
+
+/* One structural block as block::calcAtoms sees it after ReadConfig
+ * (ReadXML.cpp:254-323) and block::sortAtoms (CalcFunctions.cpp:158-196).
+ * Row-major 3-vectors; p_symm[s*9 + row*3 + col] (rows R1, R2, R3 of
+ * SymmEqPos s), cf[c*9 + row*3 + col] (rotation rows of copy c), cell = a, b, c.
+ * rcut > 0 / rcut_copies > 0 switch the two cut-offs on; dev and dev_mol are
+ * standard deviations (sqrt of Uiso). */
+typedef struct {
+  unsigned natom, nsymm, ncopy;
+  unsigned ncell[3];
+  const unsigned *el;
+  const double *r_atom, *occ, *dev, *p_symm, *r_symm, *cell, *r_copy, *cf;
+  double r_mean[3];
+  double rcut, rcut_copies, dev_mol;
+  int mol_rotation;
+} or_block;
+
+/* block::calcAtoms (CalcFunctions.cpp:251-397) for each block in turn:
+ * sites = symmetry images of the cell atoms (268-274); sites closer than
+ * 0.5 A under translations of up to +-2 cells share a position, their
+ * occupancies accumulated across elements (276-319); then for every copy and
+ * cell, one occupancy draw per not-yet-covered site picks at most one atom
+ * (325-392), with the molecule and atom displacements, molecule rotation and
+ * both cut-offs of the reference.  The reference's clock-seeded
+ * default_random_engine is replaced by Philox4x64-10 counters:
+ *   key = {seed, block}; candidate L = (((copy*Nx + X)*Ny + Y)*Nz + Z)*R + r
+ *   (r = ordinal of the covering site in the cell, R = their number),
+ *   C = L / R;
+ *   philox({L,0,0,0}): occupancy draw, phi, theta, radius uniform of the atom
+ *   displacement; philox({L,1,0,0})[0]: its angle uniform;
+ *   philox({C,2,s,0}): phi, theta, radius uniform of molecule s's
+ *   displacement; philox({C,3,s,0})[0]: its angle uniform;
+ *   philox({C,4,s,0}): molecule s's Euler angles.
+ * u = (x >> 11) 2^-53; |N(0, sd)| = sd |sqrt(-2 ln(1 - u)) cos(2 pi v)|.
+ * Writes atom k's element, block and position in placement order; returns
+ * the number of atoms, or -1 if more than cap. */
+long long or_calc_atoms(const or_block *blocks, unsigned nblocks, uint64_t seed, long long cap,
+                        unsigned *el_out, unsigned *block_out, double *xyz_out);
+
 #ifdef __cplusplus
 }
 #endif

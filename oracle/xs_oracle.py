@@ -60,6 +60,10 @@ def lib():
         L.or_calc_int2d.argtypes = [f64p, u32p, C.c_uint, f64p, C.c_uint, C.c_uint, f64p, f64p,
                                     u32p, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
                                     f64p, f64p, C.c_int]
+        L.or_philox4x64.argtypes = [u64p, u64p, u64p]
+        L.or_calc_atoms.restype = C.c_longlong
+        L.or_calc_atoms.argtypes = [C.c_void_p, C.c_uint, C.c_uint64, C.c_longlong, u32p, u32p,
+                                    f64p]
         _LIB = L
     return _LIB
 
@@ -201,3 +205,47 @@ def calc_int2d(xyz64, el_count, q, nfi, euler_min, euler_max, euler_n, conventio
                         _cu32(euler_n), int(convention), float(lam), int(polar), _ptr(ffa),
                         _ptr(sla), I2D, I1D, _threads(threads))
     return I2D.reshape(q.size, nfi), I1D
+
+
+def philox4x64(ctr, key):
+    out = np.zeros(4, np.uint64)
+    lib().or_philox4x64(np.ascontiguousarray(ctr, np.uint64), np.ascontiguousarray(key, np.uint64),
+                        out)
+    return out
+
+
+class _OrBlock(C.Structure):
+    _fields_ = [("natom", C.c_uint), ("nsymm", C.c_uint), ("ncopy", C.c_uint),
+                ("ncell", C.c_uint * 3), ("el", C.c_void_p), ("r_atom", C.c_void_p),
+                ("occ", C.c_void_p), ("dev", C.c_void_p), ("p_symm", C.c_void_p),
+                ("r_symm", C.c_void_p), ("cell", C.c_void_p), ("r_copy", C.c_void_p),
+                ("cf", C.c_void_p), ("r_mean", C.c_double * 3), ("rcut", C.c_double),
+                ("rcut_copies", C.c_double), ("dev_mol", C.c_double), ("mol_rotation", C.c_int)]
+
+
+def calc_atoms(blocks, seed):
+    """block::calcAtoms over `blocks` (objects with the or_block fields as
+    numpy arrays / numbers).  Returns (el, block, xyz[n, 3]) in placement
+    order; a stable sort by element gives the reference's ra[el] lists."""
+    keep, cs = [], (_OrBlock * len(blocks))()
+    cap = 0
+    for b, B in zip(blocks, cs):
+        arrs = dict(el=_cu32(b.el), r_atom=_c64(b.r_atom), occ=_c64(b.occ), dev=_c64(b.dev),
+                    p_symm=_c64(b.p_symm), r_symm=_c64(b.r_symm), cell=_c64(b.cell),
+                    r_copy=_c64(b.r_copy), cf=_c64(b.cf))
+        keep.append(arrs)
+        B.natom, B.nsymm, B.ncopy = int(b.natom), int(b.nsymm), int(b.ncopy)
+        B.ncell = (C.c_uint * 3)(*[int(v) for v in b.ncell])
+        for k, a in arrs.items():
+            setattr(B, k, a.ctypes.data_as(C.c_void_p).value)
+        B.r_mean = (C.c_double * 3)(*[float(v) for v in b.r_mean])
+        B.rcut, B.rcut_copies, B.dev_mol = float(b.rcut), float(b.rcut_copies), float(b.dev_mol)
+        B.mol_rotation = int(b.mol_rotation)
+        cap += int(b.natom) * int(b.nsymm) * int(b.ncopy) * int(np.prod(b.ncell))
+    el = np.zeros(max(cap, 1), np.uint32)
+    blk = np.zeros(max(cap, 1), np.uint32)
+    xyz = np.zeros(3 * max(cap, 1))
+    n = lib().or_calc_atoms(C.addressof(cs), len(blocks), int(seed) & (2**64 - 1), cap, el, blk, xyz)
+    if n < 0:
+        raise RuntimeError("or_calc_atoms: more atoms than candidate sites")
+    return el[:n], blk[:n], xyz[:3 * n].reshape(n, 3)

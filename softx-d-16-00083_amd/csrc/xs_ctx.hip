@@ -162,6 +162,7 @@ int xs_free(xs_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   while (!ctx->plans.empty()) destroy_batch(*ctx->plans.begin());
+  while (!ctx->atom_sets.empty()) destroy_atoms(*ctx->atom_sets.begin());
   for (auto &r : ctx->prof_pending) {
     (void)hipEventDestroy(r.start);
     (void)hipEventDestroy(r.stop);

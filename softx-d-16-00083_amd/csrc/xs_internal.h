@@ -80,6 +80,16 @@ struct xs_batch {
   xs::BatchRunState run;
 };
 
+// Atoms built by xs_atoms_build: element-major SoA on the device.
+struct xs_atoms {
+  xs_ctx *ctx = nullptr;
+  unsigned n_el = 0, nblocks = 0;
+  std::vector<unsigned> el_count;        // [n_el]
+  std::vector<unsigned> block_el_count;  // [nblocks * n_el]
+  size_t n = 0;
+  double *xyz = nullptr;  // x[n], y[n], z[n]
+};
+
 struct xs_ctx {
   int device = 0;
   hipStream_t own_stream = nullptr;
@@ -102,6 +112,8 @@ struct xs_ctx {
 
   // Batch plans made on this context (freed by xs_free if still alive).
   std::set<xs_batch *> plans;
+  // Built ensembles on this context (freed by xs_free if still alive).
+  std::set<xs_atoms *> atom_sets;
 
   bool profiling = false;
   std::vector<xs::ProfRecord> prof_pending;
@@ -133,6 +145,9 @@ void destroy_batch(xs_batch *b);
 // synchronisation, so they can be captured into a graph.  `prof` wraps them
 // in profiling events (not while capturing).
 int batch_hist_launch(xs_batch *b, unsigned long long *hist_d, bool prof);
+
+// Free a built ensemble's device memory and unregister it from its context.
+void destroy_atoms(xs_atoms *a);
 
 // Pair sums of many histograms: one chunk = up to 2048 bins of one pair type.
 struct PairsumChunk {

@@ -1,10 +1,12 @@
 // HIP drop-in for the XaNSoNS CPU calculators.
 //
 // Defines the four calculators with the CPU signatures that main.cpp declares
-// and calls (main.cpp:65-68, 151-233) and computes them on an MI355X through
-// the C ABI of libxs_hip (include/xs_hip.h).  Compiled against the
+// and calls (main.cpp:65-68, 151-233), and the ensemble builder
+// CalcAndPrintAtoms (main.cpp:38, 130), and computes them on an MI355X
+// through the C ABI of libxs_hip (include/xs_hip.h).  Compiled against the
 // reference's own typedefs.h (config, block, vect3d); linked in place of the
-// CPU calculator definitions (CalcFunctions.cpp:546-1092).
+// CPU calculator definitions (CalcFunctions.cpp:546-1092) and of
+// CalcAndPrintAtoms (CalcFunctions.cpp:398-450).
 //
 // What stays on the host, in FP64, exactly as the CPU path does it:
 //   the form-factor weighting x 2 f_a f_b, the self term N_el f_el^2, the
@@ -27,6 +29,8 @@
 //                        which offsets pair types p >= 1 by p*Nhist bins
 //                        (CalcFunctions.cpp:942, 950); "local": the radius of
 //                        the CUDA/OpenCL kernels (CalcFunctionsCUDA.cu:1057)
+//   XANSONS_HIP_SEED     seed of the ensemble builder's random draws
+//                        (default: the clock, as the reference seeds its own)
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -320,4 +324,104 @@ void calcInt2D(double ***I2D, double **I, const config *cfg, const unsigned int 
     (*I)[iq] = s / Nfi;
   }
   if (!myid) cout << "2D pattern calculation time: " << seconds_since(t0) << " s" << endl;
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble builder.  CalcAndPrintAtoms (CalcFunctions.cpp:398-450) with the
+// signature main.cpp declares (main.cpp:38): the atoms of every block are
+// built on the GPU by xs_atoms_build (block::calcAtoms, 251-397), copied into
+// the reference's ra[el] lists, and the reference's own helpers then compute
+// the bin count and the density and print the atoms as it does.
+// The random draws come from Philox counters keyed by XANSONS_HIP_SEED
+// (default: the clock, like the reference's generator, 259-260).
+
+double GetAtomicDensity(const vector<vect3d<double> > *r, unsigned int Nel, unsigned int Ntot);
+unsigned int GetHistSize(const vector<vect3d<double> > *r, unsigned int Nel, double hist_bin);
+int printAtoms(const vector<vect3d<double> > *ra, string name, map<string, unsigned int> ID,
+               unsigned int Ntot);
+
+static_assert(sizeof(vect3d<double>) == 3 * sizeof(double), "vect3d<double> must be 3 doubles");
+
+unsigned int CalcAndPrintAtoms(config *cfg, block *Block, vector<vect3d<double> > **ra,
+                               unsigned int **NatomEl, map<string, unsigned int> ID) {
+  unsigned int Ntot = 0;
+  *ra = new vector<vect3d<double> >[cfg->Nel];
+  *NatomEl = new unsigned int[cfg->Nel];
+  if (myid) return Ntot;
+  const unsigned nb = cfg->Nblocks, nel = cfg->Nel;
+  std::vector<xs_block_desc> desc(nb);
+  std::vector<std::vector<unsigned> > el(nb);
+  std::vector<std::vector<double> > ps(nb), cf(nb);
+  for (unsigned b = 0; b < nb; ++b) {
+    const block &B = Block[b];
+    for (unsigned t = 0; t < B.Nid; ++t) el[b].insert(el[b].end(), B.NatomEl[t], B.id[t]);
+    for (unsigned s = 0; s < B.Nsymm; ++s)
+      for (int row = 0; row < 3; ++row) {
+        const vect3d<double> &v = B.pSymm[row][s];
+        ps[b].insert(ps[b].end(), {v.x, v.y, v.z});
+      }
+    for (unsigned c = 0; c < B.Ncopy; ++c)
+      for (int row = 0; row < 3; ++row) {
+        const vect3d<double> &v = B.cf[row][c];
+        cf[b].insert(cf[b].end(), {v.x, v.y, v.z});
+      }
+    xs_block_desc &d = desc[b];
+    d.natom = B.Natom;
+    d.nsymm = B.Nsymm;
+    d.ncopy = B.Ncopy;
+    d.ncell[0] = B.Ncell.x;
+    d.ncell[1] = B.Ncell.y;
+    d.ncell[2] = B.Ncell.z;
+    d.el = el[b].data();
+    d.r_atom = &B.rAtom[0].x;
+    d.occ = B.occ;
+    d.dev = B.dev;
+    d.p_symm = ps[b].data();
+    d.r_symm = &B.rSymm[0].x;
+    d.cell = &B.e[0].x;
+    d.r_copy = &B.rCopy[0].x;
+    d.cf = cf[b].data();
+    d.r_mean[0] = B.rMean.x;
+    d.r_mean[1] = B.rMean.y;
+    d.r_mean[2] = B.rMean.z;
+    d.rcut = B.cutoff ? B.Rcut : 0.0;
+    d.rcut_copies = B.cutoffcopies ? B.RcutCopies : 0.0;
+    d.dev_mol = B.dev_mol;
+    d.mol_rotation = B.ro_mol ? 1 : 0;
+  }
+  const char *s = std::getenv("XANSONS_HIP_SEED");
+  const uint64_t seed =
+      s ? std::strtoull(s, nullptr, 10)
+        : (uint64_t)std::chrono::system_clock::now().time_since_epoch().count();
+  xs_atoms *A = nullptr;
+  check(xs_atoms_build(ctx(), desc.data(), nb, nel, seed, &A), "xs_atoms_build");
+  std::vector<unsigned> el_count(nel), block_el_count((size_t)nb * nel);
+  uint64_t n = 0;
+  check(xs_atoms_info(A, el_count.data(), block_el_count.data(), &n), "xs_atoms_info");
+  std::vector<double> xyz(3 * n);
+  check(xs_atoms_copy(A, xyz.data(), 0), "xs_atoms_copy");
+  xs_atoms_free(A);
+  size_t o = 0;
+  for (unsigned e = 0; e < nel; ++e) {
+    (*ra)[e].resize(el_count[e]);
+    for (unsigned i = 0; i < el_count[e]; ++i, ++o)
+      (*ra)[e][i].assign(xyz[o], xyz[n + o], xyz[2 * n + o]);
+    (*NatomEl)[e] = el_count[e];
+    Ntot += el_count[e];
+  }
+  for (unsigned b = 0; b < nb; ++b) {  // NatomElAll and N of each block (CalcFunctions.cpp:320, 394)
+    block &B = Block[b];
+    B.N = 0;
+    for (unsigned t = 0; t < B.Nid; ++t) {
+      B.NatomElAll[t] = block_el_count[(size_t)b * nel + B.id[t]];
+      B.N += B.NatomElAll[t];
+    }
+  }
+  if (cfg->scenario > Debye) cfg->Nhist = GetHistSize(*ra, cfg->Nel, cfg->hist_bin);
+  if (cfg->PrintAtoms) printAtoms(*ra, cfg->name, ID, Ntot);
+  if ((cfg->scenario > Debye_hist) && (!cfg->p0)) {
+    cfg->p0 = GetAtomicDensity(*ra, cfg->Nel, Ntot);
+    cout << "Approximate atomic density of the sample is " << cfg->p0 << " A^(-3).\n" << endl;
+  }
+  return Ntot;
 }

@@ -71,6 +71,10 @@ def lib():
         L.xs_batch_run.argtypes = [vp, vp, vp, u, vp, i, vp, vp, vp, u]
         L.xs_batch_overflow.argtypes = [vp, vp]
         L.xs_batch_free.argtypes = [vp]
+        L.xs_atoms_build.argtypes = [vp, vp, u, u, C.c_uint64, C.POINTER(vp)]
+        L.xs_atoms_info.argtypes = [vp, vp, vp, vp]
+        L.xs_atoms_copy.argtypes = [vp, vp, u]
+        L.xs_atoms_free.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -105,11 +109,12 @@ class Context:
     def __init__(self, device: int = 0):
         self.h = C.c_void_p()
         self._batches = weakref.WeakSet()
+        self._atoms = weakref.WeakSet()
         _check(lib().xs_init(int(device), C.byref(self.h)))
 
     def close(self):
-        """Free the context; batch plans made on it are freed first."""
-        for b in list(self._batches):
+        """Free the context; batch plans and atom sets made on it are freed first."""
+        for b in list(self._batches) + list(self._atoms):
             b.close()
         if self.h:
             lib().xs_free(self.h)

@@ -122,3 +122,17 @@ def test_2d(tmp, conv):
     I2 = np.loadtxt(os.path.join(tmp, "D_neut_2D.txt"), ndmin=2)
     assert rel_err(I2, g[f"I2D_{conv}"])[0] < 2e-6
     assert rel_err(col(tmp, "D_neut_1D.txt"), g[f"I1D_{conv}"])[0] < 2e-6
+
+
+@needs_exe
+@pytest.mark.parametrize("case", ["rocksalt", "hex_copies", "shared_sites", "single_cell"])
+def test_atoms_file_equals_reference(tmp, case):
+    """The drop-in's CalcAndPrintAtoms (GPU builder) prints the same
+    <name>_atoms.xyz as the reference's CPU builder, byte for byte."""
+    g = np.load(os.path.join(REPO, "tests", "golden", f"atoms_{case}.npz"))
+    with open(os.path.join(tmp, "sl.txt"), "w") as f:
+        f.write(str(g["sl"]))
+    out = run(tmp, str(g["xml"]))
+    assert "Total calculation time" in out
+    with open(os.path.join(tmp, f"{case}_atoms.xyz")) as f:
+        assert f.read() == str(g["xyz_text"])
