@@ -475,6 +475,107 @@ def debye_leg(ctx):
     return out
 
 
+def atoms_workloads(n_cells=150, n_single=60000):
+    """Ensemble-builder workloads: (name, blocks, n_el, files for the reference).
+
+    rocksalt_sphere: rock-salt cell given as a 2-atom basis, three fcc
+    centring translations and an inversion (10 images per cell, 8 after the
+    0.5 A merge), n_cells^3 cells, cut to a sphere around the origin.
+    single_cell: a finished disordered structure of n_single atoms plus 5 %
+    near-duplicates given as one cell, the O(N^2) merge of the reference."""
+    from xansons_hip.atoms import Block
+    sl = "Na 3.63\nCl 9.577\n"
+    calc = ('  <Calculation name="b" source="neutron" scenario="Debye" FFfilename="sl.txt">\n'
+            '    <q min="0.5" max="5" N="4"/>\n  </Calculation>\n')
+    a, half = 5.64, n_cells // 2
+    rcut = half * a
+    I = np.eye(3)
+    symm_f = [((0.5, 0.5, 0.0), I), ((0.5, 0.0, 0.5), I), ((0.0, 0.5, 0.5), I), ((0.0, 0.0, 0.0), -I)]
+    basis = np.array([[-half, -half, -half], [-half + 0.5, -half, -half]], dtype=np.float64) * a
+    rs = Block(r_atom=basis, el=[0, 1],
+               p_symm=np.array([I] + [P for _, P in symm_f]),
+               r_symm=np.array([[0.0, 0.0, 0.0]] + [np.array(t) * a for t, _ in symm_f]),
+               cell=np.eye(3) * a, ncell=(n_cells,) * 3, rcut=rcut)
+    v = lambda x: " ".join(repr(float(c)) for c in x)  # noqa: E731
+    xml_rs = ('<?xml version="1.0"?>\n<Config>\n' + calc + '  <Block>\n    <Atoms>\n' +
+              f'      <Atom name="Na" r="{v(basis[0])}"/>\n      <Atom name="Cl" r="{v(basis[1])}"/>\n'
+              '    </Atoms>\n' +
+              "".join(f'    <SymmEqPos r="{v(t)}" R1="{v(P[0])}" R2="{v(P[1])}" R3="{v(P[2])}"/>\n'
+                      for t, P in symm_f) +
+              f'    <CutOff Rcut="{rcut!r}"/>\n'
+              f'    <CellVectors a="{a!r} 0 0" b="0 {a!r} 0" c="0 0 {a!r}" N="{n_cells} {n_cells} {n_cells}"/>\n'
+              '  </Block>\n</Config>\n')
+    rng = np.random.default_rng(60000)
+    nd = n_single // 20
+    r = rng.uniform(0, (n_single / 0.06) ** (1 / 3), (n_single, 3))
+    r = np.concatenate([r, r[rng.choice(n_single, nd, replace=False)] +
+                        rng.normal(size=(nd, 3)) * 0.12])
+    el = rng.integers(0, 2, n_single + nd)
+    names = np.array(["Na", "Cl"])
+    atoms_txt = "".join(f"{names[e]} {float(x)!r} {float(y)!r} {float(z)!r}\n"
+                        for e, (x, y, z) in zip(el, r))
+    order = np.argsort(el, kind="stable")   # block::sortAtoms groups by element, stably
+    sc = Block(r_atom=r[order], el=el[order])
+    xml_sc = ('<?xml version="1.0"?>\n<Config>\n' + calc +
+              '  <Block>\n    <Atoms filename="atoms.txt"/>\n  </Block>\n</Config>\n')
+    return [("rocksalt_sphere", [rs], 2, {"start.xml": xml_rs, "sl.txt": sl}),
+            ("single_cell", [sc], 2, {"start.xml": xml_sc, "sl.txt": sl, "atoms.txt": atoms_txt})]
+
+
+def reference_atoms_seconds(files):
+    """The reference's CalcAndPrintAtoms (block::calcAtoms, one thread) on the
+    same input through oracle/_ref/ref_atoms; (seconds, el_count) or None."""
+    exe = os.path.join(REPO, "oracle", "_ref", "ref_atoms")
+    if not os.path.exists(exe):
+        return None
+    tmp = tempfile.mkdtemp(prefix="xs_atoms_ref_")
+    try:
+        for fn, text in files.items():
+            with open(os.path.join(tmp, fn), "w") as f:
+                f.write(text)
+        subprocess.run([exe, "start.xml", "out.json", "--no-atoms"], cwd=tmp, check=True,
+                       stdout=subprocess.DEVNULL, timeout=600)
+        with open(os.path.join(tmp, "out.json")) as f:
+            d = json.load(f)
+        return d["seconds"], [int(c) for c in d["el_count"]]
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def atoms_leg(ctx, cpu=True, reps=3):
+    """GPU ensemble builder (xs_atoms_build, host blocks in, element-major
+    atoms on the device out) vs the reference's builder on the same input."""
+    from xansons_hip.atoms import Atoms
+    out = {}
+    for name, blocks, n_el, files in atoms_workloads():
+        Atoms(ctx, blocks, n_el, seed=1).close()  # warm-up (module load, allocations)
+        ts = []
+        for rep in range(reps):
+            ctx.profile(True)
+            t0 = time.perf_counter()
+            a = Atoms(ctx, blocks, n_el, seed=1 + rep)
+            ts.append(time.perf_counter() - t0)
+            phases = {k: round(ctx.profile_read(k)[0], 3)
+                      for k in ("atoms_merge", "atoms_place", "atoms_order")}
+            ctx.profile(False)
+            counts = [int(c) for c in a.el_count]
+            a.close()
+        t = float(np.median(ts))
+        cand = sum(b.candidates() for b in blocks)
+        n = sum(counts)
+        rec = {"candidates": cand, "atoms": n, "ms": round(t * 1e3, 3), "phases_ms": phases,
+               "matoms_per_s": round(n / t / 1e6, 1)}
+        if cpu:
+            ref = reference_atoms_seconds(files)
+            if ref is not None:
+                rec["reference_s"] = round(ref[0], 3)
+                rec["reference_cores"] = 1
+                rec["speedup"] = round(ref[0] / t, 1)
+                rec["el_count_equal_reference"] = ref[1] == counts
+        out[name] = rec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -489,7 +590,8 @@ def main():
                     help="skip the many-histogram pair-sum leg (FP64 vector vs int8 matrix cores)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
-    ap.add_argument("--only", choices=["matrix", "2d", "ensemble", "debye"],
+    ap.add_argument("--no-atoms", action="store_true", help="skip the ensemble-builder leg")
+    ap.add_argument("--only", choices=["matrix", "2d", "ensemble", "debye", "atoms"],
                     help="run one extra leg alone and print its object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -507,6 +609,7 @@ def main():
         leg = {"matrix": lambda: pairsum_matrix_leg(ctx, max(1, args.steps)),
                "2d": lambda: amp2d_leg(ctx),
                "debye": lambda: debye_leg(ctx),
+               "atoms": lambda: atoms_leg(ctx, cpu=not args.no_cpu_baseline),
                "ensemble": lambda: ensemble_leg(ctx, args.ensemble, max(1, args.steps // 2), 1)}
         print(json.dumps({args.only: leg[args.only]()}))
         ctx.close()
@@ -581,6 +684,8 @@ def main():
         del hist, ovf
         torch.cuda.empty_cache()
         line["ensemble"] = ensemble_leg(ctx, args.ensemble, max(1, args.steps // 2), 1)
+    if not args.no_atoms:
+        line["atoms_builder"] = atoms_leg(ctx, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(ens, nh)
     ctx.close()

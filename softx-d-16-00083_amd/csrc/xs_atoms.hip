@@ -250,23 +250,33 @@ struct PlaceArgs {
 };
 
 __global__ __launch_bounds__(kThreads) void place_atoms(PlaceArgs A) {
-  const uint64_t L = blockIdx.x * (uint64_t)kThreads + threadIdx.x;
+  // candidate counts stay below 2^31 (checked by the host), so the index
+  // arithmetic is 32-bit
+  const uint32_t L = blockIdx.x * (uint32_t)kThreads + threadIdx.x;
   if (L >= A.ncand) return;
-  const uint64_t C = L / A.nrep;  // (copy, cell) index
-  const unsigned r = (unsigned)(L - C * A.nrep);
-  uint64_t t = C;
-  const unsigned Z = (unsigned)(t % A.ncell[2]);
+  const uint32_t C = L / A.nrep;  // (copy, cell) index
+  const unsigned r = L - C * A.nrep;
+  uint32_t t = C;
+  const unsigned Z = t % A.ncell[2];
   t /= A.ncell[2];
-  const unsigned Y = (unsigned)(t % A.ncell[1]);
+  const unsigned Y = t % A.ncell[1];
   t /= A.ncell[1];
-  const unsigned X = (unsigned)(t % A.ncell[0]);
-  const uint64_t copy = t / A.ncell[0];
+  const unsigned X = t % A.ncell[0];
+  const uint32_t copy = t / A.ncell[0];
   uint32_t key = A.key_none;
-  const P4 d = philox(L, 0, 0, A.seed, A.block);
-  const double randval = unit(d.v[0]);
   const unsigned e1 = A.rep_off[r + 1];
   unsigned k = A.rep_off[r];
-  while (k < e1 && !(randval <= A.ent[k].occ)) ++k;  // CalcFunctions.cpp:346
+  // The occupancy draw is in [0, 1): an entry with occ >= 1 at the head of
+  // the list is taken whatever it is, so the draw is only made otherwise
+  // (counter-based, so drawing later gives the same numbers).
+  P4 d;
+  bool drawn = false;
+  if (!(A.ent[k].occ >= 1.0)) {
+    d = philox(L, 0, 0, A.seed, A.block);
+    drawn = true;
+    const double randval = unit(d.v[0]);
+    while (k < e1 && !(randval <= A.ent[k].occ)) ++k;  // CalcFunctions.cpp:346
+  }
   if (k < e1) {
     const Entry E = A.ent[k];
     const double *ra = A.r_atom + 3 * (size_t)E.atom;
@@ -316,6 +326,7 @@ __global__ __launch_bounds__(kThreads) void place_atoms(PlaceArgs A) {
       double z2 = rc[2] + (ux * F[6] + uy * F[7] + uz * F[8]);
       const double sd = A.dev[E.atom];
       if (sd >= 1.e-7) {  // CalcFunctions.cpp:357-363
+        if (!drawn) d = philox(L, 0, 0, A.seed, A.block);
         const P4 v = philox(L, 1, 0, A.seed, A.block);
         const double R = half_normal(sd, d.v[3], v.v[0]);
         const double fi = 2.0 * kPi * unit(d.v[1]), th = kPi * unit(d.v[2]);
