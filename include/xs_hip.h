@@ -161,6 +161,26 @@ int xs_amp2d(xs_ctx *ctx, const double *xyz, size_t n_atoms,
              size_t nqp, const double *rot, unsigned nrot, const double *ff,
              double *I, unsigned flags);
 
+/* Cut-off pair sums of RearrangementInt (CalcFunctions.cpp:485-499):
+ * S[(el[i]*n_el + el[j])*nq + iq] = sum over ordered pairs (i, j) with
+ * 1e-6 < r_ij^2 < rmax2[i] of sin(q r_ij)/(q r_ij + 1e-8), r_ij^2 =
+ * (dx^2 + dy^2) + dz^2 in FP64 as the reference computes it (so the same
+ * pairs are accepted); npairs[el[i]*n_el + el[j]] counts them.  xyz is SoA
+ * (n_atoms), el[i] < n_el <= 64.  S (n_el^2 * nq) and npairs (n_el^2) are
+ * overwritten.  Host arrays only (synchronous).  Replaces the pair loop of
+ * RearrangementInt; its caller keeps the weighting and normalisation. */
+int xs_cutoff_pairsum(xs_ctx *ctx, const double *xyz, size_t n_atoms, const unsigned *el,
+                      unsigned n_el, const double *rmax2, const double *q, unsigned nq,
+                      double *S, uint64_t *npairs, unsigned flags);
+
+/* Sine sums over a list of distances: S[type[p]*nq + iq] = sum_p
+ * sin(q r_p)/(q r_p + 1e-8), S (ntypes * nq) overwritten.  Host arrays only
+ * (synchronous).  Replaces the neighbour loop of RearrangementInt
+ * (CalcFunctions.cpp:506-529). */
+int xs_pairlist_pairsum(xs_ctx *ctx, const double *r, const unsigned *type, size_t npairs,
+                        unsigned ntypes, const double *q, unsigned nq, double *S,
+                        unsigned flags);
+
 /* Ensembles.  The reference computes one structure per run of main
  * (main.cpp:151-233: calcPDFandDebye per configuration); an ensemble of
  * particles is a sequence of such runs.  A batch plan holds every structure

@@ -11,7 +11,13 @@
 // steady clock, and prints the atoms it placed.  Doubles are printed as C99
 // hex floats, so the output is exact.
 //
-//   ref_atoms start.xml out.json [--no-atoms]
+//   ref_atoms start.xml out.json [--no-atoms] [--rearrangement]
+//
+// With --rearrangement it also runs the reference's RearrangementInt
+// (CalcFunctions.cpp:469-545) on I0[k] = 1 + k/64 (length (1 + Nparts) Nq
+// with PartialIntensity, else Nq), timed, and adds "q", "sl", "ff",
+// "rearr_seconds", "rearr_I" and per block "rcorr" and "neighbours"
+// ([[el, x, y, z], ...] per cell atom, in the block's atom order).
 //
 // Output (one JSON object in out.json; ReadConfig's messages go to stdout): {"nel": .., "blocks": [..], "seconds": ..,
 // "el_count": [..], "block_el_count": [[..]..], "atoms": [[x, y, z, ...] per
@@ -26,6 +32,8 @@ int myid = 0, numprocs = 1;
 
 int ReadConfig(config *cfg, double **q, block **Block, const char *FileName, string elements[],
                map<string, unsigned int> *ID, vector<double> *SL, vector<double *> *FF);
+void RearrangementInt(double *I, const double *q, const config *cfg, const block *Block,
+                      vector<double *> FF, vector<double> SL, unsigned int Ntot);
 unsigned int CalcAndPrintAtoms(config *cfg, block *Block, vector<vect3d<double> > **ra,
                                unsigned int **NatomEl, map<string, unsigned int> ID);
 
@@ -69,7 +77,23 @@ static void dump_block(FILE *f, const block &B) {
   hex(f, B.cutoffcopies ? B.RcutCopies : 0.0);
   fprintf(f, ",\"dev_mol\":");
   hex(f, B.dev_mol);
-  fprintf(f, ",\"mol_rotation\":%d}", B.ro_mol ? 1 : 0);
+  fprintf(f, ",\"mol_rotation\":%d", B.ro_mol ? 1 : 0);
+  if (B.rearrangement) {
+    fprintf(f, ",\"rcorr\":");
+    hex(f, B.Rcorr);
+    fprintf(f, ",\"neighbours\":[");
+    for (unsigned i = 0; i < B.Natom; ++i) {
+      fprintf(f, "%s[", i ? "," : "");
+      for (size_t k = 0; k < B.rAtomNeighb[i].size(); ++k) {
+        fprintf(f, "%s[%u,", k ? "," : "", B.idNeighb[i][k]);
+        hex_vec(f, B.rAtomNeighb[i][k]);
+        fputc(']', f);
+      }
+      fputc(']', f);
+    }
+    fputc(']', f);
+  }
+  fputc('}', f);
 }
 
 int main(int argc, char **argv) {
@@ -77,7 +101,11 @@ int main(int argc, char **argv) {
     fprintf(stderr, "usage: %s start.xml out.json [--no-atoms]\n", argv[0]);
     return 2;
   }
-  const bool atoms = !(argc > 3 && !strcmp(argv[3], "--no-atoms"));
+  bool atoms = true, rearr = false;
+  for (int a = 3; a < argc; ++a) {
+    if (!strcmp(argv[a], "--no-atoms")) atoms = false;
+    if (!strcmp(argv[a], "--rearrangement")) rearr = true;
+  }
   config cfg;
   block *Block = NULL;
   double *q = NULL;
@@ -122,7 +150,37 @@ int main(int argc, char **argv) {
       for (size_t i = 0; i < ra[e].size(); ++i) { if (i) fputc(',', f); hex_vec(f, ra[e][i]); }
     fprintf(f, "]");
   }
-  fprintf(f, "]}\n");
+  fprintf(f, "]");
+  if (rearr) {
+    const unsigned Nq = cfg.q.N;
+    const unsigned nparts = cfg.Nblocks * (cfg.Nblocks + 1) / 2;
+    const size_t len = cfg.calcPartialIntensity ? (size_t)(1 + nparts) * Nq : Nq;
+    vector<double> I(len);
+    for (size_t k = 0; k < len; ++k) I[k] = 1.0 + (double)k / 64.0;
+    unsigned Ntot = 0;
+    for (unsigned e = 0; e < cfg.Nel; ++e) Ntot += NatomEl[e];
+    const auto t1 = std::chrono::steady_clock::now();
+    RearrangementInt(I.data(), q, &cfg, Block, FF, SL, Ntot);
+    const double rs =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+    fprintf(f, ",\"source\":%u,\"partial\":%d,\"polar\":%d,\"lambda\":", cfg.source,
+            cfg.calcPartialIntensity ? 1 : 0, cfg.PolarFactor ? 1 : 0);
+    hex(f, cfg.lambda);
+    fprintf(f, ",\"q\":[");
+    for (unsigned k = 0; k < Nq; ++k) { if (k) fputc(',', f); hex(f, q[k]); }
+    fprintf(f, "],\"sl\":[");
+    for (size_t e = 0; e < SL.size(); ++e) { if (e) fputc(',', f); hex(f, SL[e]); }
+    fprintf(f, "],\"ff\":[");
+    for (size_t e = 0; e < FF.size(); ++e) {
+      fprintf(f, "%s[", e ? "," : "");
+      for (unsigned k = 0; k < Nq; ++k) { if (k) fputc(',', f); hex(f, FF[e][k]); }
+      fputc(']', f);
+    }
+    fprintf(f, "],\"rearr_seconds\":%.6f,\"rearr_I\":[", rs);
+    for (size_t k = 0; k < len; ++k) { if (k) fputc(',', f); hex(f, I[k]); }
+    fputc(']', f);
+  }
+  fprintf(f, "}\n");
   fclose(f);
   return 0;
 }

@@ -716,3 +716,37 @@ long long or_calc_atoms(const or_block *blocks, unsigned nblocks, uint64_t seed,
   }
   return n;
 }
+
+/* ---- RearrangementInt pair sums ----------------------------------------- */
+
+void or_cutoff_pairsum(const double *xyz, size_t n, const unsigned *el, unsigned n_el,
+                       const double *rmax2, const double *q, unsigned nq, double *S,
+                       uint64_t *npairs) {
+  const double *x = xyz, *y = xyz + n, *z = xyz + 2 * n;
+  memset(S, 0, sizeof(double) * (size_t)n_el * n_el * nq);
+  memset(npairs, 0, sizeof(uint64_t) * (size_t)n_el * n_el);
+  for (size_t i = 0; i < n; ++i)
+    for (size_t j = 0; j < n; ++j) {
+      const double dx = x[i] - x[j], dy = y[i] - y[j], dz = z[i] - z[j];
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 < rmax2[i] && r2 > 0.000001) {
+        const size_t t = (size_t)el[i] * n_el + el[j];
+        const double r = sqrt(r2);
+        ++npairs[t];
+        for (unsigned k = 0; k < nq; ++k) {
+          const double qr = r * q[k];
+          S[t * nq + k] += sin(qr) / (qr + 0.00000001);
+        }
+      }
+    }
+}
+
+void or_pairlist_pairsum(const double *r, const unsigned *type, size_t np, unsigned ntypes,
+                         const double *q, unsigned nq, double *S) {
+  memset(S, 0, sizeof(double) * (size_t)ntypes * nq);
+  for (size_t p = 0; p < np; ++p)
+    for (unsigned k = 0; k < nq; ++k) {
+      const double qr = r[p] * q[k];
+      S[(size_t)type[p] * nq + k] += sin(qr) / (qr + 0.00000001);
+    }
+}

@@ -191,6 +191,23 @@ typedef struct {
 long long or_calc_atoms(const or_block *blocks, unsigned nblocks, uint64_t seed, long long cap,
                         unsigned *el_out, unsigned *block_out, double *xyz_out);
 
+/* ---- RearrangementInt pair sums ----------------------------------------- */
+
+/* S[(el[i]*n_el + el[j])*nq + iq] = sum over ordered pairs (i, j) with
+ * 1e-6 < r_ij^2 < rmax2[i] of sin(q r_ij)/(q r_ij + 1e-8), r_ij^2 =
+ * (dx^2 + dy^2) + dz^2; npairs[el[i]*n_el + el[j]] counts them.  xyz is SoA
+ * (n atoms).  i and j ascend, so for element-major atoms each sum is
+ * accumulated in the order of RearrangementInt's Itemp
+ * (CalcFunctions.cpp:485-499). */
+void or_cutoff_pairsum(const double *xyz, size_t n, const unsigned *el, unsigned n_el,
+                       const double *rmax2, const double *q, unsigned nq, double *S,
+                       uint64_t *npairs);
+
+/* S[type[p]*nq + iq] = sum_p sin(q r_p)/(q r_p + 1e-8), p ascending: the
+ * neighbour terms of RearrangementInt (CalcFunctions.cpp:506-529). */
+void or_pairlist_pairsum(const double *r, const unsigned *type, size_t np, unsigned ntypes,
+                         const double *q, unsigned nq, double *S);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,9 @@ def lib():
                                     u32p, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
                                     f64p, f64p, C.c_int]
         L.or_philox4x64.argtypes = [u64p, u64p, u64p]
+        L.or_cutoff_pairsum.argtypes = [f64p, C.c_size_t, u32p, C.c_uint, f64p, f64p, C.c_uint,
+                                        f64p, u64p]
+        L.or_pairlist_pairsum.argtypes = [f64p, u32p, C.c_size_t, C.c_uint, f64p, C.c_uint, f64p]
         L.or_calc_atoms.restype = C.c_longlong
         L.or_calc_atoms.argtypes = [C.c_void_p, C.c_uint, C.c_uint64, C.c_longlong, u32p, u32p,
                                     f64p]
@@ -249,3 +252,20 @@ def calc_atoms(blocks, seed):
     if n < 0:
         raise RuntimeError("or_calc_atoms: more atoms than candidate sites")
     return el[:n], blk[:n], xyz[:3 * n].reshape(n, 3)
+
+
+def cutoff_pairsum(xyz64, el, n_el, rmax2, q):
+    """(S[n_el*n_el, nq], npairs[n_el*n_el]) of or_cutoff_pairsum."""
+    xyz64, q = _c64(xyz64), _c64(q)
+    S = np.zeros(n_el * n_el * q.size)
+    npairs = np.zeros(n_el * n_el, np.uint64)
+    lib().or_cutoff_pairsum(xyz64, xyz64.shape[-1], _cu32(el), n_el, _c64(rmax2), q, q.size, S,
+                            npairs)
+    return S.reshape(n_el * n_el, q.size), npairs
+
+
+def pairlist_pairsum(r, types, ntypes, q):
+    r, q = _c64(r), _c64(q)
+    S = np.zeros(ntypes * q.size)
+    lib().or_pairlist_pairsum(r, _cu32(types), r.size, ntypes, q, q.size, S)
+    return S.reshape(ntypes, q.size)

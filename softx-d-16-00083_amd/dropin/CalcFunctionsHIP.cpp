@@ -425,3 +425,90 @@ unsigned int CalcAndPrintAtoms(config *cfg, block *Block, vector<vect3d<double> 
   }
   return Ntot;
 }
+
+// ---------------------------------------------------------------------------
+// RearrangementInt (CalcFunctions.cpp:469-545), signature of main.cpp:44.
+// The cut-off pair sums and the neighbour sums run on the GPU
+// (xs_cutoff_pairsum, xs_pairlist_pairsum); the weights, the pair count N,
+// the partial-intensity slots and the polarisation factor are computed here
+// as the reference does (its Rmax2 and PolarFactor1D are called as they are).
+
+double Rmax2(vect3d<double> rAtom, vector<vect3d<double> > rAtomNeighb);
+void PolarFactor1D(double *I, const double *q, double lambda, unsigned int Nq);
+
+void RearrangementInt(double *I0, const double *q, const config *cfg, const block *Block,
+                      vector<double *> FF, vector<double> SL, unsigned int Ntot) {
+  const unsigned Nq = cfg->q.N, nel = cfg->Nel, nb = cfg->Nblocks;
+  const bool xr = cfg->source == xray, partial = cfg->calcPartialIntensity;
+  long long N = SQR(Ntot);  // formed in unsigned int, as the reference does
+  std::vector<double> I(partial ? (size_t)(nb + 1) * Nq : Nq, 0.0);
+  auto weight = [&](unsigned coeff, unsigned a, unsigned b, unsigned iq) {
+    return xr ? coeff * FF[a][iq] * FF[b][iq] : coeff * SL[a] * SL[b];
+  };
+  for (unsigned iB = 0; iB < nb; ++iB) {
+    const unsigned iqST = partial ? Nq : 0;  // one slot for every block (CalcFunctions.cpp:478-479)
+    const block &B = Block[iB];
+    if (!B.rearrangement) continue;
+    const unsigned coeff = B.Ncopy * B.Ncell.x * B.Ncell.y * B.Ncell.z * B.Nsymm;
+    const size_t n = B.Natom;
+    std::vector<unsigned> el;
+    for (unsigned t = 0; t < B.Nid; ++t) el.insert(el.end(), B.NatomEl[t], B.id[t]);
+    std::vector<double> xyz(3 * n), r2(n);
+    for (size_t i = 0; i < n; ++i) {
+      xyz[i] = B.rAtom[i].x;
+      xyz[n + i] = B.rAtom[i].y;
+      xyz[2 * n + i] = B.rAtom[i].z;
+      r2[i] = Rmax2(B.rAtom[i], B.rAtomNeighb[i]) + SQR(B.Rcorr);
+    }
+    const size_t nt = (size_t)nel * nel;
+    std::vector<double> S_cut(nt * Nq), S_nb(nt * Nq);
+    std::vector<uint64_t> npairs(nt);
+    check(xs_cutoff_pairsum(ctx(), xyz.data(), n, el.data(), nel, r2.data(), q, Nq, S_cut.data(),
+                            npairs.data(), 0),
+          "xs_cutoff_pairsum");
+    std::vector<double> rn;
+    std::vector<unsigned> tn;
+    std::vector<unsigned long long> per_type(nt, 0), per_el(nel, 0);
+    for (size_t i = 0; i < n; ++i)
+      for (size_t k = 0; k < B.rAtomNeighb[i].size(); ++k) {
+        rn.push_back((B.rAtom[i] - B.rAtomNeighb[i][k]).mag());
+        tn.push_back(el[i] * nel + B.idNeighb[i][k]);
+        ++per_type[tn.back()];
+        ++per_el[el[i]];
+      }
+    check(xs_pairlist_pairsum(ctx(), rn.data(), tn.data(), rn.size(), (unsigned)nt, q, Nq,
+                              S_nb.data(), 0),
+          "xs_pairlist_pairsum");
+    for (unsigned ia = 0; ia < B.Nid; ++ia) {
+      const unsigned id = B.id[ia];
+      for (unsigned ja = 0; ja < B.Nid; ++ja) {  // CalcFunctions.cpp:484-503
+        const unsigned jd = B.id[ja];
+        const size_t t = (size_t)id * nel + jd;
+        N -= (long long)coeff * (long long)npairs[t];
+        for (unsigned iq = 0; iq < Nq; ++iq)
+          I[iqST + iq] -= weight(coeff, id, jd, iq) * S_cut[t * Nq + iq];
+      }
+      for (unsigned b = 0; b < nel; ++b) {  // CalcFunctions.cpp:506-529
+        const size_t t = (size_t)id * nel + b;
+        if (!per_type[t]) continue;
+        for (unsigned iq = 0; iq < Nq; ++iq)
+          I[iqST + iq] += weight(coeff, id, b, iq) * S_nb[t * Nq + iq];
+      }
+      N += (long long)coeff * (long long)per_el[id];
+    }
+  }
+  if (partial) {  // CalcFunctions.cpp:532-541
+    unsigned iqST = Nq;
+    for (unsigned iB = 0; iB < nb; ++iB, iqST += Nq) {
+      const unsigned i0qST = Nq * (1 + iB * (2 * nb - iB + 1) / 2);
+      if (cfg->PolarFactor) PolarFactor1D(I.data() + iqST, q, cfg->lambda, Nq);
+      for (unsigned iq = 0; iq < Nq; ++iq) {
+        I[iq] += I[iqST + iq];
+        I0[i0qST + iq] = (I0[i0qST + iq] * Ntot + I[iqST + iq]) / sqrt(double(N));
+      }
+    }
+  } else if (cfg->PolarFactor) {
+    PolarFactor1D(I.data(), q, cfg->lambda, Nq);
+  }
+  for (unsigned iq = 0; iq < Nq; ++iq) I0[iq] = (I0[iq] * Ntot + I[iq]) / sqrt(double(N));
+}

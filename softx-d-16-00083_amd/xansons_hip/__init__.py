@@ -71,6 +71,8 @@ def lib():
         L.xs_batch_run.argtypes = [vp, vp, vp, u, vp, i, vp, vp, vp, u]
         L.xs_batch_overflow.argtypes = [vp, vp]
         L.xs_batch_free.argtypes = [vp]
+        L.xs_cutoff_pairsum.argtypes = [vp, vp, sz, vp, u, vp, vp, u, vp, vp, u]
+        L.xs_pairlist_pairsum.argtypes = [vp, vp, vp, sz, u, vp, u, vp, u]
         L.xs_atoms_build.argtypes = [vp, vp, u, u, C.c_uint64, C.POINTER(vp)]
         L.xs_atoms_info.argtypes = [vp, vp, vp, vp]
         L.xs_atoms_copy.argtypes = [vp, vp, u]
@@ -230,6 +232,26 @@ class Context:
         return S.reshape(ntypes(len(ec)), nparts, q.size)
 
     # ---------------------------------------------------------------- H7
+    def cutoff_pairsum(self, xyz, el, n_el: int, rmax2, q):
+        """RearrangementInt's cut-off sums: (S[n_el*n_el, nq], npairs[n_el*n_el])."""
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        ea, r2, qa = _u32(el), np.ascontiguousarray(rmax2, np.float64), \
+            np.ascontiguousarray(q, np.float64)
+        S = np.zeros(n_el * n_el * qa.size)
+        npairs = np.zeros(n_el * n_el, dtype=np.uint64)
+        _check(lib().xs_cutoff_pairsum(self.h, _p(x), x.shape[-1], _p(ea), int(n_el), _p(r2),
+                                       _p(qa), qa.size, _p(S), _p(npairs), 0))
+        return S.reshape(n_el * n_el, qa.size), npairs
+
+    def pairlist_pairsum(self, r, types, ntypes: int, q):
+        """Sine sums over a distance list: S[ntypes, nq]."""
+        ra, ta, qa = np.ascontiguousarray(r, np.float64), _u32(types), \
+            np.ascontiguousarray(q, np.float64)
+        S = np.zeros(ntypes * qa.size)
+        _check(lib().xs_pairlist_pairsum(self.h, _p(ra), _p(ta), ra.size, int(ntypes), _p(qa),
+                                         qa.size, _p(S), 0))
+        return S.reshape(ntypes, qa.size)
+
     def amp2d(self, xyz, el_count, qvec, rot, ff):
         ec = _u32(el_count)
         xyz = np.ascontiguousarray(xyz, dtype=np.float64)

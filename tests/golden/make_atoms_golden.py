@@ -52,8 +52,9 @@ def vec(v):
 
 
 def block_xml(atoms, cell=((1, 0, 0), (0, 1, 0), (0, 0, 1)), ncell=(1, 1, 1), symm=(),
-              copies=(), rcut=None, rcut_copies=None, attrs=None):
-    """atoms: (name, r, occ, Uiso); symm: (r, R1, R2, R3); copies: (r, euler)."""
+              copies=(), rcut=None, rcut_copies=None, attrs=None, extra=()):
+    """atoms: (name, r, occ, Uiso); symm: (r, R1, R2, R3); copies: (r, euler);
+    extra: further child elements (XML lines)."""
     a = " ".join(f'{k}="{v}"' for k, v in (attrs or {}).items())
     out = [f"  <Block {a}>" if a else "  <Block>", "    <Atoms>"]
     for name, r, occ, uiso in atoms:
@@ -73,6 +74,7 @@ def block_xml(atoms, cell=((1, 0, 0), (0, 1, 0), (0, 0, 1)), ncell=(1, 1, 1), sy
         for r, eu in copies:
             out.append(f'      <Copy r="{vec(r)}" Euler="{vec(eu)}"/>')
         out.append("    </Copies>")
+    out.extend("    " + e for e in extra)
     out.append(f'    <CellVectors a="{vec(cell[0])}" b="{vec(cell[1])}" c="{vec(cell[2])}" '
                f'N="{" ".join(str(int(n)) for n in ncell)}"/>')
     out.append("  </Block>")

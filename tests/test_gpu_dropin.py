@@ -136,3 +136,24 @@ def test_atoms_file_equals_reference(tmp, case):
     assert "Total calculation time" in out
     with open(os.path.join(tmp, f"{case}_atoms.xyz")) as f:
         assert f.read() == str(g["xyz_text"])
+
+
+@needs_exe
+@pytest.mark.parametrize("case", ["nacl", "xray_partial", "cluster"])
+def test_rearrangement_equals_reference(tmp, case):
+    """Debye + RearrangementInt (main.cpp:179): the drop-in's printed
+    intensity files against the reference executable's (7 digits)."""
+    import json
+    g = np.load(os.path.join(REPO, "tests", "golden", f"rearr_{case}.npz"))
+    files = json.loads(str(g["files_json"]))
+    for fn, text in files.items():
+        if fn != "start.xml":
+            with open(os.path.join(tmp, fn), "w") as f:
+                f.write(text)
+    run(tmp, files["start.xml"])
+    printed = json.loads(str(g["printed_json"]))
+    assert printed
+    for fn, text in printed.items():
+        want = np.loadtxt(text.splitlines(), ndmin=2)[:, 1]
+        got = col(tmp, fn)
+        assert rel_err(got, want)[0] < 2e-6, fn
