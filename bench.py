@@ -576,6 +576,104 @@ def atoms_leg(ctx, cpu=True, reps=3):
     return out
 
 
+def rearr_workload(m=32, nq=1024):
+    """RearrangementInt workload: one block given as a single cell of m^3
+    atoms on a simple cubic lattice (2.67 A) moved by up to 0.4 A (so no two
+    atoms merge), 3 elements element-major, each atom with its neighbours
+    within 2.7 A moved by up to 0.1 A, Rcorr 0.3, neutron, nq q on
+    0.5-10 A^-1.  Returns the RearrBlock, q, SL and the files for the
+    reference."""
+    from xansons_hip.rearrangement import RearrBlock
+    rng = np.random.default_rng(2027)
+    names = ["Na", "Cl", "Au"]
+    n = m ** 3
+    grid = np.stack(np.meshgrid(*[np.arange(m)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    r = grid * 2.67 + rng.uniform(-0.4, 0.4, (n, 3))
+    el = np.sort(rng.integers(0, 3, n))
+    # neighbours within 2.7 A through a cell list
+    cell = 2.7
+    key = np.floor((r - r.min(0)) / cell).astype(np.int64)
+    g = int(key.max()) + 1
+    lin = (key[:, 0] * g + key[:, 1]) * g + key[:, 2]
+    order = np.argsort(lin, kind="stable")
+    starts = np.searchsorted(lin[order], np.arange(g ** 3 + 1))
+    nbs = []
+    for i in range(n):
+        cand = []
+        kx, ky, kz = key[i]
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    x, y, z = kx + dx, ky + dy, kz + dz
+                    if 0 <= x < g and 0 <= y < g and 0 <= z < g:
+                        c = (x * g + y) * g + z
+                        cand.append(order[starts[c]:starts[c + 1]])
+        c = np.concatenate(cand)
+        d = np.linalg.norm(r[c] - r[i], axis=1)
+        j = c[(d < 2.7) & (c != i)]
+        nbs.append(np.column_stack([el[j], r[j] + rng.uniform(-0.1, 0.1, (len(j), 3))]))
+    q = np.array([0.5 + k * (10.0 - 0.5) / nq for k in range(nq)])
+    sl = np.array([3.63, 9.577, 7.63])
+    lines = [f"{names[e]} {float(x)!r} {float(y)!r} {float(z)!r}" for e, (x, y, z) in zip(el, r)]
+    nbl = []
+    for i, nb in enumerate(nbs):
+        nbl.append(f"{i} {len(nb)}")
+        nbl.extend(f"{names[int(e)]} {float(x)!r} {float(y)!r} {float(z)!r}" for e, x, y, z in nb)
+    xml = ('<?xml version="1.0"?>\n<Config>\n  <Calculation name="rr" source="neutron" '
+           'scenario="Debye" FFfilename="sl.txt">\n'
+           f'    <q min="0.5" max="10" N="{nq}"/>\n  </Calculation>\n  <Block>\n'
+           '    <Atoms filename="atoms.txt"/>\n'
+           '    <Rearrangement filename="nb.txt" Rcorr="0.3"/>\n  </Block>\n</Config>\n')
+    files = {"start.xml": xml, "sl.txt": "Na 3.63\nCl 9.577\nAu 7.63\n",
+             "atoms.txt": "\n".join(lines) + "\n", "nb.txt": "\n".join(nbl) + "\n"}
+    blk = RearrBlock(r_atom=r, el=el.astype(np.uint32), neighbours=np.concatenate(nbs), rcorr=0.3,
+                     coeff=1, nb_count=np.array([len(x) for x in nbs]))
+    return blk, q, sl, files
+
+
+def rearr_leg(ctx, cpu=True, reps=3):
+    """RearrangementInt with GPU pair sums vs the reference's own function
+    (oracle/_ref/ref_atoms --rearrangement, one thread) on the same input."""
+    from xansons_hip.rearrangement import rearrangement_intensity
+    blk, q, sl, files = rearr_workload()
+    n = blk.r_atom.shape[0]
+    I0 = 1.0 + np.arange(q.size) / 64.0
+    args = (I0, q, [blk], 3, n, "neutron")
+    rearrangement_intensity(*args, sl=sl, ctx=ctx)  # warm-up
+    ts = []
+    for _ in range(reps):
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        out = rearrangement_intensity(*args, sl=sl, ctx=ctx)
+        ts.append(time.perf_counter() - t0)
+        k = {name: round(ctx.profile_read(name)[0], 3) for name in ("cutoff_pairs", "pairlist_sums")}
+        ctx.profile(False)
+    t = float(np.median(ts))
+    nnb = int(blk.nb_count.sum())
+    rec = {"atoms": n, "nq": int(q.size), "neighbours": nnb, "reference_pair_tests": n * n,
+           "ms": round(t * 1e3, 3), "kernels_ms": k}
+    if cpu:
+        exe = os.path.join(REPO, "oracle", "_ref", "ref_atoms")
+        if os.path.exists(exe):
+            tmp = tempfile.mkdtemp(prefix="xs_rearr_ref_")
+            try:
+                for fn, text in files.items():
+                    with open(os.path.join(tmp, fn), "w") as f:
+                        f.write(text)
+                subprocess.run([exe, "start.xml", "out.json", "--no-atoms", "--rearrangement"],
+                               cwd=tmp, check=True, stdout=subprocess.DEVNULL, timeout=900)
+                with open(os.path.join(tmp, "out.json")) as f:
+                    d = json.load(f)
+            finally:
+                shutil.rmtree(tmp, ignore_errors=True)
+            ref = np.array([float.fromhex(v) for v in d["rearr_I"]])
+            rec["reference_s"] = round(d["rearr_seconds"], 3)
+            rec["reference_cores"] = 1
+            rec["speedup"] = round(d["rearr_seconds"] / t, 1)
+            rec["max_rel_err_vs_reference"] = float(np.abs(out - ref).max() / np.abs(ref).max())
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -591,7 +689,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration)")
     ap.add_argument("--no-atoms", action="store_true", help="skip the ensemble-builder leg")
-    ap.add_argument("--only", choices=["matrix", "2d", "ensemble", "debye", "atoms"],
+    ap.add_argument("--no-rearr", action="store_true", help="skip the RearrangementInt leg")
+    ap.add_argument("--only", choices=["matrix", "2d", "ensemble", "debye", "atoms", "rearr"],
                     help="run one extra leg alone and print its object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -610,6 +709,7 @@ def main():
                "2d": lambda: amp2d_leg(ctx),
                "debye": lambda: debye_leg(ctx),
                "atoms": lambda: atoms_leg(ctx, cpu=not args.no_cpu_baseline),
+               "rearr": lambda: rearr_leg(ctx, cpu=not args.no_cpu_baseline),
                "ensemble": lambda: ensemble_leg(ctx, args.ensemble, max(1, args.steps // 2), 1)}
         print(json.dumps({args.only: leg[args.only]()}))
         ctx.close()
@@ -686,6 +786,8 @@ def main():
         line["ensemble"] = ensemble_leg(ctx, args.ensemble, max(1, args.steps // 2), 1)
     if not args.no_atoms:
         line["atoms_builder"] = atoms_leg(ctx, cpu=not args.no_cpu_baseline)
+    if not args.no_rearr:
+        line["rearrangement"] = rearr_leg(ctx, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(ens, nh)
     ctx.close()

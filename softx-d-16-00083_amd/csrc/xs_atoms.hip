@@ -40,23 +40,16 @@
 namespace xs {
 namespace atoms {
 
-constexpr double kMergeR2 = 0.25;   // squared merge distance, CalcFunctions.cpp:301
-constexpr double kMinCell = 0.505;  // grid cell width: merge distance + 1 %
+constexpr double kMergeR2 = 0.25;  // squared merge distance, CalcFunctions.cpp:301
+constexpr double kMergeR = 0.5;
 constexpr double kPi = 3.14159265358979323846;
 constexpr int kThreads = 256;
-constexpr int kBboxBlocks = 512;
 
 // One entry of a covering image's list: the cell atom and symmetry image
 // placed if the occupancy draw is <= occ.
 struct Entry {
   double occ;
   uint32_t atom, symm, el, pad;
-};
-
-struct Grid {
-  double lo[3];
-  double inv;
-  int g[3];
 };
 
 // ---- random draws ----------------------------------------------------------
@@ -109,52 +102,6 @@ __global__ void site_positions(const double *r_atom, const double *p_symm, const
   sz[i] = (P[6] * rx + P[7] * ry + P[8] * rz) + r_symm[3 * s + 2];
 }
 
-__global__ __launch_bounds__(kThreads) void site_bbox(const double *sx, const double *sy,
-                                                      const double *sz, size_t M, double *part) {
-  __shared__ double red[6][kThreads];
-  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (size_t i = blockIdx.x * (size_t)kThreads + threadIdx.x; i < M;
-       i += (size_t)gridDim.x * kThreads) {
-    const double v[3] = {sx[i], sy[i], sz[i]};
-    for (int k = 0; k < 3; ++k) {
-      lo[k] = fmin(lo[k], v[k]);
-      hi[k] = fmax(hi[k], v[k]);
-    }
-  }
-  for (int k = 0; k < 3; ++k) {
-    red[k][threadIdx.x] = lo[k];
-    red[3 + k][threadIdx.x] = hi[k];
-  }
-  __syncthreads();
-  for (int w = kThreads / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int k = 0; k < 3; ++k) {
-        red[k][threadIdx.x] = fmin(red[k][threadIdx.x], red[k][threadIdx.x + w]);
-        red[3 + k][threadIdx.x] = fmax(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + w]);
-      }
-    __syncthreads();
-  }
-  if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
-}
-
-// Grid coordinate, clamped to [-2, g+1] (also for NaN) so that it fits an int.
-__device__ inline int grid_coord(double v, double lo, double inv, int g) {
-  double f = floor((v - lo) * inv);
-  f = fmin(fmax(f, -2.0), (double)g + 1.0);
-  return (int)f;
-}
-
-__global__ void site_keys(const double *sx, const double *sy, const double *sz, size_t M, Grid G,
-                          uint32_t *key, uint32_t *idx) {
-  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i >= M) return;
-  const int cx = min(max(grid_coord(sx[i], G.lo[0], G.inv, G.g[0]), 0), G.g[0] - 1);
-  const int cy = min(max(grid_coord(sy[i], G.lo[1], G.inv, G.g[1]), 0), G.g[1] - 1);
-  const int cz = min(max(grid_coord(sz[i], G.lo[2], G.inv, G.g[2]), 0), G.g[2] - 1);
-  key[i] = ((uint32_t)cz * (uint32_t)G.g[1] + (uint32_t)cy) * (uint32_t)G.g[0] + (uint32_t)cx;
-  idx[i] = (uint32_t)i;
-}
-
 __global__ void fill_u32(uint32_t *p, size_t n, uint32_t v) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -174,19 +121,9 @@ __global__ void key_ranges(const uint32_t *skey, size_t n, uint32_t *beg, uint32
   if (p + 1 == n || skey[p + 1] != k) end[k] = (uint32_t)(p + 1);
 }
 
-__global__ void gather_sites(const uint32_t *order, size_t M, const double *sx, const double *sy,
-                             const double *sz, double *SX, double *SY, double *SZ) {
-  const size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (p >= M) return;
-  const uint32_t i = order[p];
-  SX[p] = sx[i];
-  SY[p] = sy[i];
-  SZ[p] = sz[i];
-}
-
 struct MergeArgs {
   const double *sx, *sy, *sz;  // by image index
-  const double *SX, *SY, *SZ;  // by sorted position
+  const double *SX, *SY, *SZ;  // by sorted position (grid order)
   const uint32_t *order, *beg, *end;
   const double *trans;
   unsigned ntrans;
@@ -211,8 +148,7 @@ __global__ __launch_bounds__(kThreads) void merge_images(MergeArgs A) {
   for (int z = max(cz - 1, 0); z <= min(cz + 1, A.G.g[2] - 1); ++z)
     for (int y = max(cy - 1, 0); y <= min(cy + 1, A.G.g[1] - 1); ++y)
       for (int x = max(cx - 1, 0); x <= min(cx + 1, A.G.g[0] - 1); ++x) {
-        const uint32_t k = ((uint32_t)z * (uint32_t)A.G.g[1] + (uint32_t)y) * (uint32_t)A.G.g[0] +
-                           (uint32_t)x;
+        const uint32_t k = grid_cell(A.G, x, y, z);
         const uint32_t p1 = A.end[k];
         for (uint32_t p = A.beg[k]; p < p1; ++p) {
           const uint32_t j = A.order[p];
@@ -360,30 +296,6 @@ __global__ void gather_atoms(const uint32_t *order, size_t n, const double *px, 
 
 inline unsigned nblk(size_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
-// Device allocation owned for the duration of one build.
-struct DevMem {
-  void *p = nullptr;
-  DevMem() = default;
-  DevMem(const DevMem &) = delete;
-  DevMem &operator=(const DevMem &) = delete;
-  ~DevMem() {
-    if (p) (void)hipFree(p);
-  }
-  int alloc(size_t bytes, const char *what) {
-    if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) {
-      p = nullptr;
-      (void)hipGetLastError();
-      return fail(XS_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes (" +
-                                    what + ") failed");
-    }
-    return XS_OK;
-  }
-  template <class T>
-  T *at(size_t byte_off = 0) const {
-    return reinterpret_cast<T *>(static_cast<char *>(p) + byte_off);
-  }
-};
-
 // Host staging of several arrays for one upload.
 struct Pack {
   std::vector<char> host;
@@ -395,39 +307,6 @@ struct Pack {
     return off;
   }
 };
-
-int upload(xs_ctx *ctx, void *dst, const void *src, size_t bytes) {
-  if (!bytes) return XS_OK;
-  XS_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-  XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  return XS_OK;
-}
-
-int download(xs_ctx *ctx, void *dst, const void *src, size_t bytes) {
-  if (!bytes) return XS_OK;
-  XS_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-  XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  return XS_OK;
-}
-
-int sort_pairs(xs_ctx *ctx, const uint32_t *k_in, uint32_t *k_out, const uint32_t *v_in,
-               uint32_t *v_out, size_t n, int bits) {
-  size_t bytes = 0;
-  XS_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k_in, k_out, v_in, v_out, (int)n,
-                                                  0, bits, ctx->stream));
-  DevMem tmp;
-  XS_TRY(tmp.alloc(bytes, "radix sort"));
-  XS_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, k_in, k_out, v_in, v_out, (int)n,
-                                                  0, bits, ctx->stream));
-  XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  return XS_OK;
-}
-
-int bits_for(uint64_t v) {
-  int b = 1;
-  while (b < 32 && (v >> b)) ++b;
-  return b;
-}
 
 // Covering images of one block and their entry lists (CSR).
 struct Covering {
@@ -467,68 +346,23 @@ int merge_triples(xs_ctx *ctx, const xs_block_desc &B, const std::vector<double>
   const size_t o_ps = in.add(B.p_symm, 9 * (size_t)B.nsymm);
   const size_t o_rs = in.add(B.r_symm, 3 * (size_t)B.nsymm);
   const size_t o_tr = in.add(trans.data(), trans.size());
-  DevMem din, pos, sorted, keys, bbox;
+  DevMem din, pos;
   XS_TRY(din.alloc(in.host.size(), "merge inputs"));
   XS_TRY(upload(ctx, din.p, in.host.data(), in.host.size()));
-  XS_TRY(pos.alloc(6 * M * sizeof(double), "image positions"));
+  XS_TRY(pos.alloc(3 * M * sizeof(double), "image positions"));
   double *sx = pos.at<double>(), *sy = sx + M, *sz = sy + M;
-  double *SX = sz + M, *SY = SX + M, *SZ = SY + M;
   site_positions<<<nblk(M), kThreads, 0, ctx->stream>>>(din.at<double>(o_ra), din.at<double>(o_ps),
                                                        din.at<double>(o_rs), B.natom, B.nsymm, sx,
                                                        sy, sz);
   XS_HIP_CHECK(hipGetLastError());
-  // bounding box of the images
-  const unsigned nbb = std::min<unsigned>(kBboxBlocks, nblk(M));
-  XS_TRY(bbox.alloc(6 * nbb * sizeof(double), "bbox"));
-  site_bbox<<<nbb, kThreads, 0, ctx->stream>>>(sx, sy, sz, M, bbox.at<double>());
-  XS_HIP_CHECK(hipGetLastError());
-  std::vector<double> part(6 * nbb);
-  XS_TRY(download(ctx, part.data(), bbox.p, part.size() * sizeof(double)));
-  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (unsigned b = 0; b < nbb; ++b)
-    for (int k = 0; k < 3; ++k) {
-      lo[k] = std::min(lo[k], part[6 * b + k]);
-      hi[k] = std::max(hi[k], part[6 * b + 3 + k]);
-    }
-  double ext[3];
-  for (int k = 0; k < 3; ++k) {
-    ext[k] = hi[k] - lo[k];
-    if (!std::isfinite(ext[k]))
-      return fail(XS_ERR_RANGE, "xs_atoms_build: image positions are not finite");
-  }
-  // grid of >= 0.505 A cells, at most max(2^16, 8 M) (and 2^24) of them
-  const double cap_cells = (double)std::min<size_t>(size_t(1) << 24,
-                                                    std::max<size_t>(size_t(1) << 16, 8 * M));
-  double h = kMinCell;
-  auto ncells = [&](double inv) {
-    double n = 1;
-    for (int k = 0; k < 3; ++k) n *= std::floor(ext[k] * inv) + 1;
-    return n;
-  };
-  while (ncells(1.0 / h) > cap_cells) h *= 1.25;
-  Grid G;
-  G.inv = 1.0 / h;
-  for (int k = 0; k < 3; ++k) {
-    G.lo[k] = lo[k];
-    G.g[k] = (int)std::floor(ext[k] * G.inv) + 1;
-  }
-  const size_t nc = (size_t)G.g[0] * G.g[1] * G.g[2];
-  XS_TRY(keys.alloc((4 * M + 2 * nc) * sizeof(uint32_t), "grid keys"));
-  uint32_t *key = keys.at<uint32_t>(), *idx = key + M, *skey = idx + M, *order = skey + M;
-  uint32_t *beg = order + M, *end = beg + nc;
-  site_keys<<<nblk(M), kThreads, 0, ctx->stream>>>(sx, sy, sz, M, G, key, idx);
-  XS_HIP_CHECK(hipGetLastError());
-  XS_TRY(sort_pairs(ctx, key, skey, idx, order, M, bits_for(nc)));
-  fill_u32<<<nblk(2 * nc), kThreads, 0, ctx->stream>>>(beg, 2 * nc, 0u);
-  XS_HIP_CHECK(hipGetLastError());
-  key_ranges<<<nblk(M), kThreads, 0, ctx->stream>>>(skey, M, beg, end);
-  gather_sites<<<nblk(M), kThreads, 0, ctx->stream>>>(order, M, sx, sy, sz, SX, SY, SZ);
-  XS_HIP_CHECK(hipGetLastError());
+  // images sorted into a grid of >= 0.5 A cells
+  SortedPoints sp;
+  XS_TRY(grid_sort(ctx, sx, sy, sz, M, kMergeR, sp));
   // count pass, then write pass
   DevMem cnt, out;
   XS_TRY(cnt.alloc(sizeof(unsigned long long), "merge count"));
-  MergeArgs A{sx, sy, sz, SX, SY, SZ, order, beg, end, din.at<double>(o_tr), nT,
-              (unsigned)M, G, cnt.at<unsigned long long>(), nullptr, 0};
+  MergeArgs A{sx, sy, sz, sp.X, sp.Y, sp.Z, sp.order, sp.beg, sp.end, din.at<double>(o_tr), nT,
+              (unsigned)M, sp.G, cnt.at<unsigned long long>(), nullptr, 0};
   XS_HIP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), ctx->stream));
   const size_t nthr = M * nT;
   merge_images<<<nblk(nthr), kThreads, 0, ctx->stream>>>(A);
@@ -744,7 +578,7 @@ extern "C" int xs_atoms_build(xs_ctx *ctx, const xs_block_desc *blocks, unsigned
   XS_TRY(prof_begin(ctx, "atoms_order", &ev));
   iota_u32<<<nblk(total), kThreads, 0, ctx->stream>>>(val, total);
   XS_HIP_CHECK(hipGetLastError());
-  XS_TRY(sort_pairs(ctx, key, skey, val, sval, total, bits_for(key_none)));
+  XS_TRY(sort_pairs_u32(ctx, key, skey, val, sval, total, bits_for(key_none)));
   XS_TRY(ranges.alloc(2 * nkeys * sizeof(uint32_t), "key ranges"));
   uint32_t *kb = ranges.at<uint32_t>(), *ke = kb + nkeys;
   fill_u32<<<nblk(2 * nkeys), kThreads, 0, ctx->stream>>>(kb, 2 * nkeys, 0u);

@@ -149,6 +149,64 @@ int batch_hist_launch(xs_batch *b, unsigned long long *hist_d, bool prof);
 // Free a built ensemble's device memory and unregister it from its context.
 void destroy_atoms(xs_atoms *a);
 
+// Device allocation owned by one call (freed on scope exit).
+struct DevMem {
+  void *p = nullptr;
+  DevMem() = default;
+  DevMem(const DevMem &) = delete;
+  DevMem &operator=(const DevMem &) = delete;
+  ~DevMem() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t bytes, const char *what);
+  template <class T>
+  T *at(size_t byte_off = 0) const {
+    return reinterpret_cast<T *>(static_cast<char *>(p) + byte_off);
+  }
+};
+
+// Synchronous copies on the context stream.
+int upload(xs_ctx *ctx, void *dst, const void *src, size_t bytes);
+int download(xs_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+// Stable radix sort of (key, value) uint32 pairs on the low `bits` key bits
+// (hipcub), synchronous on the context stream.
+int sort_pairs_u32(xs_ctx *ctx, const uint32_t *k_in, uint32_t *k_out, const uint32_t *v_in,
+                   uint32_t *v_out, size_t n, int bits);
+int bits_for(uint64_t v);
+
+// Uniform grid of cells over n points (device SoA x, y, z) for searches of
+// pairs closer than a cut-off: cell edge >= min_cell (grown by 1.25x until
+// the grid has at most max(2^16, 8 n) and 2^24 cells), points sorted by cell.
+// Points i, j closer than min_cell / 1.01 lie in the same or adjacent cells,
+// computed with grid_coord from either point's coordinates.
+struct Grid {
+  double lo[3];
+  double inv;
+  int g[3];
+};
+
+__device__ inline int grid_coord(double v, double lo, double inv, int g) {
+  double f = floor((v - lo) * inv);
+  f = fmin(fmax(f, -2.0), (double)g + 1.0);  // also maps NaN into range
+  return (int)f;
+}
+
+__device__ inline uint32_t grid_cell(const Grid &G, int x, int y, int z) {
+  return ((uint32_t)z * (uint32_t)G.g[1] + (uint32_t)y) * (uint32_t)G.g[0] + (uint32_t)x;
+}
+
+struct SortedPoints {
+  Grid G;
+  size_t n = 0, ncells = 0;
+  uint32_t *order = nullptr;              // order[p]: point at sorted position p
+  uint32_t *beg = nullptr, *end = nullptr;  // [beg[c], end[c]): positions of cell c
+  double *X = nullptr, *Y = nullptr, *Z = nullptr;  // coordinates by sorted position
+  DevMem mem;
+};
+int grid_sort(xs_ctx *ctx, const double *x, const double *y, const double *z, size_t n,
+              double min_cell, SortedPoints &out);
+
 // Pair sums of many histograms: one chunk = up to 2048 bins of one pair type.
 struct PairsumChunk {
   unsigned long long hist_off;  // first bin of the chunk in the histogram array

@@ -4,13 +4,14 @@
 // 1e-6 < r_ij^2 < rmax2[i] (rmax2 = the largest squared neighbour distance
 // of atom i plus Rcorr^2) contributes sin(q r_ij)/(q r_ij + 1e-8) to its
 // element pair (CalcFunctions.cpp:485-499).  The reference tests all N^2
-// pairs on one core and evaluates Nq sines per accepted pair.  Here a tiled
-// kernel tests the pairs (FP64, the reference's expression, no
-// contraction), a count pass sizes one segment of distances per element
-// pair and a write pass fills them; a second kernel sums the sines of each
-// segment for all q (one thread per q, distances staged in LDS, FP64
-// atomics across chunks).  The neighbour terms (506-529) are a given list
-// of distances and use the second kernel alone.
+// pairs on one core and evaluates Nq sines per accepted pair.  Here the
+// atoms are sorted into a grid of cells no narrower than the largest
+// cut-off and each atom tests the atoms of the 27 cells around its own
+// (FP64, the reference's expression, no contraction); a count pass sizes one
+// segment of distances per element pair and a write pass fills them.  A
+// second kernel sums the sines of each segment for all q (one thread per q,
+// distances staged in LDS, FP64 atomics across chunks).  The neighbour terms
+// (506-529) are a given list of distances and use the second kernel alone.
 #include "xs_internal.h"
 
 #include <algorithm>
@@ -26,57 +27,64 @@ namespace rearr {
 constexpr int kThreads = 256;
 constexpr unsigned kChunk = 1024;  // distances per sum work group
 
-// Count (WRITE = false) or write (WRITE = true) the accepted pairs.
-// count[t] (t = el_i * n_el + el_j): pairs per element pair; in the write
-// pass slot[t] hands out positions inside segment t (seg_off[t] ...).
+struct CutoffArgs {
+  const double *x, *y, *z;           // by atom
+  const double *X, *Y, *Z;           // by grid position
+  const uint32_t *order, *beg, *end;  // grid order and cell ranges
+  const uint32_t *el;
+  const double *rmax2;
+  Grid G;
+  uint32_t n, n_el;
+  unsigned long long *count, *slot;
+  const unsigned long long *seg_off;
+  double *r_out;
+};
+
+// Count (WRITE = false) or write (WRITE = true) the accepted pairs of atom
+// i = thread: every atom j in the 27 grid cells around i's cell (cell edge
+// >= sqrt(max rmax2), so no accepted pair is missed).  count[t]
+// (t = el_i * n_el + el_j): pairs per element pair; in the write pass slot[t]
+// hands out positions inside segment t (seg_off[t] ...).
 template <bool WRITE>
-__global__ __launch_bounds__(kThreads) void cutoff_pairs(
-    const double *x, const double *y, const double *z, const uint32_t *el, const double *rmax2,
-    uint32_t n, uint32_t n_el, unsigned long long *count, unsigned long long *slot,
-    const unsigned long long *seg_off, double *r_out) {
-  extern __shared__ unsigned char smem[];
-  double *sx = reinterpret_cast<double *>(smem), *sy = sx + kThreads, *sz = sy + kThreads;
-  uint32_t *se = reinterpret_cast<uint32_t *>(sz + kThreads);
-  unsigned *lcount = se + kThreads;  // [n_el * n_el], count pass only
-  const uint32_t ntypes = n_el * n_el;
-  if (!WRITE)
+__global__ __launch_bounds__(kThreads) void cutoff_pairs(CutoffArgs A) {
+  extern __shared__ unsigned lcount[];  // [n_el * n_el], count pass only
+  const uint32_t ntypes = A.n_el * A.n_el;
+  if (!WRITE) {
     for (uint32_t t = threadIdx.x; t < ntypes; t += kThreads) lcount[t] = 0;
+    __syncthreads();
+  }
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  const bool vi = i < n;
-  const double xi = vi ? x[i] : 0.0, yi = vi ? y[i] : 0.0, zi = vi ? z[i] : 0.0;
-  const double ri2 = vi ? rmax2[i] : -1.0;
-  const uint32_t ei = vi ? el[i] * n_el : 0;
-  for (uint32_t j0 = 0; j0 < n; j0 += kThreads) {
-    __syncthreads();
-    const uint32_t jl = j0 + threadIdx.x;
-    if (jl < n) {
-      sx[threadIdx.x] = x[jl];
-      sy[threadIdx.x] = y[jl];
-      sz[threadIdx.x] = z[jl];
-      se[threadIdx.x] = el[jl];
-    }
-    __syncthreads();
-    const uint32_t m = min((uint32_t)kThreads, n - j0);
-    if (!vi) continue;
-    for (uint32_t jj = 0; jj < m; ++jj) {
-      // (rAtom[i] - rAtom[j]).sqr(), CalcFunctions.cpp:491
-      const double dx = xi - sx[jj], dy = yi - sy[jj], dz = zi - sz[jj];
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 < ri2 && r2 > 0.000001) {
-        const uint32_t t = ei + se[jj];
-        if (WRITE) {
-          const unsigned long long s = atomicAdd(&slot[t], 1ull);
-          r_out[seg_off[t] + s] = sqrt(r2);
-        } else {
-          atomicAdd(&lcount[t], 1u);
+  if (i < A.n) {
+    const double xi = A.x[i], yi = A.y[i], zi = A.z[i], ri2 = A.rmax2[i];
+    const uint32_t ei = A.el[i] * A.n_el;
+    const int cx = grid_coord(xi, A.G.lo[0], A.G.inv, A.G.g[0]);
+    const int cy = grid_coord(yi, A.G.lo[1], A.G.inv, A.G.g[1]);
+    const int cz = grid_coord(zi, A.G.lo[2], A.G.inv, A.G.g[2]);
+    for (int gz = max(cz - 1, 0); gz <= min(cz + 1, A.G.g[2] - 1); ++gz)
+      for (int gy = max(cy - 1, 0); gy <= min(cy + 1, A.G.g[1] - 1); ++gy)
+        for (int gx = max(cx - 1, 0); gx <= min(cx + 1, A.G.g[0] - 1); ++gx) {
+          const uint32_t k = grid_cell(A.G, gx, gy, gz);
+          const uint32_t p1 = A.end[k];
+          for (uint32_t p = A.beg[k]; p < p1; ++p) {
+            // (rAtom[i] - rAtom[j]).sqr(), CalcFunctions.cpp:491
+            const double dx = xi - A.X[p], dy = yi - A.Y[p], dz = zi - A.Z[p];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < ri2 && r2 > 0.000001) {
+              const uint32_t t = ei + A.el[A.order[p]];
+              if (WRITE) {
+                const unsigned long long s = atomicAdd(&A.slot[t], 1ull);
+                A.r_out[A.seg_off[t] + s] = sqrt(r2);
+              } else {
+                atomicAdd(&lcount[t], 1u);
+              }
+            }
+          }
         }
-      }
-    }
   }
   if (!WRITE) {
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < ntypes; t += kThreads)
-      if (lcount[t]) atomicAdd(&count[t], (unsigned long long)lcount[t]);
+      if (lcount[t]) atomicAdd(&A.count[t], (unsigned long long)lcount[t]);
   }
 }
 
@@ -121,49 +129,29 @@ std::vector<Chunk> make_chunks(const std::vector<unsigned long long> &seg_off,
   return ch;
 }
 
-// Device buffer released on scope exit.
-struct Buf {
-  void *p = nullptr;
-  ~Buf() {
-    if (p) (void)hipFree(p);
-  }
-  int alloc(size_t bytes) {
-    if (hipMalloc(&p, std::max<size_t>(bytes, 64)) != hipSuccess) {
-      p = nullptr;
-      (void)hipGetLastError();
-      return fail(XS_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
-    }
-    return XS_OK;
-  }
-  template <class T>
-  T *as(size_t byte_off = 0) const {
-    return reinterpret_cast<T *>(static_cast<char *>(p) + byte_off);
-  }
-};
-
 // Sum the sines of distances r_d (device, grouped in segments) into S (host).
 int segment_sums(xs_ctx *ctx, const double *r_d, const std::vector<unsigned long long> &seg_off,
                  const std::vector<unsigned long long> &seg_len, const double *q, unsigned nq,
                  double *S_host, const char *prof_name) {
   const size_t ntypes = seg_len.size();
   const std::vector<Chunk> ch = make_chunks(seg_off, seg_len);
-  Buf b;
+  DevMem b;
   const size_t o_q = 0, o_S = align_up(nq * sizeof(double), 256),
                o_c = o_S + align_up(ntypes * nq * sizeof(double), 256);
-  XS_TRY(b.alloc(o_c + ch.size() * sizeof(Chunk)));
-  XS_HIP_CHECK(hipMemcpyAsync(b.as<double>(o_q), q, nq * sizeof(double), hipMemcpyHostToDevice,
+  XS_TRY(b.alloc(o_c + ch.size() * sizeof(Chunk), "pair sums"));
+  XS_HIP_CHECK(hipMemcpyAsync(b.at<double>(o_q), q, nq * sizeof(double), hipMemcpyHostToDevice,
                               ctx->stream));
   if (!ch.empty())
-    XS_HIP_CHECK(hipMemcpyAsync(b.as<Chunk>(o_c), ch.data(), ch.size() * sizeof(Chunk),
+    XS_HIP_CHECK(hipMemcpyAsync(b.at<Chunk>(o_c), ch.data(), ch.size() * sizeof(Chunk),
                                 hipMemcpyHostToDevice, ctx->stream));
-  double *S = b.as<double>(o_S);
+  double *S = b.at<double>(o_S);
   zero_f64<<<nblk(ntypes * nq), kThreads, 0, ctx->stream>>>(S, ntypes * nq);
   XS_HIP_CHECK(hipGetLastError());
   hipEvent_t ev;
   XS_TRY(prof_begin(ctx, prof_name, &ev));
   if (!ch.empty()) {
     sinc_sums<<<dim3((unsigned)ch.size(), nblk(nq)), kThreads, 0, ctx->stream>>>(
-        r_d, b.as<Chunk>(o_c), b.as<double>(o_q), nq, S);
+        r_d, b.at<Chunk>(o_c), b.at<double>(o_q), nq, S);
     XS_HIP_CHECK(hipGetLastError());
   }
   XS_TRY(prof_end(ctx, prof_name, ev));
@@ -195,33 +183,34 @@ extern "C" int xs_cutoff_pairsum(xs_ctx *ctx, const double *xyz, size_t n_atoms,
   std::fill(S, S + ntypes * nq, 0.0);
   std::fill(npairs, npairs + ntypes, 0ull);
   if (n < 2) return XS_OK;
-  Buf in;
+  const double maxr2 = *std::max_element(rmax2, rmax2 + n);
+  if (!(maxr2 > 0.000001)) return XS_OK;  // no pair can pass 1e-6 < r^2 < rmax2
+  if (!std::isfinite(maxr2)) return fail(XS_ERR_ARG, "xs_cutoff_pairsum: rmax2 not finite");
+  DevMem in;
   const size_t o_x = 0, o_r = align_up(3 * n * sizeof(double), 256),
                o_e = o_r + align_up(n * sizeof(double), 256),
                o_c = o_e + align_up(n * sizeof(uint32_t), 256);
-  XS_TRY(in.alloc(o_c + 3 * ntypes * sizeof(unsigned long long)));
-  XS_HIP_CHECK(hipMemcpyAsync(in.as<double>(o_x), xyz, 3 * n * sizeof(double),
+  XS_TRY(in.alloc(o_c + 3 * ntypes * sizeof(unsigned long long), "cut-off inputs"));
+  XS_HIP_CHECK(hipMemcpyAsync(in.at<double>(o_x), xyz, 3 * n * sizeof(double),
                               hipMemcpyHostToDevice, ctx->stream));
-  XS_HIP_CHECK(hipMemcpyAsync(in.as<double>(o_r), rmax2, n * sizeof(double), hipMemcpyHostToDevice,
+  XS_HIP_CHECK(hipMemcpyAsync(in.at<double>(o_r), rmax2, n * sizeof(double), hipMemcpyHostToDevice,
                               ctx->stream));
-  XS_HIP_CHECK(hipMemcpyAsync(in.as<uint32_t>(o_e), el, n * sizeof(uint32_t),
+  XS_HIP_CHECK(hipMemcpyAsync(in.at<uint32_t>(o_e), el, n * sizeof(uint32_t),
                               hipMemcpyHostToDevice, ctx->stream));
-  unsigned long long *count = in.as<unsigned long long>(o_c), *slot = count + ntypes,
+  unsigned long long *count = in.at<unsigned long long>(o_c), *slot = count + ntypes,
                      *seg = slot + ntypes;
   XS_HIP_CHECK(hipMemsetAsync(count, 0, 2 * ntypes * sizeof(unsigned long long), ctx->stream));
-  const double *x = in.as<double>(o_x), *y = x + n, *z = y + n;
-  const size_t lds = 3 * kThreads * sizeof(double) + kThreads * sizeof(uint32_t) +
-                     ntypes * sizeof(unsigned);
+  const double *x = in.at<double>(o_x), *y = x + n, *z = y + n;
   hipEvent_t ev;
   XS_TRY(prof_begin(ctx, "cutoff_pairs", &ev));
-  cutoff_pairs<false><<<nblk(n), kThreads, lds, ctx->stream>>>(
-      x, y, z, in.as<uint32_t>(o_e), in.as<double>(o_r), (uint32_t)n, n_el, count, nullptr,
-      nullptr, nullptr);
+  SortedPoints sp;
+  XS_TRY(grid_sort(ctx, x, y, z, n, std::sqrt(maxr2), sp));
+  CutoffArgs A{x, y, z, sp.X, sp.Y, sp.Z, sp.order, sp.beg, sp.end, in.at<uint32_t>(o_e),
+               in.at<double>(o_r), sp.G, (uint32_t)n, n_el, count, slot, seg, nullptr};
+  cutoff_pairs<false><<<nblk(n), kThreads, ntypes * sizeof(unsigned), ctx->stream>>>(A);
   XS_HIP_CHECK(hipGetLastError());
   std::vector<unsigned long long> len(ntypes), off(ntypes);
-  XS_HIP_CHECK(hipMemcpyAsync(len.data(), count, ntypes * sizeof(unsigned long long),
-                              hipMemcpyDeviceToHost, ctx->stream));
-  XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  XS_TRY(download(ctx, len.data(), count, ntypes * sizeof(unsigned long long)));
   unsigned long long total = 0;
   for (size_t t = 0; t < ntypes; ++t) {
     off[t] = total;
@@ -229,14 +218,13 @@ extern "C" int xs_cutoff_pairsum(xs_ctx *ctx, const double *xyz, size_t n_atoms,
   }
   if (total > (1ull << 31))
     return fail(XS_ERR_RANGE, "xs_cutoff_pairsum: more than 2^31 pairs within the cut-offs");
-  Buf rd;
-  XS_TRY(rd.alloc(total * sizeof(double)));
+  DevMem rd;
+  XS_TRY(rd.alloc(total * sizeof(double), "pair distances"));
   if (total) {
     XS_HIP_CHECK(hipMemcpyAsync(seg, off.data(), ntypes * sizeof(unsigned long long),
                                 hipMemcpyHostToDevice, ctx->stream));
-    cutoff_pairs<true><<<nblk(n), kThreads, lds, ctx->stream>>>(
-        x, y, z, in.as<uint32_t>(o_e), in.as<double>(o_r), (uint32_t)n, n_el, nullptr, slot, seg,
-        rd.as<double>());
+    A.r_out = rd.at<double>();
+    cutoff_pairs<true><<<nblk(n), kThreads, 0, ctx->stream>>>(A);
     XS_HIP_CHECK(hipGetLastError());
   }
   XS_TRY(prof_end(ctx, "cutoff_pairs", ev));
@@ -245,7 +233,7 @@ extern "C" int xs_cutoff_pairsum(xs_ctx *ctx, const double *xyz, size_t n_atoms,
     XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return XS_OK;
   }
-  return segment_sums(ctx, rd.as<double>(), off, len, q, nq, S, "pairlist_sums");
+  return segment_sums(ctx, rd.at<double>(), off, len, q, nq, S, "pairlist_sums");
 }
 
 extern "C" int xs_pairlist_pairsum(xs_ctx *ctx, const double *r, const unsigned *type,
@@ -271,9 +259,9 @@ extern "C" int xs_pairlist_pairsum(xs_ctx *ctx, const double *r, const unsigned 
   std::vector<double> sorted(npairs);
   std::vector<unsigned long long> pos(off);
   for (size_t p = 0; p < npairs; ++p) sorted[pos[type[p]]++] = r[p];
-  Buf rd;
-  XS_TRY(rd.alloc(npairs * sizeof(double)));
+  DevMem rd;
+  XS_TRY(rd.alloc(npairs * sizeof(double), "pair distances"));
   XS_HIP_CHECK(hipMemcpyAsync(rd.p, sorted.data(), npairs * sizeof(double), hipMemcpyHostToDevice,
                               ctx->stream));
-  return segment_sums(ctx, rd.as<double>(), off, len, q, nq, S, "pairlist_sums");
+  return segment_sums(ctx, rd.at<double>(), off, len, q, nq, S, "pairlist_sums");
 }

@@ -26,7 +26,9 @@ class RearrBlock:
     neighbour lists.  neighbours[i] is a (k, 4) array of (element, x, y, z)
     rows: the list read for cell atom i (block::readRearrangement, which is
     indexed by the atom's position in the input file while r_atom is in
-    sorted order -- the reference pairs them by index as well).
+    sorted order -- the reference pairs them by index as well).  The lists
+    may also be given flat: neighbours a (K, 4) array of all rows in atom
+    order and nb_count[i] the number of rows of atom i.
     coeff = Ncopy * Ncell.x * Ncell.y * Ncell.z * Nsymm."""
     r_atom: np.ndarray
     el: np.ndarray
@@ -34,35 +36,50 @@ class RearrBlock:
     rcorr: float
     coeff: int
     rearrangement: bool = True
+    nb_count: Optional[np.ndarray] = None
 
 
-def rmax2(r_atom, neighbours, rcorr) -> np.ndarray:
-    """Rmax2(rAtom[i], rAtomNeighb[i]) + SQR(Rcorr) (CalcFunctions.cpp:244-250, 490)."""
-    out = np.zeros(len(r_atom))
-    for i, nb in enumerate(neighbours):
-        m = 0.0
-        if len(nb):
-            d = r_atom[i][None, :] - np.asarray(nb, np.float64)[:, 1:4]
-            for s in (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]:
-                m = m if m > s else float(s)
-        out[i] = m + rcorr * rcorr
-    return out
+def _flat(neighbours, nb_count=None):
+    """All neighbour rows (k, 4) in atom order and the atom of each."""
+    if nb_count is not None:
+        counts = np.asarray(nb_count, np.int64)
+        return (np.asarray(neighbours, np.float64).reshape(-1, 4),
+                np.repeat(np.arange(counts.size), counts))
+    counts = np.array([len(nb) for nb in neighbours], dtype=np.int64)
+    rows = [np.asarray(nb, np.float64).reshape(-1, 4) for nb in neighbours if len(nb)]
+    flat = np.concatenate(rows) if rows else np.zeros((0, 4))
+    return flat, np.repeat(np.arange(len(neighbours)), counts)
 
 
-def neighbour_list(r_atom, el, neighbours, n_el):
-    """Distances |r_i - r_n| and types el_i * n_el + el_n of all listed
-    neighbours (CalcFunctions.cpp:509-510), atom by atom."""
-    rs, ts = [], []
-    for i, nb in enumerate(neighbours):
-        if not len(nb):
-            continue
-        nb = np.asarray(nb, np.float64)
-        d = r_atom[i][None, :] - nb[:, 1:4]
-        rs.append(np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]))
-        ts.append(int(el[i]) * n_el + nb[:, 0].astype(np.int64))
-    if not rs:
-        return np.zeros(0), np.zeros(0, np.uint32)
-    return np.concatenate(rs), np.concatenate(ts).astype(np.uint32)
+def _sqr(r_atom, flat, idx):
+    # (rAtom[i] - rAtomNeighb[i][k]).sqr(): (dx^2 + dy^2) + dz^2
+    d = r_atom[idx] - flat[:, 1:4]
+    return (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+
+
+def _prepare(r_atom, el, neighbours, rcorr, n_el, nb_count=None):
+    """(rmax2, neighbour distances, neighbour types) of one block."""
+    r_atom = np.asarray(r_atom, np.float64)
+    flat, idx = _flat(neighbours, nb_count)
+    m = np.zeros(len(r_atom))
+    if not idx.size:
+        return m + rcorr * rcorr, np.zeros(0), np.zeros(0, np.uint32)
+    s = _sqr(r_atom, flat, idx)
+    np.maximum.at(m, idx, s)
+    t = np.asarray(el, np.int64)[idx] * n_el + flat[:, 0].astype(np.int64)
+    return m + rcorr * rcorr, np.sqrt(s), t.astype(np.uint32)
+
+
+def rmax2(r_atom, neighbours, rcorr, nb_count=None) -> np.ndarray:
+    """Rmax2(rAtom[i], rAtomNeighb[i]) + SQR(Rcorr) (CalcFunctions.cpp:244-250, 490):
+    the largest squared neighbour distance (0 without neighbours)."""
+    return _prepare(r_atom, np.zeros(len(r_atom), np.int64), neighbours, rcorr, 1, nb_count)[0]
+
+
+def neighbour_list(r_atom, el, neighbours, n_el, nb_count=None):
+    """Distances |r_i - r_n| (mag(), CalcFunctions.cpp:510) and types
+    el_i * n_el + el_n of all listed neighbours, atom by atom."""
+    return _prepare(r_atom, el, neighbours, 0.0, n_el, nb_count)[1:]
 
 
 def polar_1d(I, q, lam):
@@ -103,12 +120,13 @@ def rearrangement_intensity(I0, q, blocks: Sequence[RearrBlock], n_el: int, n_to
         coeff = int(B.coeff) & 0xFFFFFFFF
         r = np.ascontiguousarray(B.r_atom, np.float64)
         el = np.asarray(B.el, np.uint32)
-        S_cut, npairs = cutoff_sums(np.ascontiguousarray(r.T), el, n_el,
-                                    rmax2(r, B.neighbours, B.rcorr), q)
-        rr, tt = neighbour_list(r, el, B.neighbours, n_el)
+        r2, rr, tt = _prepare(r, el, B.neighbours, B.rcorr, n_el, B.nb_count)
+        S_cut, npairs = cutoff_sums(np.ascontiguousarray(r.T), el, n_el, r2, q)
         S_nb = list_sums(rr, tt, n_el * n_el, q)
         nnb = np.bincount(tt // n_el, minlength=n_el) if tt.size else np.zeros(n_el, np.int64)
-        present = sorted(set(int(e) for e in el))  # block::id: ascending ensemble elements
+        nb_types = np.bincount(tt, minlength=n_el * n_el) if tt.size else \
+            np.zeros(n_el * n_el, np.int64)
+        present = [int(e) for e in np.unique(el)]  # block::id: ascending ensemble elements
         seg = slice(iqst, iqst + nq)
         for a in present:
             for b in present:
@@ -118,7 +136,7 @@ def rearrangement_intensity(I0, q, blocks: Sequence[RearrBlock], n_el: int, n_to
                 I[seg] -= w * S_cut[t]
             for b in range(n_el):
                 t = a * n_el + b
-                if not np.any(tt == t):
+                if not nb_types[t]:
                     continue
                 w = (coeff * ff[a] * ff[b]) if xray else (coeff * sl[a] * sl[b])
                 I[seg] += w * S_nb[t]
