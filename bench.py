@@ -98,33 +98,38 @@ def time_steps(ctx, step, steps, warmup):
     return (t1 - t0) * 1e3 / steps, kern
 
 
-def cpu_baseline(ens, nh, stride=3):
+def cpu_baseline(m=30, a=5.64, sigma=0.08):
     """Time the reference's own CPU histogram (XaNSoNS_OMP built from the
-    reference sources, calcHist timer) on one atom in `stride` of each
-    element (stride 3: ~5e10 pairs, about 10 s on 16 host threads); falls
-    back to the oracle port where that executable is absent."""
-    sub = [ens.element(e)[:, ::stride] for e in range(ens.n_el)]
-    counts = [s.shape[1] for s in sub]
-    n = sum(counts)
+    reference sources, its calcHist stage timer) on a bounded sample of the
+    workload: a rock-salt cube of m^3 conventional cells (the workload's
+    crystal, a = 5.64 A, displacements of sd 0.08 A; 8 m^3 atoms, all sites
+    occupied so the pair count is exact), built by the reference's own
+    ensemble builder from an 8-atom cell; m = 30 gives 216,000 atoms and
+    2.3e10 pairs, about 10 s on 16 host threads (the coordinates no longer
+    fit a core's L2 cache, as for the whole workload).  Falls back to the
+    oracle port where that executable is absent."""
+    n = 8 * m ** 3
     pairs = n * (n - 1) // 2
     threads = max(1, min(16, os.cpu_count() or 1))
-    sample = (f"one atom in {stride} of each element of the {ens.n}-atom rock-salt cube "
-              f"({n} atoms, {pairs:.3e} pairs), bin {BIN} A, histogram stage only")
+    sample = (f"rock-salt cube of {m}^3 conventional cells ({n} atoms, {pairs:.3e} pairs; the "
+              f"workload's crystal, a = {a} A, displacement sd {sigma} A), bin {BIN} A, "
+              "histogram stage only")
+    fcc = [(0, 0, 0), (0.5, 0.5, 0), (0.5, 0, 0.5), (0, 0.5, 0.5)]
     exe = os.path.join(REPO, "oracle", "_ref", "XaNSoNS_OMP")
     if os.path.exists(exe):
         tmp = tempfile.mkdtemp(prefix="xs_cpu_")
         try:
-            with open(os.path.join(tmp, "atoms.txt"), "w") as f:
-                for e, s in enumerate(sub):
-                    for j in range(s.shape[1]):
-                        f.write(f"{ens.names[e]} {float(s[0, j])!r} {float(s[1, j])!r} {float(s[2, j])!r}\n")
+            atoms = "".join(f'      <Atom name="{nm}" r="{f[0] + dx} {f[1]} {f[2]}" Uiso="{sigma ** 2!r}"/>\n'
+                            for nm, dx in (("Na", 0.0), ("Cl", 0.5)) for f in fcc)
             with open(os.path.join(tmp, "sl.txt"), "w") as f:
                 f.write("Na 3.63\nCl 9.577\n")
             with open(os.path.join(tmp, "start.xml"), "w") as f:
                 f.write('<?xml version="1.0"?>\n<Config>\n  <Calculation name="cpu" source="neutron" '
                         f'scenario="PDFonly" FFfilename="sl.txt" hist_bin="{BIN}">\n'
                         '    <PDF type="RDF" density="0.05"/>\n  </Calculation>\n'
-                        '  <Block>\n    <Atoms filename="atoms.txt"/>\n  </Block>\n</Config>\n')
+                        '  <Block fractional="Yes">\n    <Atoms>\n' + atoms + '    </Atoms>\n'
+                        f'    <CellVectors a="{a} 0 0" b="0 {a} 0" c="0 0 {a}" N="{m} {m} {m}"/>\n'
+                        '  </Block>\n</Config>\n')
             out = subprocess.run([exe, "start.xml", str(threads)], cwd=tmp, capture_output=True,
                                  text=True, timeout=600, check=True).stdout
             t = float(re.search(r"Histogram calculation time: ([0-9.eE+-]+) s", out).group(1))
@@ -135,9 +140,16 @@ def cpu_baseline(ens, nh, stride=3):
             shutil.rmtree(tmp, ignore_errors=True)
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import xs_oracle as O
-    xyz = np.concatenate(sub, axis=1).astype(np.float64)
+    r = np.arange(m, dtype=np.float64)
+    cells = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+    f = np.array(fcc, dtype=np.float64)
+    rng = np.random.default_rng(34)
+    na = ((cells[:, None] + f[None]).reshape(-1, 3)) * a
+    cl = ((cells[:, None] + (f + [0.5, 0, 0])[None]).reshape(-1, 3)) * a
+    xyz = np.concatenate([na, cl]).T + rng.normal(0, sigma, (3, n))
+    nh = O.hist_size(xyz, BIN)
     t0 = time.perf_counter()
-    O.hist_f64(xyz, counts, BIN, nh, threads)
+    O.hist_f64(np.ascontiguousarray(xyz), [n // 2, n // 2], BIN, nh, threads)
     t = time.perf_counter() - t0
     return dict(value=pairs / t / 1e9, unit="Gpairs/s", cores=threads, kind="port",
                 sample=sample + "; C restatement of calcHist (FP64, zig-zag rows)", seconds=t)
@@ -475,6 +487,11 @@ def debye_leg(ctx):
     return out
 
 
+def progress(msg):
+    """One line per finished leg on stderr (stdout carries only the result line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def atoms_workloads(n_cells=150, n_single=60000):
     """Ensemble-builder workloads: (name, blocks, n_el, files for the reference).
 
@@ -774,22 +791,30 @@ def main():
                         "hist_kernel_ms": round(kern32["hist"][0] / max(kern32["hist"][1], 1), 3)},
         "checks": {"pair_totals_exact": totals_ok, "overflow": int(ovf.item())},
     }
+    progress(f"headline done: {line['value']} Gpairs/s")
     if not args.no_matrix:
         line["pairsum_matrix"] = pairsum_matrix_leg(ctx, max(1, args.steps))
+        progress("pair-sum matrix leg done")
     if not args.no_2d:
         line["pattern_2d"] = amp2d_leg(ctx)
+        progress("2D leg done")
     if not args.no_debye:
         line["debye"] = debye_leg(ctx)
+        progress("Debye leg done")
     if args.ensemble > 0:
         del hist, ovf
         torch.cuda.empty_cache()
         line["ensemble"] = ensemble_leg(ctx, args.ensemble, max(1, args.steps // 2), 1)
+        progress("ensemble leg done")
     if not args.no_atoms:
         line["atoms_builder"] = atoms_leg(ctx, cpu=not args.no_cpu_baseline)
+        progress("ensemble-builder leg done")
     if not args.no_rearr:
         line["rearrangement"] = rearr_leg(ctx, cpu=not args.no_cpu_baseline)
+        progress("RearrangementInt leg done")
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(ens, nh)
+        line["cpu_baseline"] = cpu_baseline()
+        progress("CPU baseline done")
     ctx.close()
     print(json.dumps(line))
 
