@@ -98,16 +98,15 @@ def time_steps(ctx, step, steps, warmup):
     return (t1 - t0) * 1e3 / steps, kern
 
 
-def cpu_baseline(m=30, a=5.64, sigma=0.08):
+def cpu_baseline(m=36, a=5.64, sigma=0.08):
     """Time the reference's own CPU histogram (XaNSoNS_OMP built from the
     reference sources, its calcHist stage timer) on a bounded sample of the
     workload: a rock-salt cube of m^3 conventional cells (the workload's
     crystal, a = 5.64 A, displacements of sd 0.08 A; 8 m^3 atoms, all sites
     occupied so the pair count is exact), built by the reference's own
-    ensemble builder from an 8-atom cell; m = 30 gives 216,000 atoms and
-    2.3e10 pairs, about 10 s on 16 host threads (the coordinates no longer
-    fit a core's L2 cache, as for the whole workload).  Falls back to the
-    oracle port where that executable is absent."""
+    ensemble builder from an 8-atom cell; m = 36 gives 373,248 atoms and
+    7.0e10 pairs, about 11 s on 16 host threads.  Falls back to the oracle
+    port where that executable is absent."""
     n = 8 * m ** 3
     pairs = n * (n - 1) // 2
     threads = max(1, min(16, os.cpu_count() or 1))

@@ -40,52 +40,61 @@ struct CutoffArgs {
   double *r_out;
 };
 
-// Count (WRITE = false) or write (WRITE = true) the accepted pairs of atom
-// i = thread: every atom j in the 27 grid cells around i's cell (cell edge
-// >= sqrt(max rmax2), so no accepted pair is missed).  count[t]
-// (t = el_i * n_el + el_j): pairs per element pair; in the write pass slot[t]
-// hands out positions inside segment t (seg_off[t] ...).
+// Visit the accepted pairs of atom i: every atom j in the 27 grid cells
+// around i's cell (cell edge >= sqrt(max rmax2), so no accepted pair is
+// missed), f(type, r^2) for each.
+template <class F>
+__device__ inline void for_pairs(const CutoffArgs &A, uint32_t i, F f) {
+  const double xi = A.x[i], yi = A.y[i], zi = A.z[i], ri2 = A.rmax2[i];
+  const uint32_t ei = A.el[i] * A.n_el;
+  const int cx = grid_coord(xi, A.G.lo[0], A.G.inv, A.G.g[0]);
+  const int cy = grid_coord(yi, A.G.lo[1], A.G.inv, A.G.g[1]);
+  const int cz = grid_coord(zi, A.G.lo[2], A.G.inv, A.G.g[2]);
+  for (int gz = max(cz - 1, 0); gz <= min(cz + 1, A.G.g[2] - 1); ++gz)
+    for (int gy = max(cy - 1, 0); gy <= min(cy + 1, A.G.g[1] - 1); ++gy)
+      for (int gx = max(cx - 1, 0); gx <= min(cx + 1, A.G.g[0] - 1); ++gx) {
+        const uint32_t k = grid_cell(A.G, gx, gy, gz);
+        const uint32_t p1 = A.end[k];
+        for (uint32_t p = A.beg[k]; p < p1; ++p) {
+          // (rAtom[i] - rAtom[j]).sqr(), CalcFunctions.cpp:491
+          const double dx = xi - A.X[p], dy = yi - A.Y[p], dz = zi - A.Z[p];
+          const double r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 < ri2 && r2 > 0.000001) f(ei + A.el[A.order[p]], r2);
+        }
+      }
+}
+
+// Count (WRITE = false) or write (WRITE = true) the accepted pairs, one
+// thread per atom.  Counts are gathered per work group in LDS.  The write
+// pass counts again, reserves each element pair's range in its segment with
+// one global atomic per work group, and writes through LDS slots.
 template <bool WRITE>
 __global__ __launch_bounds__(kThreads) void cutoff_pairs(CutoffArgs A) {
-  extern __shared__ unsigned lcount[];  // [n_el * n_el], count pass only
+  extern __shared__ unsigned long long lds[];
   const uint32_t ntypes = A.n_el * A.n_el;
-  if (!WRITE) {
-    for (uint32_t t = threadIdx.x; t < ntypes; t += kThreads) lcount[t] = 0;
-    __syncthreads();
-  }
+  unsigned *lcount = reinterpret_cast<unsigned *>(lds + (WRITE ? ntypes : 0));
+  unsigned long long *base = lds;  // [ntypes], write pass only
+  for (uint32_t t = threadIdx.x; t < ntypes; t += kThreads) lcount[t] = 0;
+  __syncthreads();
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i < A.n) {
-    const double xi = A.x[i], yi = A.y[i], zi = A.z[i], ri2 = A.rmax2[i];
-    const uint32_t ei = A.el[i] * A.n_el;
-    const int cx = grid_coord(xi, A.G.lo[0], A.G.inv, A.G.g[0]);
-    const int cy = grid_coord(yi, A.G.lo[1], A.G.inv, A.G.g[1]);
-    const int cz = grid_coord(zi, A.G.lo[2], A.G.inv, A.G.g[2]);
-    for (int gz = max(cz - 1, 0); gz <= min(cz + 1, A.G.g[2] - 1); ++gz)
-      for (int gy = max(cy - 1, 0); gy <= min(cy + 1, A.G.g[1] - 1); ++gy)
-        for (int gx = max(cx - 1, 0); gx <= min(cx + 1, A.G.g[0] - 1); ++gx) {
-          const uint32_t k = grid_cell(A.G, gx, gy, gz);
-          const uint32_t p1 = A.end[k];
-          for (uint32_t p = A.beg[k]; p < p1; ++p) {
-            // (rAtom[i] - rAtom[j]).sqr(), CalcFunctions.cpp:491
-            const double dx = xi - A.X[p], dy = yi - A.Y[p], dz = zi - A.Z[p];
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < ri2 && r2 > 0.000001) {
-              const uint32_t t = ei + A.el[A.order[p]];
-              if (WRITE) {
-                const unsigned long long s = atomicAdd(&A.slot[t], 1ull);
-                A.r_out[A.seg_off[t] + s] = sqrt(r2);
-              } else {
-                atomicAdd(&lcount[t], 1u);
-              }
-            }
-          }
-        }
-  }
+  const bool vi = i < A.n;
+  if (vi) for_pairs(A, i, [&](uint32_t t, double) { atomicAdd(&lcount[t], 1u); });
+  __syncthreads();
   if (!WRITE) {
-    __syncthreads();
     for (uint32_t t = threadIdx.x; t < ntypes; t += kThreads)
       if (lcount[t]) atomicAdd(&A.count[t], (unsigned long long)lcount[t]);
+    return;
   }
+  for (uint32_t t = threadIdx.x; t < ntypes; t += kThreads) {
+    base[t] = lcount[t] ? A.seg_off[t] + atomicAdd(&A.slot[t], (unsigned long long)lcount[t]) : 0;
+    lcount[t] = 0;
+  }
+  __syncthreads();
+  if (vi)
+    for_pairs(A, i, [&](uint32_t t, double r2) {
+      const unsigned s = atomicAdd(&lcount[t], 1u);
+      A.r_out[base[t] + s] = sqrt(r2);
+    });
 }
 
 struct Chunk {
@@ -207,7 +216,9 @@ extern "C" int xs_cutoff_pairsum(xs_ctx *ctx, const double *xyz, size_t n_atoms,
   XS_TRY(grid_sort(ctx, x, y, z, n, std::sqrt(maxr2), sp));
   CutoffArgs A{x, y, z, sp.X, sp.Y, sp.Z, sp.order, sp.beg, sp.end, in.at<uint32_t>(o_e),
                in.at<double>(o_r), sp.G, (uint32_t)n, n_el, count, slot, seg, nullptr};
-  cutoff_pairs<false><<<nblk(n), kThreads, ntypes * sizeof(unsigned), ctx->stream>>>(A);
+  const size_t lds_count = align_up(ntypes * sizeof(unsigned), 8);
+  const size_t lds_write = ntypes * sizeof(unsigned long long) + lds_count;
+  cutoff_pairs<false><<<nblk(n), kThreads, lds_count, ctx->stream>>>(A);
   XS_HIP_CHECK(hipGetLastError());
   std::vector<unsigned long long> len(ntypes), off(ntypes);
   XS_TRY(download(ctx, len.data(), count, ntypes * sizeof(unsigned long long)));
@@ -224,7 +235,7 @@ extern "C" int xs_cutoff_pairsum(xs_ctx *ctx, const double *xyz, size_t n_atoms,
     XS_HIP_CHECK(hipMemcpyAsync(seg, off.data(), ntypes * sizeof(unsigned long long),
                                 hipMemcpyHostToDevice, ctx->stream));
     A.r_out = rd.at<double>();
-    cutoff_pairs<true><<<nblk(n), kThreads, 0, ctx->stream>>>(A);
+    cutoff_pairs<true><<<nblk(n), kThreads, lds_write, ctx->stream>>>(A);
     XS_HIP_CHECK(hipGetLastError());
   }
   XS_TRY(prof_end(ctx, "cutoff_pairs", ev));
