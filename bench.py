@@ -383,13 +383,20 @@ def amp2d_leg(ctx, nrot=16):
     """2D pattern throughput: the 50,000-atom three-element disordered solid,
     a 512 x 512 Cartesian q plane (|q_x|, |q_y| <= 8 A^-1, q_z = 0) and
     `nrot` of the 256 seeded uniform rotations; complex-exponential terms per
-    second.  Checks: Friedel symmetry I(q) = I(-q) over the plane, and a
-    sample of 64 q points x 2 rotations against an FP64 numpy sum."""
+    second of the grid kernel (xs_amp2d_grid), with the list kernel
+    (xs_amp2d, same points as an explicit list) timed beside it.  Checks:
+    Friedel symmetry I(q) = I(-q) over the plane, grid vs list, and a sample
+    of 64 q points x 2 rotations against an FP64 numpy sum."""
     import torch
     ens = st.disordered_cubic()
     x64 = ens.xyz64()
-    qv = st.q_plane(8.0, 512)
+    n = 512
+    qv = st.q_plane(8.0, n)
     nqp = qv.shape[0]
+    # q_plane's point k = i n + j has q = (g[i], g[j], 0): rows along q_y
+    step = 2.0 * 8.0 / (n - 1)
+    q0, du, dv = np.array([qv[0, 0], qv[0, 1], 0.0]), np.array([0.0, step, 0.0]), \
+        np.array([step, 0.0, 0.0])
     rot = st.uniform_rotations(256, seed=8)[:nrot].reshape(nrot, 9)
     ff = st.gaussian_ff(np.linalg.norm(qv, axis=1), st.gaussian_ff_coeffs(3, seed=9))
     dev = torch.device("cuda", 0)
@@ -398,23 +405,35 @@ def amp2d_leg(ctx, nrot=16):
     rd = torch.from_numpy(np.ascontiguousarray(rot)).to(dev)
     fd = torch.from_numpy(np.ascontiguousarray(ff)).to(dev)
     Id = torch.empty(nqp, dtype=torch.float64, device=dev)
+    Il = torch.empty(nqp, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
 
-    def run():
-        ctx.amp2d_dev(xd.data_ptr(), ens.n, ens.el_count, qd.data_ptr(), nqp, rd.data_ptr(), nrot,
-                      fd.data_ptr(), Id.data_ptr())
+    def grid():
+        ctx.amp2d_grid_dev(xd.data_ptr(), ens.n, ens.el_count, q0, du, dv, n, n, rd.data_ptr(),
+                           nrot, fd.data_ptr(), Id.data_ptr())
 
-    run()
-    ctx.synchronize()
-    ctx.profile(True)
-    t0 = time.perf_counter()
-    run()
-    ctx.synchronize()
-    t = time.perf_counter() - t0
-    kms = ctx.profile_read("amp2d")[0]
-    ctx.profile(False)
+    def listed():
+        ctx.amp2d_dev(xd.data_ptr(), ens.n, ens.el_count, qd.data_ptr(), nqp, rd.data_ptr(), nrot,
+                      fd.data_ptr(), Il.data_ptr())
+
+    def timed(fn):
+        fn()
+        ctx.synchronize()
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        fn()
+        ctx.synchronize()
+        t = time.perf_counter() - t0
+        k = ctx.profile_read("amp2d")[0]
+        ctx.profile(False)
+        return t, k
+
+    t, kms = timed(grid)
+    _, kms_list = timed(listed)
     I = Id.cpu().numpy()
     fried = float(np.max(np.abs(I - I[::-1]) / np.maximum(np.abs(I), 1e-300)))
+    IL = Il.cpu().numpy()
+    vs_list = float(np.max(np.abs(I - IL)) / np.abs(IL).max())
     # FP64 numpy evaluation of 64 q points x 2 rotations
     idx = np.random.Generator(np.random.PCG64(1)).choice(nqp, 64, replace=False)
     Ie = np.zeros(64)
@@ -427,13 +446,10 @@ def amp2d_leg(ctx, nrot=16):
             amp += ff[e, idx] * np.exp(1j * ph).sum(axis=1)
             off += c
         Ie += np.abs(amp) ** 2
-    Ig = torch.empty(64, dtype=torch.float64, device=dev)
-    qs = torch.from_numpy(np.ascontiguousarray(qv[idx])).to(dev)
-    fs = torch.from_numpy(np.ascontiguousarray(ff[:, idx])).to(dev)
-    ctx.amp2d_dev(xd.data_ptr(), ens.n, ens.el_count, qs.data_ptr(), 64, rd.data_ptr(), 2,
-                  fs.data_ptr(), Ig.data_ptr())
+    ctx.amp2d_grid_dev(xd.data_ptr(), ens.n, ens.el_count, q0, du, dv, n, n, rd.data_ptr(), 2,
+                       fd.data_ptr(), Id.data_ptr())
     ctx.synchronize()
-    err = float(np.max(np.abs(Ig.cpu().numpy() - Ie) / Ie))
+    err = float(np.max(np.abs(Id.cpu().numpy()[idx] - Ie) / Ie))
     terms = ens.n * nqp * nrot
     return {
         "workload": f"{ens.n}-atom 3-element disordered solid, 512 x 512 q plane (|q| components <= 8), "
@@ -441,7 +457,10 @@ def amp2d_leg(ctx, nrot=16):
         "terms": terms, "ms": round(t * 1e3, 2), "kernel_ms": round(kms, 2),
         "gterms_per_s": round(terms / (kms * 1e-3) / 1e9, 1),
         "full_256_rotations_s_extrapolated": round(kms * 1e-3 * 256 / nrot, 3),
-        "checks": {"friedel_max_rel": fried, "fp64_sample_max_rel": err,
+        "list_kernel_ms": round(kms_list, 2),
+        "list_gterms_per_s": round(terms / (kms_list * 1e-3) / 1e9, 1),
+        "checks": {"friedel_max_rel": fried, "grid_vs_list_max_rel": vs_list,
+                   "fp64_sample_max_rel": err,
                    "fp64_sample": "64 q points x 2 rotations, FP64 numpy sum"},
     }
 

@@ -161,6 +161,19 @@ int xs_amp2d(xs_ctx *ctx, const double *xyz, size_t n_atoms,
              size_t nqp, const double *rot, unsigned nrot, const double *ff,
              double *I, unsigned flags);
 
+/* H7 on a regular q grid: q(iu, iv) = q0 + iu du + iv dv (iu < nu, iv < nv),
+ * point k = iv*nu + iu; otherwise as xs_amp2d (ff[el*nu*nv + k], I[k]).
+ * Along a row the phase of an atom advances by (R^T du).x per point, so
+ * runs of 8 points are evaluated from one FP32 hardware sine/cosine of the
+ * FP64-reduced phase (the list kernel's accuracy) and FP64 complex
+ * rotations.  q0, du, dv are host arrays of 3; the rest as xs_amp2d.  The
+ * Cartesian q planes of 2D patterns are such grids; the reference's polar
+ * (q, phi) mesh is not (use xs_amp2d). */
+int xs_amp2d_grid(xs_ctx *ctx, const double *xyz, size_t n_atoms, const unsigned *el_count,
+                  unsigned n_el, const double *q0, const double *du, const double *dv,
+                  unsigned nu, unsigned nv, const double *rot, unsigned nrot, const double *ff,
+                  double *I, unsigned flags);
+
 /* Cut-off pair sums of RearrangementInt (CalcFunctions.cpp:485-499):
  * S[(el[i]*n_el + el[j])*nq + iq] = sum over ordered pairs (i, j) with
  * 1e-6 < r_ij^2 < rmax2[i] of sin(q r_ij)/(q r_ij + 1e-8), r_ij^2 =

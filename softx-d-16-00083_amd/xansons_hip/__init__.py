@@ -71,6 +71,7 @@ def lib():
         L.xs_batch_run.argtypes = [vp, vp, vp, u, vp, i, vp, vp, vp, u]
         L.xs_batch_overflow.argtypes = [vp, vp]
         L.xs_batch_free.argtypes = [vp]
+        L.xs_amp2d_grid.argtypes = [vp, vp, sz, vp, u, vp, vp, vp, u, u, vp, u, vp, vp, u]
         L.xs_cutoff_pairsum.argtypes = [vp, vp, sz, vp, u, vp, vp, u, vp, vp, u]
         L.xs_pairlist_pairsum.argtypes = [vp, vp, vp, sz, u, vp, u, vp, u]
         L.xs_atoms_build.argtypes = [vp, vp, u, u, C.c_uint64, C.POINTER(vp)]
@@ -262,6 +263,28 @@ class Context:
         _check(lib().xs_amp2d(self.h, _p(xyz), int(ec.sum()), _p(ec), len(ec), _p(qvec),
                               qvec.shape[0], _p(rot), rot.shape[0], _p(ff), _p(I), 0))
         return I
+
+    def amp2d_grid(self, xyz, el_count, q0, du, dv, nu: int, nv: int, rot, ff):
+        """xs_amp2d_grid: q(iu, iv) = q0 + iu du + iv dv, point k = iv nu + iu."""
+        ec = _u32(el_count)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        rot = np.ascontiguousarray(rot, dtype=np.float64).reshape(-1, 9)
+        ff = np.ascontiguousarray(ff, dtype=np.float64).reshape(len(ec), nu * nv)
+        v3 = [np.ascontiguousarray(v, dtype=np.float64).reshape(3) for v in (q0, du, dv)]
+        I = np.zeros(nu * nv)
+        _check(lib().xs_amp2d_grid(self.h, _p(xyz), int(ec.sum()), _p(ec), len(ec), _p(v3[0]),
+                                   _p(v3[1]), _p(v3[2]), int(nu), int(nv), _p(rot), rot.shape[0],
+                                   _p(ff), _p(I), 0))
+        return I
+
+    def amp2d_grid_dev(self, xyz_ptr, n, el_count, q0, du, dv, nu, nv, rot_ptr, nrot, ff_ptr,
+                       I_ptr):
+        ec = _u32(el_count)
+        v3 = [np.ascontiguousarray(v, dtype=np.float64).reshape(3) for v in (q0, du, dv)]
+        _check(lib().xs_amp2d_grid(self.h, C.c_void_p(xyz_ptr), int(n), _p(ec), len(ec),
+                                   _p(v3[0]), _p(v3[1]), _p(v3[2]), int(nu), int(nv),
+                                   C.c_void_p(rot_ptr), int(nrot), C.c_void_p(ff_ptr),
+                                   C.c_void_p(I_ptr), XS_DEVICE_PTRS))
 
     def amp2d_dev(self, xyz_ptr, n, el_count, qvec_ptr, nqp, rot_ptr, nrot, ff_ptr, I_ptr):
         ec = _u32(el_count)

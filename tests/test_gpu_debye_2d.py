@@ -114,3 +114,28 @@ def test_amp2d_disordered_plane(ctx):
     allrel = float(np.max(np.abs(Ig - Io) / Io))
     print(f"2d plane: {pw:.3e} {l2:.3e} all-points {allrel:.3e}", flush=True)
     assert pw <= TOL and l2 <= TOL
+
+
+@pytest.mark.parametrize("nu,nv", [(64, 64), (61, 7)])
+def test_amp2d_grid_equals_list_and_oracle(ctx, nu, nv):
+    """Regular q grid (xs_amp2d_grid, runs of 8 points advanced by FP64
+    complex rotations) against the same points as a list (xs_amp2d) and the
+    FP64 oracle; rows of 61 points end in a partial run."""
+    ens = st.disordered_cubic(natoms=4000, n=16)
+    x64 = ens.xyz64()
+    q0 = np.array([-8.0, -7.5, 0.3])
+    du = np.array([0.0, 16.0 / 63, 0.01])
+    dv = np.array([16.0 / 63, 0.0, -0.02])
+    iv, iu = np.meshgrid(np.arange(nv), np.arange(nu), indexing="ij")
+    qv = q0[None] + iu.reshape(-1, 1) * du[None] + iv.reshape(-1, 1) * dv[None]
+    ff = st.gaussian_ff(np.linalg.norm(qv, axis=1), st.gaussian_ff_coeffs(3, seed=9))
+    rng = np.random.default_rng(18)
+    rot = np.stack([O.rot_matrix(rng.uniform(-3, 3, 3), 5).ravel() for _ in range(3)])
+    Ig = ctx.amp2d_grid(x64, ens.el_count, q0, du, dv, nu, nv, rot, ff)
+    Il = ctx.amp2d(x64, ens.el_count, qv, rot, ff)
+    Io = O.amp2d(x64, ens.el_count, qv, rot, ff)
+    pw, l2 = rel_err(Ig, Io)
+    pl, _ = rel_err(Ig, Il)
+    print(f"2d grid {nu}x{nv}: vs oracle {pw:.3e} {l2:.3e}, vs list {pl:.3e}", flush=True)
+    assert pw <= TOL and l2 <= TOL
+    assert pl <= TOL
