@@ -10,8 +10,11 @@
 // into LDS (r/2pi, 1/r and, on a uniform q mesh, sin/cos of the phase step
 // dq r), then each lane owns 16 consecutive q and each wave a quarter of the
 // pairs: sin(q r) is seeded once per pair and lane from the series in
-// xs_sincos.h and advanced to the next q by the stored rotation (4 FMAs), so a
-// pair-q term costs about 7 FP64 instructions.  FP64 because the intensity at
+// xs_sincos.h, advanced one q by the stored rotation and then by the
+// three-term recurrence s_{k+1} = 2 cos(dq r) s_k - s_{k-1} (one FMA; a
+// rounding error made at step j grows by |sin((k-j) dq r)/sin(dq r)| <= k-j,
+// so over 16 q it stays near 1e-14), so a pair-q term costs about 3 FP64
+// instructions.  FP64 because the intensity at
 // low q is a strongly cancelling difference (see xs_iq.hip).  The 1e-8 of the
 // denominator is applied to first order through a second accumulator of
 // sin(q r)/r^2.  The four waves' partial sums are combined in LDS and added to
@@ -88,15 +91,23 @@ __global__ __launch_bounds__(kBlock) void debye_kernel(const double *__restrict_
     if (v.y == 0.0) continue;
     const double ir = v.y, ir2 = v.y * v.y;
     if (UNIFORM) {
-      double s, c;
-      sincos_2pi(q0 * v.x, &s, &c);
+      // s_0 = sin(q0 r), s_1 by one rotation, then the three-term
+      // recurrence s_{k+1} = 2 cos(dq r) s_k - s_{k-1} (one FMA a q)
+      double s0, c0;
+      sincos_2pi(q0 * v.x, &s0, &c0);
+      double sp = s0, s = fma(s0, v.w, c0 * v.z);
+      const double c2 = 2.0 * v.w;
+      acc[0] = fma(sp, ir, acc[0]);
+      corr[0] = fma(sp, ir2, corr[0]);
 #pragma unroll
-      for (int k = 0; k < kQPT; ++k) {
+      for (int k = 1; k < kQPT; ++k) {
         acc[k] = fma(s, ir, acc[k]);
         corr[k] = fma(s, ir2, corr[k]);
-        const double sn = fma(s, v.w, c * v.z);
-        c = fma(c, v.w, -(s * v.z));
-        s = sn;
+        if (k + 1 < kQPT) {
+          const double sn = fma(c2, s, -sp);
+          sp = s;
+          s = sn;
+        }
       }
     } else {
 #pragma unroll
