@@ -3,7 +3,7 @@
 
 Workload (one step): the 980,219-atom two-element rock-salt cube
 (a = 5.64 A, 50^3 cells, 2% vacancies, sigma = 0.08 A):
-  1. three pair-type distance histograms, bin 0.005 A (96,711 bins, uint64)
+  1. three pair-type distance histograms, bin 0.005 A (97,251 bins, uint64)
   2. histogram -> pair sums on 4,096 q over 0.2-25 A^-1
   3. rPDF G(r) over all bins (0-483 A)
 Inputs are resident in HBM before the timed region; outputs stay on the GPU.
@@ -42,18 +42,20 @@ NQ = 4096
 FP64_PEAK_TFLOPS = 78.6           # MI355X published FP64 vector peak (binning is FP64)
 VALU_ISSUE_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12   # non-packed lane-ops/s (39.3 T)
 OPS_PER_PAIR = 11                 # 3 sub, 3 mul, 2 add, sqrt, div, cvt
-# FP64-rate VALU issue slots per pair of the histogram kernel, from rocprofv3
-# SQ counters on this workload: 18.2 VALU wave-instructions per 64 pairs, one
-# of them v_rsq_f64, which issues at a quarter of the v_fma_f64 rate (3 extra
-# slots); and the measured v_fma_f64 issue rate on this part (8 independent
-# chains per lane, 2,048 work groups).
-SLOTS_PER_PAIR = 21.4
-FMA_F64_MEASURED_TOPS = 29.57
+# VALU issue cycles per 64 pairs of the histogram kernel, from rocprofv3 SQ
+# counters on this workload: 17.66 VALU wave-instructions per 64 pairs, 1.03
+# of them v_rsq_f64, which occupies the issue port four times as long as
+# v_fma_f64 (3 extra cycles); SQ_ACTIVE_INST_VALU reads the same 20.75.  A CU
+# issues one 64-lane VALU instruction per cycle (4 SIMDs x 16 lanes), so the
+# fraction of the CUs' issue cycles at the 2.4 GHz peak clock the kernel
+# spends issuing VALU work is SLOTS_PER_PAIR * pairs / 64 / (256 * 2.4e9 * t).
+SLOTS_PER_PAIR = 20.75
+CU_ISSUE_PER_S = 256 * 2.4e9
 # HBM bytes per launch of hist_kernel<double,32768> on this workload, from
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch), scaled by
 # the counters' reading for a 1 GiB device-to-device copy (FETCH_SIZE reads
 # 524,308 KiB for the 1,048,576 KiB copied, WRITE_SIZE 1,048,576 KiB).
-HIST_FETCH_KIB, HIST_WRITE_KIB = 4533620, 1302084
+HIST_FETCH_KIB, HIST_WRITE_KIB = 1966735, 1113357
 FETCH_SCALE, WRITE_SCALE = 1048576 / 524308, 1.0
 HIST_TRAFFIC_BYTES = int((HIST_FETCH_KIB * FETCH_SCALE + HIST_WRITE_KIB * WRITE_SCALE) * 1024)
 
@@ -788,7 +790,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded rock-salt cube)",
         "config": {"workload": "rock-salt cube 980,219 atoms, 2 elements, 3 pair-type histograms "
-                               "bin 0.005 A (96,711 bins), I(q) 4,096 q 0.2-25 A^-1, rPDF",
+                               f"bin 0.005 A ({nh:,} bins), I(q) 4,096 q 0.2-25 A^-1, rPDF",
                    "atoms": ens.n, "pairs": pairs, "nhist": nh, "nq": NQ,
                    "binning": "FP64 (reference CPU expression, counts bit-exact)",
                    "parallelism": "single GPU"},
@@ -797,12 +799,12 @@ def main():
                      "traffic": HIST_TRAFFIC_BYTES, "kernel": "hist_kernel<double,32768>",
                      "ops_per_pair": OPS_PER_PAIR, "kernel_ms": round(hist_ms, 3),
                      "valu_issue_frac": round(achieved / VALU_ISSUE_TOPS, 4),
-                     "valu_slots_per_pair": SLOTS_PER_PAIR,
-                     "valu_slot_frac_of_measured_fma_rate": round(
-                         SLOTS_PER_PAIR * pairs / (hist_ms * 1e-3) / 1e12 / FMA_F64_MEASURED_TOPS, 4),
+                     "valu_issue_cycles_per_64_pairs": SLOTS_PER_PAIR,
+                     "valu_issue_cycle_frac": round(
+                         SLOTS_PER_PAIR * pairs / 64 / (CU_ISSUE_PER_S * hist_ms * 1e-3), 4),
                      "traffic_note": "HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE, copy-calibrated); "
-                                     "the J tiles are re-read per work item from HBM/MALL: ~9 GB at 26 GB/s, "
-                                     "0.3% of HBM bandwidth"},
+                                     "J tiles re-read per work item from MALL/HBM and histogram window "
+                                     "flushes: ~5.2 GB at ~17 GB/s, 0.2% of HBM bandwidth"},
         "kernels_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in kern.items()},
         "f32_binning": {"ms_per_step": round(ms32, 3),
                         "gpairs_per_s": round(pairs / (ms32 * 1e-3) / 1e9, 3),

@@ -35,6 +35,8 @@
 #include <numeric>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "xs_internal.h"
 
 namespace xs {
@@ -206,6 +208,54 @@ __global__ void fill_u64(unsigned long long *__restrict__ p, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
     p[i] = 0ull;
+}
+
+// ---- equal-count tiles on the device (see eq_tiles_device) ----
+
+__global__ void iota_u32(uint32_t *p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+
+// Part of rank r when n ranks are split into g equal-count parts, part s
+// starting at floor(s n / g).
+__device__ inline uint32_t equal_part(uint64_t r, uint64_t n, uint64_t g) {
+  uint64_t s = r * g / n;
+  while (s + 1 < g && (s + 1) * n / g <= r) ++s;
+  while (s > 0 && s * n / g > r) --s;
+  return (uint32_t)s;
+}
+
+// slab[ord[r]] = equal part of rank r (ord: atoms in x order)
+__global__ void eq_slabs(const uint32_t *ord, uint32_t n, uint32_t gx, uint32_t *slab) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) slab[ord[r]] = equal_part(r, n, gx);
+}
+
+__global__ void gather_u32(const uint32_t *src, const uint32_t *ord, uint32_t n, uint32_t *dst) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) dst[r] = src[ord[r]];
+}
+
+// col[ord[r]] = s gy + equal part of r within slab s (ord: slab-major, y order)
+__global__ void eq_columns(const uint32_t *sslab, const uint32_t *ord, uint32_t n, uint32_t gx,
+                           uint32_t gy, uint32_t *col) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint64_t s = sslab[r];
+  const uint64_t st = s * n / gx, en = (s + 1) * n / gx;
+  col[ord[r]] = (uint32_t)s * gy + equal_part(r - st, en - st, gy);
+}
+
+template <class T>
+__global__ void gather_xyz(const T *x, const T *y, const T *z, const uint32_t *ord, uint32_t n,
+                           T *sx, T *sy, T *sz) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint32_t i = ord[r];
+  sx[r] = x[i];
+  sy[r] = y[i];
+  sz[r] = z[i];
 }
 
 // ---------------------------------------------------------------- histogram kernel
@@ -678,69 +728,78 @@ static void add_types(unsigned n_el, const uint32_t *ebase, const uint32_t *en, 
     }
 }
 
-// Host-side counting sort of one structure into the sorted arrays (appended),
-// with its tiles and boxes.  xyz: SoA of the structure, element-major.
+// Equal-count k-d tiles of one element (indices idx[0, ne) into x, y, z): the
+// longest axis of a part's bounding box is split at a multiple of kTile atoms
+// (nth_element) until a part holds at most kTile atoms and spans at most
+// `cap` on every axis.  Every tile of an element but one is full, so the
+// lanes of the I tiles are busy whatever the structure's shape (grid cells
+// over a sphere leave about half of them idle), and tiles are compact boxes.
+// Leaves come out depth first, left part first (neighbouring tiles are close).
+template <class T>
+static void kd_tiles(const T *x, const T *y, const T *z, uint32_t *idx, uint32_t ne, double cap,
+                     std::vector<std::pair<uint32_t, uint32_t>> &leaves) {
+  const T *c[3] = {x, y, z};
+  std::vector<std::pair<uint32_t, uint32_t>> stack{{0u, ne}};
+  while (!stack.empty()) {
+    const auto part = stack.back();
+    stack.pop_back();
+    const uint32_t lo = part.first, n = part.second - part.first;
+    if (!n) continue;
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = lo; i < part.second; ++i)
+      for (int a = 0; a < 3; ++a) {
+        const double v = (double)c[a][idx[i]];
+        mn[a] = std::min(mn[a], v);
+        mx[a] = std::max(mx[a], v);
+      }
+    int ax = 0;
+    for (int a = 1; a < 3; ++a)
+      if (mx[a] - mn[a] > mx[ax] - mn[ax]) ax = a;
+    if (n == 1 || (n <= (uint32_t)kTile && !(mx[ax] - mn[ax] > cap))) {
+      leaves.push_back({lo, n});
+      continue;
+    }
+    const uint32_t nl = std::max<uint32_t>(2u, (n + kTile - 1) / kTile);
+    uint32_t left = (nl / 2) * (uint32_t)kTile;
+    if (left >= n) left = n / 2;
+    const T *ca = c[ax];
+    std::nth_element(idx + lo, idx + lo + left, idx + part.second,
+                     [ca](uint32_t a, uint32_t b) { return ca[a] < ca[b]; });
+    stack.push_back({lo + left, part.second});
+    stack.push_back({lo, lo + left});
+  }
+}
+
+// Host-side sort of one structure into the sorted arrays (appended), with its
+// tiles (equal-count k-d tiles per element) and boxes.  xyz: SoA of the
+// structure, element-major.
 template <class T>
 static void sort_structure_host(const T *xyz, size_t n, const unsigned *el_count, unsigned n_el,
                                 double bin, uint32_t nhist, std::vector<T> &sx, std::vector<T> &sy,
                                 std::vector<T> &sz, std::vector<Tile> &tiles,
                                 std::vector<double> &boxes, uint32_t *ebase, uint32_t *en) {
-  double bbox[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  for (int a = 0; a < 3; ++a)
-    for (size_t i = 0; i < n; ++i) {
-      const double v = (double)xyz[a * n + i];
-      bbox[a] = std::min(bbox[a], v);
-      bbox[3 + a] = std::max(bbox[3 + a], v);
-    }
-  if (n == 0)
-    for (int a = 0; a < 6; ++a) bbox[a] = 0.0;
-  // each element on its own grid, sized for full tiles of that element
   const double cap = cell_cap(bin, nhist, wcap_for(nhist));
-  std::vector<uint32_t> key, cnt;
+  std::vector<uint32_t> idx;
+  std::vector<std::pair<uint32_t, uint32_t>> leaves;
   size_t off = 0;
   for (unsigned e = 0; e < n_el; ++e) {
-    const size_t ne = el_count[e];
-    double cell = first_cell(bbox, (unsigned)ne, cap);
-    Grid gr;
-    uint32_t m;
-    key.assign(ne, 0u);
-    for (int pass = 0;; ++pass) {
-      gr = grid_with_cell(bbox, cell, cap);
-      m = gr.g[0] * gr.g[1] * gr.g[2];
-      cnt.assign(m, 0u);
-      for (size_t i = 0; i < ne; ++i) {
-        uint32_t c[3];
-        for (int a = 0; a < 3; ++a) {
-          const double f = ((double)xyz[a * n + off + i] - gr.lo[a]) * gr.inv_cell[a];
-          const int ci = std::isfinite(f) ? (int)f : 0;
-          c[a] = (uint32_t)std::min(std::max(ci, 0), (int)gr.g[a] - 1);
-        }
-        key[i] = (c[0] * gr.g[1] + c[1]) * gr.g[2] + c[2];
-        ++cnt[key[i]];
-      }
-      const uint32_t fullest = ne ? *std::max_element(cnt.begin(), cnt.end()) : 0u;
-      const double nc = pass == 0 ? refined_cell(cell, fullest, cap) : 0.0;
-      if (nc == 0.0) break;
-      cell = nc;
-    }
+    const uint32_t ne = el_count[e];
+    idx.resize(ne);
+    for (uint32_t i = 0; i < ne; ++i) idx[i] = i;
+    leaves.clear();
+    kd_tiles<T>(xyz + off, xyz + n + off, xyz + 2 * n + off, idx.data(), ne, cap, leaves);
     const size_t base = sx.size();
-    uint32_t p = (uint32_t)base;
     ebase[e] = (uint32_t)tiles.size();
-    for (uint32_t c = 0; c < m; ++c) {
-      const uint32_t k = cnt[c];
-      cnt[c] = p;  // becomes the next free slot of the cell
-      cell_tiles(p, k, tiles);
-      p += k;
-    }
+    for (const auto &l : leaves) tiles.push_back(Tile{(uint32_t)(base + l.first), l.second});
     en[e] = (uint32_t)tiles.size() - ebase[e];
     sx.resize(base + ne);
     sy.resize(base + ne);
     sz.resize(base + ne);
-    for (size_t i = 0; i < ne; ++i) {
-      const uint32_t d = cnt[key[i]]++;
-      sx[d] = xyz[off + i];
-      sy[d] = xyz[n + off + i];
-      sz[d] = xyz[2 * n + off + i];
+    for (uint32_t d = 0; d < ne; ++d) {
+      const uint32_t i = idx[d];
+      sx[base + d] = xyz[off + i];
+      sy[base + d] = xyz[n + off + i];
+      sz[base + d] = xyz[2 * n + off + i];
     }
     off += ne;
   }
@@ -775,10 +834,140 @@ static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wc
 
 // ---------------------------------------------------------------- xs_hist
 
+// Equal-count partition of one element for the device path: gx slabs along
+// x and gy columns along y in each slab, all of equal count, so every column
+// holds ne / (gx gy) atoms (+-1) whatever the shape; columns are cut along z
+// into ceil(count / kTile) (at least nz) equal tiles.  gx, gy make tiles
+// about cubic and as full as possible; nx, ny, nz make slabs, columns and
+// tiles of a uniform-density structure narrower than 0.9 of the window cap
+// (the cap itself is checked on the tile boxes).
+static void choose_equal_count(uint32_t ne, const double bbox[6], double cap, uint32_t &gx,
+                               uint32_t &gy, uint32_t &nz) {
+  double ext[3];
+  for (int a = 0; a < 3; ++a) ext[a] = std::max(bbox[3 + a] - bbox[a], 1e-6);
+  auto need = [&](double e) -> uint32_t {
+    return std::isfinite(cap) ? (uint32_t)std::max(1.0, std::ceil(e / (0.9 * cap))) : 1u;
+  };
+  const uint32_t nx = need(ext[0]), ny = need(ext[1]);
+  nz = need(ext[2]);
+  const double T = std::max(1.0, std::ceil(ne / (double)kTile));
+  const double L = std::cbrt(ext[0] * ext[1] * ext[2] / T);
+  const long cx = std::lround(ext[0] / L), cy = std::lround(ext[1] / L);
+  double best = -INFINITY;
+  gx = std::max(nx, 1u);
+  gy = std::max(ny, 1u);
+  for (long a = cx - 2; a <= cx + 2; ++a)
+    for (long b = cy - 2; b <= cy + 2; ++b) {
+      const uint64_t ga = std::max<uint64_t>(std::max<long>(a, 1), nx);
+      const uint64_t gb = std::max<uint64_t>(std::max<long>(b, 1), ny);
+      if (ga * gb > std::max<uint64_t>(ne, 1) || ga * gb > 65536) continue;
+      const double col = (double)ne / (double)(ga * gb);
+      const double tpc = std::max(std::ceil(col / kTile), (double)nz);
+      const double fill = ne / ((double)(ga * gb) * tpc * kTile);
+      const double shape = std::fabs(std::log(ext[0] / ga / L)) + std::fabs(std::log(ext[1] / gb / L)) +
+                           std::fabs(std::log(ext[2] / tpc / L));
+      const double score = fill - 0.01 * shape;
+      if (score > best) {
+        best = score;
+        gx = (uint32_t)ga;
+        gy = (uint32_t)gb;
+      }
+    }
+}
+
+// Tiles of the equal-count partition (positions from `pos`), column by column.
+static void equal_count_tiles(uint32_t pos, uint32_t ne, uint32_t gx, uint32_t gy, uint32_t nz,
+                              std::vector<Tile> &tiles) {
+  for (uint64_t sl = 0; sl < gx; ++sl) {
+    const uint64_t S = (sl + 1) * ne / gx - sl * ne / gx;
+    for (uint64_t c = 0; c < gy; ++c) {
+      const uint32_t C = (uint32_t)((c + 1) * S / gy - c * S / gy);
+      if (!C) continue;
+      const uint32_t nt = std::min(C, std::max((C + kTile - 1) / (uint32_t)kTile, nz));
+      for (uint32_t t = 0, o = 0; t < nt; ++t) {
+        const uint32_t len = (C - o) / (nt - t);
+        tiles.push_back(Tile{pos + o, len});
+        o += len;
+      }
+      pos += C;
+    }
+  }
+}
+
+// Scratch of eq_tiles_device for elements of up to maxne atoms.
+template <class T>
+static size_t eq_scratch_bytes(uint32_t maxne, size_t *sort_bytes) {
+  size_t b1 = 0, b2 = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, b1, (const T *)nullptr, (T *)nullptr,
+                                           (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                           (int)maxne);
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, b2, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                           (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                           (int)maxne);
+  *sort_bytes = align_up(std::max(b1, b2), 256);
+  return Carver::need({maxne * sizeof(T), 6 * (size_t)maxne * sizeof(uint32_t), *sort_bytes});
+}
+
+// Equal-count tiles of every element on the device: stable radix sorts by
+// x (slabs), by y then slab (columns), by z then column (tiles); the sorted
+// coordinates go to sxp, syp, szp (element-major, as the input).
+template <class T>
+static int eq_tiles_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_count,
+                           unsigned n_el, const uint32_t *eoff, const double bbox[6], double cap,
+                           void *scratch, uint32_t maxne, T *sxp, T *syp, T *szp,
+                           std::vector<Tile> &tiles, uint32_t *ebase, uint32_t *en) {
+  size_t sb = 0;
+  (void)eq_scratch_bytes<T>(maxne, &sb);
+  Carver cv(scratch);
+  T *kout = cv.take<T>(maxne);
+  uint32_t *iota = cv.take<uint32_t>(maxne), *ordA = cv.take<uint32_t>(maxne),
+           *ordB = cv.take<uint32_t>(maxne), *lab = cv.take<uint32_t>(maxne),
+           *k32a = cv.take<uint32_t>(maxne), *k32b = cv.take<uint32_t>(maxne);
+  void *tmp = cv.take<char>(sb);
+  hipStream_t st = ctx->stream;
+  for (unsigned e = 0; e < n_el; ++e) {
+    const uint32_t ne = el_count[e];
+    ebase[e] = (uint32_t)tiles.size();
+    en[e] = 0;
+    if (!ne) continue;
+    const T *ex = xyz + eoff[e], *ey = xyz + n + eoff[e], *ez = xyz + 2 * n + eoff[e];
+    uint32_t gx, gy, nz;
+    choose_equal_count(ne, bbox, cap, gx, gy, nz);
+    const unsigned nb = (ne + 255) / 256;
+    size_t b = sb;
+    hipLaunchKernelGGL(iota_u32, dim3(nb), dim3(256), 0, st, iota, ne);
+    XS_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, b, ex, kout, iota, ordA, (int)ne, 0,
+                                                    (int)(sizeof(T) * 8), st));
+    hipLaunchKernelGGL(eq_slabs, dim3(nb), dim3(256), 0, st, ordA, ne, gx, lab);
+    b = sb;
+    XS_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, b, ey, kout, iota, ordA, (int)ne, 0,
+                                                    (int)(sizeof(T) * 8), st));
+    hipLaunchKernelGGL(gather_u32, dim3(nb), dim3(256), 0, st, lab, ordA, ne, k32a);
+    b = sb;
+    XS_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, b, k32a, k32b, ordA, ordB, (int)ne, 0,
+                                                    bits_for(gx), st));
+    hipLaunchKernelGGL(eq_columns, dim3(nb), dim3(256), 0, st, k32b, ordB, ne, gx, gy, lab);
+    b = sb;
+    XS_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, b, ez, kout, iota, ordA, (int)ne, 0,
+                                                    (int)(sizeof(T) * 8), st));
+    hipLaunchKernelGGL(gather_u32, dim3(nb), dim3(256), 0, st, lab, ordA, ne, k32a);
+    b = sb;
+    XS_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, b, k32a, k32b, ordA, ordB, (int)ne, 0,
+                                                    bits_for((uint64_t)gx * gy), st));
+    hipLaunchKernelGGL((gather_xyz<T>), dim3(nb), dim3(256), 0, st, ex, ey, ez, ordB, ne,
+                       sxp + eoff[e], syp + eoff[e], szp + eoff[e]);
+    XS_HIP_CHECK(hipGetLastError());
+    equal_count_tiles(eoff[e], ne, gx, gy, nz, tiles);
+    en[e] = (uint32_t)tiles.size() - ebase[e];
+  }
+  return XS_OK;
+}
+
+
 template <class T>
 static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_count,
                       unsigned n_el, double bin, unsigned nhist, unsigned long long *hist_d,
-                      unsigned long long *ovf_user) {
+                      unsigned long long *ovf_user, bool equal_count = true) {
   const unsigned ntypes = ntypes_of(n_el);
   const size_t hsize = (size_t)ntypes * nhist;
   const uint32_t wcap = wcap_for(nhist);
@@ -807,19 +996,26 @@ static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_co
   }
   XS_TRY(check_span(bbox, bin));
 
-  // 2. workspace: sorted coordinates, cell keys, overflow counter
+  // 2. workspace: sorted coordinates, cell keys, overflow counter, scratch
+  //    of the equal-count sorts
+  uint32_t maxne = 0;
+  for (unsigned e = 0; e < n_el; ++e) maxne = std::max<uint32_t>(maxne, el_count[e]);
+  size_t sortb = 0;
+  const size_t eqb = equal_count ? eq_scratch_bytes<T>(std::max(maxne, 1u), &sortb) : 0;
   XS_TRY(ensure_workspace(ctx, Carver::need({3 * n * sizeof(T), n * sizeof(uint32_t),
-                                             sizeof(unsigned long long)})));
+                                             sizeof(unsigned long long), eqb})));
   Carver cv(ctx->ws);
   T *sorted = cv.take<T>(3 * n);
   uint32_t *keys = cv.take<uint32_t>(n);
   unsigned long long *ovf_own = cv.take<unsigned long long>(1);
+  void *eq_scratch = cv.take<char>(eqb);
   unsigned long long *ovf_d = ovf_user ? ovf_user : ovf_own;
   T *sxp = sorted, *syp = sorted + n, *szp = sorted + 2 * n;
 
-  // 3. per element: cell grid sized for full tiles (a second count when the
-  //    fullest cell is far from kFill); counts read back, tiles on the host.
-  //    The counts go through the table buffer, which holds nothing else yet.
+  // 3. tiles.  Equal-count tiles (stable sorts on the device; the tiles
+  //    follow from the counts alone) or, as the fallback, a cell grid per
+  //    element sized for full tiles (counts read back, tiles on the host;
+  //    the counts go through the table buffer, which holds nothing else yet).
   hipEvent_t ev;
   XS_TRY(prof_begin(ctx, "hist_prep", &ev));
   const double cap = cell_cap(bin, nhist, wcap);
@@ -827,7 +1023,10 @@ static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_co
   std::vector<uint32_t> m_of(n_el), soff(n_el + 1, 0u), start_h;
   std::vector<Tile> tiles;
   uint32_t ebase[kMaxEl], en[kMaxEl];
-  for (unsigned e = 0; e < n_el; ++e) {
+  if (equal_count)
+    XS_TRY(eq_tiles_device<T>(ctx, xyz, n, el_count, n_el, eoff, bbox, cap, eq_scratch,
+                              std::max(maxne, 1u), sxp, syp, szp, tiles, ebase, en));
+  for (unsigned e = 0; e < n_el && !equal_count; ++e) {
     const uint32_t ne = el_count[e];
     double cell = first_cell(bbox, ne, cap);
     for (int pass = 0;; ++pass) {
@@ -889,7 +1088,7 @@ static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_co
   const unsigned zb = (unsigned)std::min<size_t>((hsize + 255) / 256, 4096);
   hipLaunchKernelGGL(fill_u64, dim3(std::max(zb, 1u)), dim3(256), 0, ctx->stream, hist_d, hsize);
   hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, ctx->stream, ovf_d, (size_t)1);
-  for (unsigned e = 0; e < n_el; ++e)
+  for (unsigned e = 0; e < n_el && !equal_count; ++e)
     if (el_count[e])
       hipLaunchKernelGGL((hist_scatter<T>), dim3((el_count[e] + 255) / 256), dim3(256), 0, ctx->stream,
                          xyz + eoff[e], xyz + n + eoff[e], xyz + 2 * n + eoff[e], el_count[e],
@@ -898,6 +1097,21 @@ static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_co
     hipLaunchKernelGGL((hist_tile_boxes<T>), dim3((unsigned)tiles.size()), dim3(256), 0, ctx->stream,
                        sxp, syp, szp, tiles_d, boxes);
   XS_HIP_CHECK(hipGetLastError());
+  if (equal_count && std::isfinite(cap) && !tiles.empty()) {
+    // equal-count tiles of a structure far from uniform density can be wider
+    // than the window allows: then the cell grid instead
+    std::vector<double> bx(tiles.size() * 6);
+    XS_HIP_CHECK(hipMemcpyAsync(bx.data(), boxes, bb, hipMemcpyDeviceToHost, ctx->stream));
+    XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    bool wide = false;
+    for (size_t t = 0; t < tiles.size() && !wide; ++t)
+      for (int a = 0; a < 3; ++a)
+        if (bx[6 * t + 3 + a] - bx[6 * t + a] > cap) wide = true;
+    if (wide) {
+      XS_TRY(prof_end(ctx, "hist_prep", ev));
+      return run_device<T>(ctx, xyz, n, el_count, n_el, bin, nhist, hist_d, ovf_user, false);
+    }
+  }
   XS_TRY(prof_end(ctx, "hist_prep", ev));
   Args<T> A{sxp, syp, szp, tiles_d, boxes, types_d, items_d, hist_d, ovf_d};
   return launch_hist<T>(ctx, A, items.size(), wcap, true);

@@ -138,3 +138,30 @@ def test_rocksalt_million_properties(ctx):
     r32, r64 = both_modes(ctx, sub, 0.005, nhist=nh)
     assert_same(r32, "c2 subset f32")
     assert_same(r64, "c2 subset f64")
+
+
+@pytest.mark.parametrize("shape,n,n_el,bin_", [("fcc", 30000, 3, 0.01), ("core-shell", 20000, 2, 0.005),
+                                              ("cube", 25000, 1, 0.01), ("bcc", 12000, 2, 0.002)])
+def test_device_path_equal_count_tiles(ctx, shape, n, n_el, bin_):
+    """Device-pointer path (equal-count tiles by stable radix sorts, or the
+    cell grid when a tile would exceed the window) equals the oracle, FP64
+    and FP32 binning; the bcc case has more bins than the largest window."""
+    import torch
+    rng = np.random.Generator(np.random.PCG64(n))
+    ens = st.particle(n, n_el, shape, 3.9 if shape != "bcc" else 3.1, 0.05, rng)
+    x64 = ens.xyz64()
+    nh = O.hist_size(x64, bin_)
+    dev = torch.device("cuda", 0)
+    P = ens.n_el * (ens.n_el + 1) // 2
+    for f64 in (True, False):
+        xs = x64 if f64 else ens.xyz
+        x = torch.from_numpy(np.ascontiguousarray(xs)).to(dev)
+        h = torch.empty(P * nh, dtype=torch.int64, device=dev)
+        ov = torch.empty(1, dtype=torch.int64, device=dev)
+        ctx.hist_dev(x.data_ptr(), ens.n, ens.el_count, bin_, nh, h.data_ptr(), ov.data_ptr(),
+                     f64=f64)
+        ctx.synchronize()
+        ho, oo = (O.hist_f64(x64, ens.el_count, bin_, nh) if f64 else
+                  O.hist_f32(ens.xyz, ens.el_count, bin_, nh))
+        assert np.array_equal(h.cpu().numpy().astype(np.uint64).reshape(ho.shape), ho), (shape, f64)
+        assert int(ov.item()) == oo
