@@ -1,10 +1,11 @@
 // H1: pair-distance histograms on gfx950, for one structure or an ensemble.
 //
-// Tiling.  Each structure's atoms are counting-sorted by the cell of a uniform
-// grid of cubic cells sized for ~kFill atoms of its most numerous element (and
-// no wider than two cell diagonals fit the LDS window).  A tile is one cell,
-// or a 1024-atom chunk of a fuller cell, so its atoms lie in a cube; a tile
-// carries its bounding box.  Counts do not depend on atom order, so the
+// Tiling.  Each element's atoms are split into equal-count tiles of kTile
+// atoms (all full but one), compact in space and no wider than two tile
+// diagonals fit the LDS window: k-d splits on the host (kd_tiles), x-slab /
+// y-column / z-run radix sorts on the device (eq_tiles_device), or, when
+// those tiles are too wide for the window, the cells of a uniform grid.  A
+// tile carries its bounding box.  Counts do not depend on atom order, so the
 // permutation leaves the histogram unchanged.
 //
 // Kernel.  One work item = (pair type of a structure, I tile, run of J tiles).
@@ -15,6 +16,8 @@
 // WCAP uint32 bins that only slides up; bins the remaining J tiles cannot
 // reach are flushed to the uint64 histogram with one atomic per non-zero bin,
 // so flush traffic per item is at most nhist + WCAP whatever the pair count.
+// The window is rotated (rarely) so that a J tile's bins never straddle the
+// end of the slot array; the tile's pairs then count at slot k + const.
 // A tile pair whose span exceeds the window takes global atomics per pair.
 //
 // Binning.  The reference's (unsigned)(r / bin), r = sqrt((dx*dx+dy*dy)+dz*dz):
@@ -260,9 +263,11 @@ __global__ void gather_xyz(const T *x, const T *y, const T *z, const uint32_t *o
 
 // ---------------------------------------------------------------- histogram kernel
 
-// Conservative bin range [kl, ku] of all pairs between two boxes.
-// Conservative bin range [kl, ku] of all pairs between two boxes (one bin of
-// margin each side).  fits: no pair can reach nhist (ku was not clamped).
+// Conservative bin range [kl, ku] of all pairs between two boxes: the
+// distance bounds scaled by 1 -+ 2^-20 (the FP32 binning's distance errs by
+// less than 5 2^-24 relative) and one bin of margin each side.  Every pair's
+// bin lies in [kl, ku], which the unchecked window slots rely on.
+// fits: no pair can reach nhist (ku was not clamped).
 __device__ __forceinline__ void box_bins(const double *__restrict__ bi, const double *__restrict__ bj,
                                          double bin, uint32_t nhist, uint32_t &kl, uint32_t &ku,
                                          bool &fits) {
@@ -274,8 +279,8 @@ __device__ __forceinline__ void box_bins(const double *__restrict__ bi, const do
     hi2 += ext * ext;
   }
   const double top = (double)(nhist - 1);
-  double fl = floor(sqrt(lo2) / bin) - 1.0;
-  double fh = floor(sqrt(hi2) / bin) + 1.0;
+  double fl = floor(sqrt(lo2) / bin * (1.0 - 0x1p-20)) - 1.0;
+  double fh = floor(sqrt(hi2) / bin * (1.0 + 0x1p-20)) + 1.0;
   fits = fh <= top;
   fl = fl < 0.0 ? 0.0 : (fl > top ? top : fl);
   fh = fh > top ? top : (fh < fl ? fl : fh);
@@ -283,17 +288,35 @@ __device__ __forceinline__ void box_bins(const double *__restrict__ bi, const do
   ku = (uint32_t)fh;
 }
 
+// The window holds bins [B, B + WCAP); bin k sits in slot (k + D) mod WCAP.
+// D changes only when the window is rotated (rotate_window).
 template <int WCAP>
-__device__ __forceinline__ void flush_window(uint32_t *win, uint32_t lo, uint32_t hi,
+__device__ __forceinline__ void flush_window(uint32_t *win, uint32_t lo, uint32_t hi, uint32_t D,
                                              unsigned long long *__restrict__ hp) {
   for (uint32_t k = lo + threadIdx.x; k < hi; k += kTile) {
-    const uint32_t slot = k & (WCAP - 1);
+    const uint32_t slot = (k + D) & (WCAP - 1);
     const uint32_t v = win[slot];
     if (v) {
       atomicAdd(&hp[k], (unsigned long long)v);
       win[slot] = 0u;
     }
   }
+}
+
+// Rotates the window so that bin B sits in slot 0 (the new D is -B).  Called
+// by the whole work group when a J tile's bins would straddle the end of the
+// slot array, so that the tile's slots are k + const without a wrap.
+template <int WCAP>
+__device__ __forceinline__ void rotate_window(uint32_t *win, uint32_t B, uint32_t D) {
+  constexpr int kPer = WCAP / kTile;
+  uint32_t v[kPer];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < kPer; ++c) v[c] = win[(B + D + threadIdx.x + c * kTile) & (WCAP - 1)];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < kPer; ++c) win[threadIdx.x + c * kTile] = v[c];
+  __syncthreads();
 }
 
 // Bin index of a squared distance s.
@@ -379,19 +402,57 @@ __device__ __forceinline__ uint32_t exact_bin<float>(float s, float bin, uint32_
   return t < (float)nhist ? (uint32_t)t : nhist;
 }
 
+// FP64 fast path of one I atom: its coordinates relative to a centre c of
+// its tile times -2, and its squared distance from c.  With the J atoms'
+// coordinates relative to c (t) and their squared norms (tn) staged in LDS,
+// |ri - rj|^2 = n + tn + (-2 ri').rj' costs one add and three FMAs.
+struct IAtom {
+  double ax, ay, az, n;
+};
+
+// Half-width of the estimate's band for the FP64 fast path of one I tile
+// (box bi, 6 doubles) whose coordinates are taken relative to the box centre.
+// Relative to IEEE sqrt(s)/bin the estimate errs by (a) below nhist 2^-46
+// bins from v_rsq_f64, the Newton step and the roundings of the reference
+// expression, and (b) below u (64 R^2 + 36 R + 10 T) bins from forming s as
+// |ri'|^2 + |rj'|^2 - 2 ri'.rj' with shifted, rounded coordinates, where
+// u = 2^-53, R bounds |ri - c| in bins and T = distance in bins <= nhist + 1
+// where it matters: the rounding error of s is below
+// u (32 rho^2 + 36 rho d + 10 d^2) for |ri - c| <= rho, |rj - c| <= rho + d,
+// and divides by d to give the error of the root for d >= bin / 2 (below
+// that, both floors are 0).  Both bounds are taken 16-fold.
+__device__ __forceinline__ double tile_band(const double *bi, double band0, double inv_bin,
+                                            uint32_t nhist, double c[3]) {
+  double h2 = 0.0, cm = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    c[a] = 0.5 * (bi[a] + bi[3 + a]);
+    const double e = bi[3 + a] - bi[a];
+    h2 = fma(e, e, h2);
+    cm = fmax(cm, fabs(c[a]));
+  }
+  const double R = (0.5 * sqrt(h2) * (1.0 + 0x1p-20) + 0x1p-40 * cm) * inv_bin;
+  return band0 - 0x1p-49 * (fma(64.0 * R, R, 36.0 * R) + 10.0 * ((double)nhist + 1.0));
+}
+
 // Groups of 8 J atoms (not the like-element diagonal tile): 8 independent
 // distance chains and no per-pair branch on the common path.
-//  * FP64: the estimate takes s with fused multiply-adds (it only has to lie
-//    within the band of the IEEE value); a pair whose estimate is too close
-//    to a bin edge recomputes s and its bin exactly as the reference does.
+//  * FP64: the estimate takes s = |ri'|^2 + |rj'|^2 - 2 ri'.rj' from the
+//    tile-centred coordinates (it only has to lie within the band of the
+//    IEEE value); a pair whose estimate is too close to a bin edge, or whose
+//    s is not positive, recomputes s from the original coordinates and its
+//    bin exactly as the reference does.
 //  * SAFE: the J tile's bin range lies inside the window and below nhist
-//    (kl >= B, ku < B + WCAP, ku < nhist from the tile boxes), so every pair
-//    goes to the window unchecked; otherwise each pair is tested.
+//    (kl >= B, ku < B + WCAP, ku < nhist from the tile boxes) and maps to
+//    slots without a wrap, so every pair goes to slot k + off unchecked;
+//    otherwise each pair is tested.
 template <class T, int WCAP, bool SAFE>
-__device__ __forceinline__ void pair_group8(T xi, T yi, T zi, const T *sx, const T *sy,
-                                            const T *sz, uint32_t ng, T bin, double inv_bin,
-                                            double band, uint32_t nhist, uint32_t B, uint32_t wlim,
-                                            uint32_t *win, unsigned long long *__restrict__ hp,
+__device__ __forceinline__ void pair_group8(T xi, T yi, T zi, const IAtom &ia, const T *sx,
+                                            const T *sy, const T *sz, const double *tx,
+                                            const double *ty, const double *tz, const double *tn,
+                                            uint32_t ng, T bin, double inv_bin, double band,
+                                            uint32_t nhist, uint32_t B, uint32_t wlim, uint32_t D,
+                                            uint32_t off, uint32_t *win,
+                                            unsigned long long *__restrict__ hp,
                                             unsigned long long *__restrict__ ovf) {
 #pragma clang fp contract(off)
   constexpr bool F64 = sizeof(T) == 8;
@@ -400,17 +461,20 @@ __device__ __forceinline__ void pair_group8(T xi, T yi, T zi, const T *sx, const
     bool bad[8], any = false;
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
-      const T dx = xi - sx[j0 + p];
-      const T dy = yi - sy[j0 + p];
-      const T dz = zi - sz[j0 + p];
-      T s;
       if constexpr (F64) {
-        s = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+        double s = ia.n + tn[j0 + p];
+        s = __builtin_fma(ia.az, tz[j0 + p], s);
+        s = __builtin_fma(ia.ay, ty[j0 + p], s);
+        s = __builtin_fma(ia.ax, tx[j0 + p], s);
+        k[p] = fast_bin<T>(s, bin, inv_bin, band, nhist, bad[p]);
       } else {
-        s = dx * dx + dy * dy;
+        const T dx = xi - sx[j0 + p];
+        const T dy = yi - sy[j0 + p];
+        const T dz = zi - sz[j0 + p];
+        T s = dx * dx + dy * dy;
         s = s + dz * dz;
+        k[p] = fast_bin<T>(s, bin, inv_bin, band, nhist, bad[p]);
       }
-      k[p] = fast_bin<T>(s, bin, inv_bin, band, nhist, bad[p]);
       any |= bad[p];
     }
     if constexpr (F64) {
@@ -430,12 +494,12 @@ __device__ __forceinline__ void pair_group8(T xi, T yi, T zi, const T *sx, const
     }
     if constexpr (SAFE) {
 #pragma unroll
-      for (int p = 0; p < 8; ++p) atomicAdd(&win[k[p] & (WCAP - 1)], 1u);
+      for (int p = 0; p < 8; ++p) atomicAdd(&win[k[p] + off], 1u);
     } else {
 #pragma unroll
       for (int p = 0; p < 8; ++p) {
         if (k[p] - B < wlim)
-          atomicAdd(&win[k[p] & (WCAP - 1)], 1u);
+          atomicAdd(&win[(k[p] + D) & (WCAP - 1)], 1u);
         else if (k[p] < nhist)
           atomicAdd(&hp[k[p]], 1ull);
         else
@@ -449,7 +513,7 @@ template <class T, int WCAP, bool DIAG>
 __device__ __forceinline__ void pair_block(T xi, T yi, T zi, uint32_t il, const T *sx,
                                            const T *sy, const T *sz, uint32_t cn,
                                            uint32_t jbase, T bin, double inv_bin, double band,
-                                           uint32_t nhist, uint32_t B, uint32_t wlim,
+                                           uint32_t nhist, uint32_t B, uint32_t wlim, uint32_t D,
                                            uint32_t *win, unsigned long long *__restrict__ hp,
                                            unsigned long long *__restrict__ ovf) {
 #pragma clang fp contract(off)
@@ -463,7 +527,7 @@ __device__ __forceinline__ void pair_block(T xi, T yi, T zi, uint32_t il, const 
     s = s + dz * dz;
     const uint32_t k = bin_index<T>(s, bin, inv_bin, band, nhist);
     if (__builtin_expect(k - B < wlim, 1)) {
-      atomicAdd(&win[k & (WCAP - 1)], 1u);
+      atomicAdd(&win[(k + D) & (WCAP - 1)], 1u);
     } else if (k < nhist) {
       atomicAdd(&hp[k], 1ull);
     } else {
@@ -474,9 +538,12 @@ __device__ __forceinline__ void pair_block(T xi, T yi, T zi, uint32_t il, const 
 
 template <class T, int WCAP>
 __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
+  constexpr bool F64 = sizeof(T) == 8;
+  constexpr int kShift = F64 ? kStage : 1;
   __shared__ uint32_t win[WCAP];
   __shared__ unsigned long long keys[kSortCap];
   __shared__ T sx[kStage], sy[kStage], sz[kStage];
+  __shared__ double tx[kShift], ty[kShift], tz[kShift], tn[kShift];
 
   const Item it = A.items[blockIdx.x];
   const TypeDesc td = A.types[it.type];
@@ -494,6 +561,15 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
   const T bin = (T)td.bin;
   const uint32_t nhist = td.nhist;
   const double *bi = A.boxes + 6 * (size_t)tiI;
+  // FP64 fast path: coordinates relative to the I tile's box centre
+  double c[3] = {0.0, 0.0, 0.0};
+  double band = td.band;
+  IAtom ia{0.0, 0.0, 0.0, 0.0};
+  if constexpr (F64) {
+    band = tile_band(bi, td.band, td.inv_bin, nhist, c);
+    const double u = (double)xi - c[0], v = (double)yi - c[1], w = (double)zi - c[2];
+    ia = IAtom{-2.0 * u, -2.0 * v, -2.0 * w, fma(w, w, fma(v, v, u * u))};
+  }
   const uint32_t n = it.jcount;
   uint32_t npow = 1;
   while (npow < n) npow <<= 1;
@@ -531,6 +607,7 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
   }
 
   uint32_t B = (uint32_t)(keys[0] >> 32);
+  uint32_t D = 0u;  // bin k is counted in slot (k + D) mod WCAP
   for (uint32_t s = 0; s < n; ++s) {
     const uint32_t m = (uint32_t)(keys[s] & 0xffffffffu);
     const uint32_t tj = (it.jstart + m) % td.b_n;
@@ -543,42 +620,61 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
     if (span_ok) {
       const uint32_t nb = ku + 1 > (uint32_t)WCAP ? ku + 1 - WCAP : 0u;
       if (nb > B) {
-        flush_window<WCAP>(win, B, min(min(nb, B + (uint32_t)WCAP), nhist), hp);
+        flush_window<WCAP>(win, B, min(min(nb, B + (uint32_t)WCAP), nhist), D, hp);
         B = nb;
         __syncthreads();
       }
       wlim = min((uint32_t)WCAP, nhist - B);
+    }
+    // unchecked slots k + off for the tile's bins [kl, ku] (inside [B, B + WCAP))
+    const bool safe = span_ok && fits;
+    uint32_t off = 0u;
+    if (safe) {
+      if (((kl + D) ^ (ku + D)) >= (uint32_t)WCAP) {
+        rotate_window<WCAP>(win, B, D);
+        D = 0u - B;
+      }
+      off = D - ((kl + D) & ~(uint32_t)(WCAP - 1));
     }
     const Tile J = A.tiles[tjg];
     const bool diag = td.same && tj == it.ti;
     for (uint32_t s0 = 0; s0 < J.cnt; s0 += kStage) {
       const uint32_t cn = min((uint32_t)kStage, J.cnt - s0);
       if (tid < cn) {
-        sx[tid] = A.x[(size_t)J.off + s0 + tid];
-        sy[tid] = A.y[(size_t)J.off + s0 + tid];
-        sz[tid] = A.z[(size_t)J.off + s0 + tid];
+        const size_t g = (size_t)J.off + s0 + tid;
+        const T x = A.x[g], y = A.y[g], z = A.z[g];
+        sx[tid] = x;
+        sy[tid] = y;
+        sz[tid] = z;
+        if constexpr (F64) {
+          const double u = x - c[0], v = y - c[1], w = z - c[2];
+          tx[tid] = u;
+          ty[tid] = v;
+          tz[tid] = w;
+          tn[tid] = fma(w, w, fma(v, v, u * u));
+        }
       }
       __syncthreads();
       if (ivalid) {
         if (diag) {
           pair_block<T, WCAP, true>(xi, yi, zi, tid, sx, sy, sz, cn, s0, bin, td.inv_bin, td.band,
-                                    nhist, B, wlim, win, hp, ovf);
+                                    nhist, B, wlim, D, win, hp, ovf);
         } else {
           const uint32_t ng = cn & ~7u;
-          if (span_ok && fits)
-            pair_group8<T, WCAP, true>(xi, yi, zi, sx, sy, sz, ng, bin, td.inv_bin, td.band, nhist, B,
-                                       wlim, win, hp, ovf);
+          if (safe)
+            pair_group8<T, WCAP, true>(xi, yi, zi, ia, sx, sy, sz, tx, ty, tz, tn, ng, bin,
+                                       td.inv_bin, band, nhist, B, wlim, D, off, win, hp, ovf);
           else
-            pair_group8<T, WCAP, false>(xi, yi, zi, sx, sy, sz, ng, bin, td.inv_bin, td.band, nhist, B,
-                                        wlim, win, hp, ovf);
+            pair_group8<T, WCAP, false>(xi, yi, zi, ia, sx, sy, sz, tx, ty, tz, tn, ng, bin,
+                                        td.inv_bin, band, nhist, B, wlim, D, off, win, hp, ovf);
           pair_block<T, WCAP, false>(xi, yi, zi, tid, sx + ng, sy + ng, sz + ng, cn - ng, s0 + ng, bin,
-                                     td.inv_bin, td.band, nhist, B, wlim, win, hp, ovf);
+                                     td.inv_bin, td.band, nhist, B, wlim, D, win, hp, ovf);
         }
       }
       __syncthreads();
     }
   }
-  flush_window<WCAP>(win, B, min(B + (uint32_t)WCAP, nhist), hp);
+  flush_window<WCAP>(win, B, min(B + (uint32_t)WCAP, nhist), D, hp);
 }
 
 // ---------------------------------------------------------------- host planning

@@ -165,3 +165,49 @@ def test_device_path_equal_count_tiles(ctx, shape, n, n_el, bin_):
                   O.hist_f32(ens.xyz, ens.el_count, bin_, nh))
         assert np.array_equal(h.cpu().numpy().astype(np.uint64).reshape(ho.shape), ho), (shape, f64)
         assert int(ov.item()) == oo
+
+
+def _hist64_host_and_device(ctx, x64, el_count, bin_, nh):
+    import torch
+    host, ov_h = ctx.hist(x64, el_count, bin_, nh, f64=True)
+    dev = torch.device("cuda", 0)
+    x = torch.from_numpy(np.ascontiguousarray(x64)).to(dev)
+    P = len(el_count) * (len(el_count) + 1) // 2
+    h = torch.empty(P * nh, dtype=torch.int64, device=dev)
+    ov = torch.empty(1, dtype=torch.int64, device=dev)
+    ctx.hist_dev(x.data_ptr(), x64.shape[1], el_count, bin_, nh, h.data_ptr(), ov.data_ptr(), f64=True)
+    ctx.synchronize()
+    return host, ov_h, h.cpu().numpy().astype(np.uint64).reshape(host.shape), int(ov.item())
+
+
+@pytest.mark.parametrize("case", ["far_from_origin", "sparse_tiles", "shifted_integer_lattice"])
+def test_fp64_tile_centred_distances(ctx, case):
+    """The FP64 fast path forms s = |ri'|^2 + |rj'|^2 - 2 ri'.rj' from
+    coordinates relative to the I tile's centre; its band must send every
+    pair whose estimate could fall in another bin to the exact evaluation.
+    Cases: FP64 coordinates 10^4 A from the origin (rounding of ri - c),
+    a sparse cluster whose tiles span the whole structure (large tile
+    radius), and an integer lattice shifted by 1000.25 A (exact bin edges
+    with rounded, shifted coordinates).  Host and device paths, bit-exact."""
+    rng = np.random.Generator(np.random.PCG64(2718))
+    if case == "far_from_origin":
+        ens = st.fcc_sphere(radius=24.0, seed=5)
+        x64 = ens.xyz64() + rng.normal(0, 0.05, ens.xyz.shape) + np.array([[1.0e4], [-3.1e4], [5.3e3]])
+        el, bin_ = ens.el_count, 0.01
+    elif case == "sparse_tiles":
+        x64 = rng.uniform(-40.0, 40.0, (3, 2500))
+        el, bin_ = [1300, 1200], 0.02
+    else:
+        r = np.arange(11, dtype=np.float64)
+        g = np.stack(np.meshgrid(r, r, r, indexing="ij"), axis=0).reshape(3, -1)
+        x64 = g + 1000.25
+        el, bin_ = [g.shape[1] // 2, g.shape[1] - g.shape[1] // 2], 0.25
+    x64 = np.ascontiguousarray(x64)
+    nh = O.hist_size(x64, bin_)
+    want, oo = O.hist_f64(x64, el, bin_, nh)
+    host, ov_h, devh, ov_d = _hist64_host_and_device(ctx, x64, el, bin_, nh)
+    print(f"{case}: nhist={nh} pairs={int(want.sum())} host mismatched={int(np.sum(host != want))} "
+          f"device mismatched={int(np.sum(devh != want))}", flush=True)
+    assert ov_h == oo and ov_d == oo
+    assert np.array_equal(host, want)
+    assert np.array_equal(devh, want)
