@@ -677,6 +677,258 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
   flush_window<WCAP>(win, B, min(B + (uint32_t)WCAP, nhist), D, hp);
 }
 
+// FP64 groups of 4 J atoms against the thread's two I atoms a and b (8
+// independent chains; each J record read from LDS serves two pairs).  Counts
+// of an I atom that does not exist (va / vb false) are dropped.
+template <int WCAP, bool SAFE>
+__device__ __forceinline__ void pair_group4x2(const IAtom &ia, const IAtom &ib, bool va, bool vb,
+                                              double xa, double ya, double za, double xb,
+                                              double yb, double zb, const double *sx,
+                                              const double *sy, const double *sz, const double *tx,
+                                              const double *ty, const double *tz, const double *tn,
+                                              uint32_t ng, double bin, double inv_bin, double band,
+                                              uint32_t nhist, uint32_t B, uint32_t wlim, uint32_t D,
+                                              uint32_t off, uint32_t *win,
+                                              unsigned long long *__restrict__ hp,
+                                              unsigned long long *__restrict__ ovf) {
+#pragma clang fp contract(off)
+  for (uint32_t j0 = 0; j0 < ng; j0 += 4) {
+    uint32_t k[8];
+    bool bad[8], any = false;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      const double rx = tx[j0 + p], ry = ty[j0 + p], rz = tz[j0 + p], rn = tn[j0 + p];
+      double s = ia.n + rn;
+      s = __builtin_fma(ia.az, rz, s);
+      s = __builtin_fma(ia.ay, ry, s);
+      s = __builtin_fma(ia.ax, rx, s);
+      k[p] = fast_bin<double>(s, bin, inv_bin, band, nhist, bad[p]);
+      double t = ib.n + rn;
+      t = __builtin_fma(ib.az, rz, t);
+      t = __builtin_fma(ib.ay, ry, t);
+      t = __builtin_fma(ib.ax, rx, t);
+      k[4 + p] = fast_bin<double>(t, bin, inv_bin, band, nhist, bad[4 + p]);
+      any |= bad[p] | bad[4 + p];
+    }
+    if (__builtin_expect(any, 0)) {
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        if (bad[p]) {
+          const double xi = p < 4 ? xa : xb, yi = p < 4 ? ya : yb, zi = p < 4 ? za : zb;
+          const double dx = xi - sx[j0 + (p & 3)];
+          const double dy = yi - sy[j0 + (p & 3)];
+          const double dz = zi - sz[j0 + (p & 3)];
+          double s = dx * dx + dy * dy;
+          s = s + dz * dz;
+          k[p] = exact_bin<double>(s, bin, nhist);
+        }
+      }
+    }
+    if constexpr (SAFE) {
+      if (va) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) atomicAdd(&win[k[p] + off], 1u);
+      }
+      if (vb) {
+#pragma unroll
+        for (int p = 4; p < 8; ++p) atomicAdd(&win[k[p] + off], 1u);
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < 8; ++p) {
+        if (!(p < 4 ? va : vb)) continue;
+        if (k[p] - B < wlim)
+          atomicAdd(&win[(k[p] + D) & (WCAP - 1)], 1u);
+        else if (k[p] < nhist)
+          atomicAdd(&hp[k[p]], 1ull);
+        else
+          atomicAdd(ovf, 1ull);
+      }
+    }
+  }
+}
+
+// FP64 binning: one work item = (pair type, I tile pair {2 ti, 2 ti + 1},
+// run of J tiles), two I atoms per thread (atom tid of each tile).  The J
+// tiles are sorted and the window managed for the union box of the two I
+// tiles; J records are taken relative to that box's centre.  Like-element
+// types visit tile pairs by a half rotation over I tile pairs (add_items), so
+// within the item's own tile pair the J tile 2 ti pairs with I tile 2 ti only
+// (j > i) and J tile 2 ti + 1 with I tile 2 ti fully and with 2 ti + 1 for
+// j > i; those two J tiles take the per-pair loop.
+template <int WCAP>
+__global__ __launch_bounds__(kTile) void hist_kernel_pair(Args<double> A) {
+  __shared__ uint32_t win[WCAP];
+  __shared__ unsigned long long keys[kSortCap];
+  __shared__ double sx[kStage], sy[kStage], sz[kStage];
+  __shared__ double tx[kStage], ty[kStage], tz[kStage], tn[kStage];
+
+  const Item it = A.items[blockIdx.x];
+  const TypeDesc td = A.types[it.type];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t tA = 2 * it.ti, tB = tA + 1;
+  const bool hasB = tB < td.a_n;
+  const Tile IA = A.tiles[td.a_base + tA];
+  const Tile IB = hasB ? A.tiles[td.a_base + tB] : Tile{0u, 0u};
+  const bool va = tid < IA.cnt, vb = tid < IB.cnt;
+  double xa = 0, ya = 0, za = 0, xb = 0, yb = 0, zb = 0;
+  if (va) {
+    const size_t g = (size_t)IA.off + tid;
+    xa = A.x[g];
+    ya = A.y[g];
+    za = A.z[g];
+  }
+  if (vb) {
+    const size_t g = (size_t)IB.off + tid;
+    xb = A.x[g];
+    yb = A.y[g];
+    zb = A.z[g];
+  }
+  const uint32_t nhist = td.nhist;
+  const double bin = td.bin;
+  // union box of the two I tiles
+  double bu[6];
+  {
+    const double *b0 = A.boxes + 6 * (size_t)(td.a_base + tA);
+    for (int a = 0; a < 6; ++a) bu[a] = b0[a];
+    if (hasB) {
+      const double *b1 = A.boxes + 6 * (size_t)(td.a_base + tB);
+      for (int a = 0; a < 3; ++a) {
+        bu[a] = fmin(bu[a], b1[a]);
+        bu[3 + a] = fmax(bu[3 + a], b1[3 + a]);
+      }
+    }
+  }
+  double c[3];
+  const double band = tile_band(bu, td.band, td.inv_bin, nhist, c);
+  IAtom ia, ib;
+  {
+    double u = xa - c[0], v = ya - c[1], w = za - c[2];
+    ia = IAtom{-2.0 * u, -2.0 * v, -2.0 * w, fma(w, w, fma(v, v, u * u))};
+    u = xb - c[0];
+    v = yb - c[1];
+    w = zb - c[2];
+    ib = IAtom{-2.0 * u, -2.0 * v, -2.0 * w, fma(w, w, fma(v, v, u * u))};
+  }
+  const uint32_t n = it.jcount;
+  uint32_t npow = 1;
+  while (npow < n) npow <<= 1;
+  unsigned long long *hp = A.hist + td.hist_off;
+  unsigned long long *ovf = A.overflow + td.ovf;
+
+  for (uint32_t m = tid; m < npow; m += kTile) {
+    unsigned long long key = ~0ull;
+    if (m < n) {
+      const uint32_t tj = (it.jstart + m) % td.b_n;
+      uint32_t kl, ku;
+      bool fits;
+      box_bins(bu, A.boxes + 6 * (size_t)(td.b_base + tj), bin, nhist, kl, ku, fits);
+      key = ((unsigned long long)kl << 32) | m;
+    }
+    keys[m] = key;
+  }
+  for (uint32_t w = tid; w < WCAP; w += kTile) win[w] = 0u;
+  __syncthreads();
+  for (uint32_t k = 2; k <= npow; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = tid; i < npow; i += kTile) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const unsigned long long u = keys[i], v = keys[ixj];
+          const bool up = (i & k) == 0;
+          if ((u > v) == up) {
+            keys[i] = v;
+            keys[ixj] = u;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  uint32_t B = (uint32_t)(keys[0] >> 32);
+  uint32_t D = 0u;  // bin k is counted in slot (k + D) mod WCAP
+  for (uint32_t s = 0; s < n; ++s) {
+    const uint32_t m = (uint32_t)(keys[s] & 0xffffffffu);
+    const uint32_t tj = (it.jstart + m) % td.b_n;
+    const uint32_t tjg = td.b_base + tj;
+    uint32_t kl, ku;
+    bool fits;
+    box_bins(bu, A.boxes + 6 * (size_t)tjg, bin, nhist, kl, ku, fits);
+    uint32_t wlim = 0;
+    const bool span_ok = ku - kl + 1 <= (uint32_t)WCAP;
+    if (span_ok) {
+      const uint32_t nb = ku + 1 > (uint32_t)WCAP ? ku + 1 - WCAP : 0u;
+      if (nb > B) {
+        flush_window<WCAP>(win, B, min(min(nb, B + (uint32_t)WCAP), nhist), D, hp);
+        B = nb;
+        __syncthreads();
+      }
+      wlim = min((uint32_t)WCAP, nhist - B);
+    }
+    const bool safe = span_ok && fits;
+    uint32_t off = 0u;
+    if (safe) {
+      if (((kl + D) ^ (ku + D)) >= (uint32_t)WCAP) {
+        rotate_window<WCAP>(win, B, D);
+        D = 0u - B;
+      }
+      off = D - ((kl + D) & ~(uint32_t)(WCAP - 1));
+    }
+    const Tile J = A.tiles[tjg];
+    // like-element types: J tiles of the item's own tile pair
+    const bool jA = td.same && tj == tA, jB = td.same && hasB && tj == tB;
+    for (uint32_t s0 = 0; s0 < J.cnt; s0 += kStage) {
+      const uint32_t cn = min((uint32_t)kStage, J.cnt - s0);
+      if (tid < cn) {
+        const size_t g = (size_t)J.off + s0 + tid;
+        const double x = A.x[g], y = A.y[g], z = A.z[g];
+        sx[tid] = x;
+        sy[tid] = y;
+        sz[tid] = z;
+        const double u = x - c[0], v = y - c[1], w = z - c[2];
+        tx[tid] = u;
+        ty[tid] = v;
+        tz[tid] = w;
+        tn[tid] = fma(w, w, fma(v, v, u * u));
+      }
+      __syncthreads();
+      if (jA) {
+        if (va)
+          pair_block<double, WCAP, true>(xa, ya, za, tid, sx, sy, sz, cn, s0, bin, td.inv_bin,
+                                         td.band, nhist, B, wlim, D, win, hp, ovf);
+      } else if (jB) {
+        if (va)
+          pair_block<double, WCAP, false>(xa, ya, za, tid, sx, sy, sz, cn, s0, bin, td.inv_bin,
+                                          td.band, nhist, B, wlim, D, win, hp, ovf);
+        if (vb)
+          pair_block<double, WCAP, true>(xb, yb, zb, tid, sx, sy, sz, cn, s0, bin, td.inv_bin,
+                                         td.band, nhist, B, wlim, D, win, hp, ovf);
+      } else if (va || vb) {
+        const uint32_t ng = cn & ~3u;
+        if (safe)
+          pair_group4x2<WCAP, true>(ia, ib, va, vb, xa, ya, za, xb, yb, zb, sx, sy, sz, tx, ty, tz,
+                                    tn, ng, bin, td.inv_bin, band, nhist, B, wlim, D, off, win, hp,
+                                    ovf);
+        else
+          pair_group4x2<WCAP, false>(ia, ib, va, vb, xa, ya, za, xb, yb, zb, sx, sy, sz, tx, ty, tz,
+                                     tn, ng, bin, td.inv_bin, band, nhist, B, wlim, D, off, win, hp,
+                                     ovf);
+        if (va)
+          pair_block<double, WCAP, false>(xa, ya, za, tid, sx + ng, sy + ng, sz + ng, cn - ng,
+                                          s0 + ng, bin, td.inv_bin, td.band, nhist, B, wlim, D,
+                                          win, hp, ovf);
+        if (vb)
+          pair_block<double, WCAP, false>(xb, yb, zb, tid, sx + ng, sy + ng, sz + ng, cn - ng,
+                                          s0 + ng, bin, td.inv_bin, td.band, nhist, B, wlim, D,
+                                          win, hp, ovf);
+      }
+      __syncthreads();
+    }
+  }
+  flush_window<WCAP>(win, B, min(B + (uint32_t)WCAP, nhist), D, hp);
+}
+
 // ---------------------------------------------------------------- host planning
 
 // Cell sizing.  Lanes of a work group hold the atoms of one I tile, so a
@@ -756,44 +1008,63 @@ static int check_span(const double bbox[6], double bin) {
   return XS_OK;
 }
 
-// Work items of pair types [first, last) (appended).  Like-element pairs:
-// tile ti with itself and the next h tiles cyclically (h = (n-1)/2, or n/2 for
-// the first half when n is even), so each unordered tile pair is visited once
-// and items have equal length.  Cross-element pairs: each I tile against the J
-// tiles in two halves.
+// Work items of pair types [first, last) (appended).  The I side of an item
+// is one tile, or with `paired` (FP64 binning) the tile pair {2 ti, 2 ti + 1}
+// (an I unit).  Like-element pairs: I unit ti with itself and the next h
+// units cyclically (h = (n-1)/2, or n/2 for the first half when n is even,
+// n = units), so each unordered unit pair is visited once and items have
+// equal length; the J run is the tiles of those units.  Cross-element pairs:
+// each I unit against the J tiles in parts.  Paired items are split twice as
+// finely, so that there are as many items as unpaired ones.
 static void add_items(const std::vector<TypeDesc> &types, uint32_t first, uint32_t last,
-                      std::vector<Item> &items) {
+                      std::vector<Item> &items, bool paired) {
   for (uint32_t t = first; t < last; ++t) {
     const TypeDesc &d = types[t];
     if (!d.a_n || !d.b_n) continue;
     if (d.same) {
       const uint32_t nt = d.a_n;
-      for (uint32_t ti = 0; ti < nt; ++ti) {
-        const uint32_t h = (nt & 1) ? (nt - 1) / 2 : (ti < nt / 2 ? nt / 2 : nt / 2 - 1);
-        const uint32_t total = h + 1;
-        for (uint32_t c = 0; c < total; c += kSortCap)
-          items.push_back(Item{t, ti, (ti + c) % nt, std::min<uint32_t>(kSortCap, total - c)});
+      const uint32_t nu = paired ? (nt + 1) / 2 : nt;
+      for (uint32_t u = 0; u < nu; ++u) {
+        const uint32_t h = (nu & 1) ? (nu - 1) / 2 : (u < nu / 2 ? nu / 2 : nu / 2 - 1);
+        uint32_t start = u, total = h + 1;
+        if (paired) {
+          start = 2 * u;
+          total = 2 * (h + 1);
+          // the last unit holds one tile when nt is odd
+          if ((nt & 1) && (nu - 1 + nu - u) % nu <= h) --total;
+        }
+        const uint32_t chunk =
+            std::min<uint32_t>(paired ? (total + 1) / 2 : total, (uint32_t)kSortCap);
+        for (uint32_t c = 0; c < total; c += chunk)
+          items.push_back(Item{t, u, (start + c) % nt, std::min<uint32_t>(chunk, total - c)});
       }
     } else {
-      const uint32_t parts = d.b_n >= 2 ? 2 : 1;
+      const uint32_t nu = paired ? (d.a_n + 1) / 2 : d.a_n;
+      const uint32_t parts = std::min<uint32_t>(d.b_n, paired ? 4u : 2u);
       const uint32_t per = std::min<uint32_t>((d.b_n + parts - 1) / parts, kSortCap);
-      for (uint32_t ti = 0; ti < d.a_n; ++ti)
+      for (uint32_t u = 0; u < nu; ++u)
         for (uint32_t j0 = 0; j0 < d.b_n; j0 += per)
-          items.push_back(Item{t, ti, j0, std::min(per, d.b_n - j0)});
+          items.push_back(Item{t, u, j0, std::min(per, d.b_n - j0)});
     }
   }
 }
 
-// Longest first (estimated pairs: I tile atoms x atoms of its J run).
+// Longest first (estimated pairs: I unit atoms x atoms of its J run).
 static void order_items(std::vector<Item> &items, const std::vector<TypeDesc> &types,
-                        const std::vector<Tile> &tiles) {
+                        const std::vector<Tile> &tiles, bool paired) {
   std::vector<std::pair<double, uint32_t>> cost(items.size());
   for (size_t i = 0; i < items.size(); ++i) {
     const Item &it = items[i];
     const TypeDesc &d = types[it.type];
     double jn = 0;
     for (uint32_t m = 0; m < it.jcount; ++m) jn += tiles[d.b_base + (it.jstart + m) % d.b_n].cnt;
-    cost[i] = {-(double)tiles[d.a_base + it.ti].cnt * jn, (uint32_t)i};
+    double in = 0;
+    if (paired) {
+      for (uint32_t a = 2 * it.ti; a < std::min(2 * it.ti + 2, d.a_n); ++a) in += tiles[d.a_base + a].cnt;
+    } else {
+      in = tiles[d.a_base + it.ti].cnt;
+    }
+    cost[i] = {-in * jn, (uint32_t)i};
   }
   std::stable_sort(cost.begin(), cost.end());
   std::vector<Item> out(items.size());
@@ -825,24 +1096,23 @@ static void add_types(unsigned n_el, const uint32_t *ebase, const uint32_t *en, 
 }
 
 // Equal-count k-d tiles of one element (indices idx[0, ne) into x, y, z): the
-// longest axis of a part's bounding box is split at a multiple of kTile atoms
-// (nth_element) until a part holds at most kTile atoms and spans at most
-// `cap` on every axis.  Every tile of an element but one is full, so the
+// longest axis of a part's bounding box is split at a multiple of `unit`
+// atoms (nth_element) until a part holds at most `unit` atoms and spans at
+// most `cap` on every axis.  Every part of an element but one is full, so the
 // lanes of the I tiles are busy whatever the structure's shape (grid cells
-// over a sphere leave about half of them idle), and tiles are compact boxes.
+// over a sphere leave about half of them idle), and parts are compact boxes.
+// unit = kTile: the parts are the tiles.  unit = 2 kTile (paired, for the
+// FP64 kernel's I tile pairs): each part of two or more atoms is halved along
+// its longest axis into tiles 2u and 2u + 1.
 // Leaves come out depth first, left part first (neighbouring tiles are close).
 template <class T>
 static void kd_tiles(const T *x, const T *y, const T *z, uint32_t *idx, uint32_t ne, double cap,
-                     std::vector<std::pair<uint32_t, uint32_t>> &leaves) {
+                     bool paired, std::vector<std::pair<uint32_t, uint32_t>> &leaves) {
   const T *c[3] = {x, y, z};
-  std::vector<std::pair<uint32_t, uint32_t>> stack{{0u, ne}};
-  while (!stack.empty()) {
-    const auto part = stack.back();
-    stack.pop_back();
-    const uint32_t lo = part.first, n = part.second - part.first;
-    if (!n) continue;
+  const uint32_t unit = paired ? 2u * kTile : (uint32_t)kTile;
+  auto longest = [&](uint32_t lo, uint32_t hi) {
     double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t i = lo; i < part.second; ++i)
+    for (uint32_t i = lo; i < hi; ++i)
       for (int a = 0; a < 3; ++a) {
         const double v = (double)c[a][idx[i]];
         mn[a] = std::min(mn[a], v);
@@ -851,16 +1121,35 @@ static void kd_tiles(const T *x, const T *y, const T *z, uint32_t *idx, uint32_t
     int ax = 0;
     for (int a = 1; a < 3; ++a)
       if (mx[a] - mn[a] > mx[ax] - mn[ax]) ax = a;
-    if (n == 1 || (n <= (uint32_t)kTile && !(mx[ax] - mn[ax] > cap))) {
-      leaves.push_back({lo, n});
+    return std::make_pair(ax, mx[ax] - mn[ax]);
+  };
+  auto split = [&](uint32_t lo, uint32_t hi, uint32_t left, int ax) {
+    const T *ca = c[ax];
+    std::nth_element(idx + lo, idx + lo + left, idx + hi,
+                     [ca](uint32_t a, uint32_t b) { return ca[a] < ca[b]; });
+  };
+  std::vector<std::pair<uint32_t, uint32_t>> stack{{0u, ne}};
+  while (!stack.empty()) {
+    const auto part = stack.back();
+    stack.pop_back();
+    const uint32_t lo = part.first, n = part.second - part.first;
+    if (!n) continue;
+    const auto ax = longest(lo, part.second);
+    if (n == 1 || (n <= unit && !(ax.second > cap))) {
+      if (paired && n >= 2) {
+        const uint32_t h = n - n / 2;
+        split(lo, part.second, h, ax.first);
+        leaves.push_back({lo, h});
+        leaves.push_back({lo + h, n - h});
+      } else {
+        leaves.push_back({lo, n});
+      }
       continue;
     }
-    const uint32_t nl = std::max<uint32_t>(2u, (n + kTile - 1) / kTile);
-    uint32_t left = (nl / 2) * (uint32_t)kTile;
+    const uint32_t nl = std::max<uint32_t>(2u, (n + unit - 1) / unit);
+    uint32_t left = (nl / 2) * unit;
     if (left >= n) left = n / 2;
-    const T *ca = c[ax];
-    std::nth_element(idx + lo, idx + lo + left, idx + part.second,
-                     [ca](uint32_t a, uint32_t b) { return ca[a] < ca[b]; });
+    split(lo, part.second, left, ax.first);
     stack.push_back({lo + left, part.second});
     stack.push_back({lo, lo + left});
   }
@@ -883,7 +1172,8 @@ static void sort_structure_host(const T *xyz, size_t n, const unsigned *el_count
     idx.resize(ne);
     for (uint32_t i = 0; i < ne; ++i) idx[i] = i;
     leaves.clear();
-    kd_tiles<T>(xyz + off, xyz + n + off, xyz + 2 * n + off, idx.data(), ne, cap, leaves);
+    kd_tiles<T>(xyz + off, xyz + n + off, xyz + 2 * n + off, idx.data(), ne, cap, sizeof(T) == 8,
+                leaves);
     const size_t base = sx.size();
     ebase[e] = (uint32_t)tiles.size();
     for (const auto &l : leaves) tiles.push_back(Tile{(uint32_t)(base + l.first), l.second});
@@ -918,10 +1208,16 @@ static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wc
   hipEvent_t ev = nullptr;
   if (prof) XS_TRY(prof_begin(ctx, "hist", &ev));
   const dim3 grid((unsigned)nitems), block(kTile);
-  if (wcap == 8192) {
-    hipLaunchKernelGGL((hist_kernel<T, 8192>), grid, block, 0, ctx->stream, A);
+  if constexpr (sizeof(T) == 8) {
+    if (wcap == 8192)
+      hipLaunchKernelGGL((hist_kernel_pair<8192>), grid, block, 0, ctx->stream, A);
+    else
+      hipLaunchKernelGGL((hist_kernel_pair<32768>), grid, block, 0, ctx->stream, A);
   } else {
-    hipLaunchKernelGGL((hist_kernel<T, 32768>), grid, block, 0, ctx->stream, A);
+    if (wcap == 8192)
+      hipLaunchKernelGGL((hist_kernel<T, 8192>), grid, block, 0, ctx->stream, A);
+    else
+      hipLaunchKernelGGL((hist_kernel<T, 32768>), grid, block, 0, ctx->stream, A);
   }
   XS_HIP_CHECK(hipGetLastError());
   if (prof) XS_TRY(prof_end(ctx, "hist", ev));
@@ -937,8 +1233,8 @@ static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wc
 // about cubic and as full as possible; nx, ny, nz make slabs, columns and
 // tiles of a uniform-density structure narrower than 0.9 of the window cap
 // (the cap itself is checked on the tile boxes).
-static void choose_equal_count(uint32_t ne, const double bbox[6], double cap, uint32_t &gx,
-                               uint32_t &gy, uint32_t &nz) {
+static void choose_equal_count(uint32_t ne, const double bbox[6], double cap, bool paired,
+                               uint32_t &gx, uint32_t &gy, uint32_t &nz) {
   double ext[3];
   for (int a = 0; a < 3; ++a) ext[a] = std::max(bbox[3 + a] - bbox[a], 1e-6);
   auto need = [&](double e) -> uint32_t {
@@ -946,6 +1242,7 @@ static void choose_equal_count(uint32_t ne, const double bbox[6], double cap, ui
   };
   const uint32_t nx = need(ext[0]), ny = need(ext[1]);
   nz = need(ext[2]);
+  if (paired) nz += nz & 1u;
   const double T = std::max(1.0, std::ceil(ne / (double)kTile));
   const double L = std::cbrt(ext[0] * ext[1] * ext[2] / T);
   const long cx = std::lround(ext[0] / L), cy = std::lround(ext[1] / L);
@@ -958,7 +1255,8 @@ static void choose_equal_count(uint32_t ne, const double bbox[6], double cap, ui
       const uint64_t gb = std::max<uint64_t>(std::max<long>(b, 1), ny);
       if (ga * gb > std::max<uint64_t>(ne, 1) || ga * gb > 65536) continue;
       const double col = (double)ne / (double)(ga * gb);
-      const double tpc = std::max(std::ceil(col / kTile), (double)nz);
+      double tpc = std::max(std::ceil(col / kTile), (double)nz);
+      if (paired) tpc = 2.0 * std::ceil(tpc / 2.0);
       const double fill = ne / ((double)(ga * gb) * tpc * kTile);
       const double shape = std::fabs(std::log(ext[0] / ga / L)) + std::fabs(std::log(ext[1] / gb / L)) +
                            std::fabs(std::log(ext[2] / tpc / L));
@@ -972,14 +1270,18 @@ static void choose_equal_count(uint32_t ne, const double bbox[6], double cap, ui
 }
 
 // Tiles of the equal-count partition (positions from `pos`), column by column.
+// paired: an even number of runs per column, so that tiles 2u and 2u + 1 are
+// neighbouring runs of one column (the I tile pairs of the FP64 kernel).
 static void equal_count_tiles(uint32_t pos, uint32_t ne, uint32_t gx, uint32_t gy, uint32_t nz,
-                              std::vector<Tile> &tiles) {
+                              bool paired, std::vector<Tile> &tiles) {
   for (uint64_t sl = 0; sl < gx; ++sl) {
     const uint64_t S = (sl + 1) * ne / gx - sl * ne / gx;
     for (uint64_t c = 0; c < gy; ++c) {
       const uint32_t C = (uint32_t)((c + 1) * S / gy - c * S / gy);
       if (!C) continue;
-      const uint32_t nt = std::min(C, std::max((C + kTile - 1) / (uint32_t)kTile, nz));
+      uint32_t runs = std::max((C + kTile - 1) / (uint32_t)kTile, nz);
+      if (paired) runs += runs & 1u;
+      const uint32_t nt = std::min(C, runs);
       for (uint32_t t = 0, o = 0; t < nt; ++t) {
         const uint32_t len = (C - o) / (nt - t);
         tiles.push_back(Tile{pos + o, len});
@@ -1028,7 +1330,7 @@ static int eq_tiles_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *
     if (!ne) continue;
     const T *ex = xyz + eoff[e], *ey = xyz + n + eoff[e], *ez = xyz + 2 * n + eoff[e];
     uint32_t gx, gy, nz;
-    choose_equal_count(ne, bbox, cap, gx, gy, nz);
+    choose_equal_count(ne, bbox, cap, sizeof(T) == 8, gx, gy, nz);
     const unsigned nb = (ne + 255) / 256;
     size_t b = sb;
     hipLaunchKernelGGL(iota_u32, dim3(nb), dim3(256), 0, st, iota, ne);
@@ -1053,7 +1355,7 @@ static int eq_tiles_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *
     hipLaunchKernelGGL((gather_xyz<T>), dim3(nb), dim3(256), 0, st, ex, ey, ez, ordB, ne,
                        sxp + eoff[e], syp + eoff[e], szp + eoff[e]);
     XS_HIP_CHECK(hipGetLastError());
-    equal_count_tiles(eoff[e], ne, gx, gy, nz, tiles);
+    equal_count_tiles(eoff[e], ne, gx, gy, nz, sizeof(T) == 8, tiles);
     en[e] = (uint32_t)tiles.size() - ebase[e];
   }
   return XS_OK;
@@ -1163,8 +1465,8 @@ static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_co
   std::vector<TypeDesc> types;
   add_types(n_el, ebase, en, nhist, bin, 0ull, 0u, types);
   std::vector<Item> items;
-  add_items(types, 0, (uint32_t)types.size(), items);
-  order_items(items, types, tiles);
+  add_items(types, 0, (uint32_t)types.size(), items, sizeof(T) == 8);
+  order_items(items, types, tiles, sizeof(T) == 8);
 
   // 4. tables (exact sizes) in the table buffer; the stream is idle here
   const size_t sb = start_h.size() * sizeof(uint32_t), tb = tiles.size() * sizeof(Tile),
@@ -1235,8 +1537,8 @@ static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_coun
   std::vector<TypeDesc> types;
   add_types(n_el, ebase, en, nhist, bin, 0ull, 0u, types);
   std::vector<Item> items;
-  add_items(types, 0, (uint32_t)types.size(), items);
-  order_items(items, types, tiles);
+  add_items(types, 0, (uint32_t)types.size(), items, sizeof(T) == 8);
+  order_items(items, types, tiles, sizeof(T) == 8);
 
   const size_t nt = std::max<size_t>(tiles.size(), 1), ni = std::max<size_t>(items.size(), 1);
   XS_TRY(ensure_pinned(ctx, 64));
@@ -1369,10 +1671,10 @@ static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const
   b->ntypes_total = types.size();
   // work items: the 8192-bin window class, then the 32768 class, each longest first
   std::vector<Item> all, small, large;
-  add_items(types, 0, (uint32_t)types.size(), all);
+  add_items(types, 0, (uint32_t)types.size(), all, sizeof(T) == 8);
   for (const Item &it : all) (types[it.type].nhist <= 8192 ? small : large).push_back(it);
-  order_items(small, types, tiles);
-  order_items(large, types, tiles);
+  order_items(small, types, tiles, sizeof(T) == 8);
+  order_items(large, types, tiles, sizeof(T) == 8);
   b->items8192 = small.size();
   b->items32768 = large.size();
   small.insert(small.end(), large.begin(), large.end());
