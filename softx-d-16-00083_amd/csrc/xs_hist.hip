@@ -8,11 +8,13 @@
 // tile carries its bounding box.  Counts do not depend on atom order, so the
 // permutation leaves the histogram unchanged.
 //
-// Kernel.  One work item = (pair type of a structure, I tile, run of J tiles).
-// The I tile's atoms sit one per thread in registers; J tiles are sorted by
-// the lowest bin their pairs with the I tile can reach (from the two boxes)
-// and swept in that order; J atoms are staged 256 at a time in LDS and read
-// by every lane at the same address.  Counts go to a circular LDS window of
+// Kernel.  One work item = (pair type of a structure, I unit, run of J
+// tiles); an I unit is one tile (FP32 binning) or two neighbouring tiles
+// (FP64 binning, two I atoms per thread, so each J atom read from LDS serves
+// two pairs).  The I atoms sit in registers; J tiles are sorted by the
+// lowest bin their pairs with the I unit can reach (from the boxes) and
+// swept in that order; J atoms are staged 256 at a time in LDS and read by
+// every lane at the same address.  Counts go to a circular LDS window of
 // WCAP uint32 bins that only slides up; bins the remaining J tiles cannot
 // reach are flushed to the uint64 histogram with one atomic per non-zero bin,
 // so flush traffic per item is at most nhist + WCAP whatever the pair count.
@@ -367,27 +369,12 @@ __device__ __forceinline__ uint32_t bin_index<float>(float s, float bin, double 
   return t < (float)nhist ? (uint32_t)t : nhist;
 }
 
-// Fast part of bin_index: the estimate's floor and whether it may differ
-// from the IEEE result (then exact_bin decides).
-template <class T>
-__device__ __forceinline__ uint32_t fast_bin(T s, T bin, double inv_bin, double band,
-                                             uint32_t nhist, bool &bad);
-
-template <>
-__device__ __forceinline__ uint32_t fast_bin<double>(double s, double bin, double inv_bin,
-                                                     double band, uint32_t nhist, bool &bad) {
-  (void)bin;
-  (void)nhist;
+// Fast part of the FP64 bin_index: the estimate's floor and whether it may
+// differ from the IEEE result (then exact_bin decides).
+__device__ __forceinline__ uint32_t fast_bin(double s, double inv_bin, double band, bool &bad) {
   const double kd = sqrt_over_bin(s, inv_bin);
   bad = !(fabs(__builtin_amdgcn_fract(kd) - 0.5) < band);
   return (uint32_t)kd;
-}
-
-template <>
-__device__ __forceinline__ uint32_t fast_bin<float>(float s, float bin, double inv_bin, double band,
-                                                    uint32_t nhist, bool &bad) {
-  bad = false;
-  return bin_index<float>(s, bin, inv_bin, band, nhist);
 }
 
 template <class T>
@@ -403,15 +390,15 @@ __device__ __forceinline__ uint32_t exact_bin<float>(float s, float bin, uint32_
 }
 
 // FP64 fast path of one I atom: its coordinates relative to a centre c of
-// its tile times -2, and its squared distance from c.  With the J atoms'
-// coordinates relative to c (t) and their squared norms (tn) staged in LDS,
-// |ri - rj|^2 = n + tn + (-2 ri').rj' costs one add and three FMAs.
+// its I tiles' box times -2, and its squared distance from c.  With the J
+// atoms' coordinates relative to c (t) and their squared norms (tn) staged in
+// LDS, |ri - rj|^2 = n + tn + (-2 ri').rj' costs one add and three FMAs.
 struct IAtom {
   double ax, ay, az, n;
 };
 
-// Half-width of the estimate's band for the FP64 fast path of one I tile
-// (box bi, 6 doubles) whose coordinates are taken relative to the box centre.
+// Centre c of an I box (bi, 6 doubles) and the half-width of the FP64 fast
+// path's band for I atoms in that box taken relative to c.
 // Relative to IEEE sqrt(s)/bin the estimate errs by (a) below nhist 2^-46
 // bins from v_rsq_f64, the Newton step and the roundings of the reference
 // expression, and (b) below u (64 R^2 + 36 R + 10 T) bins from forming s as
@@ -434,63 +421,30 @@ __device__ __forceinline__ double tile_band(const double *bi, double band0, doub
   return band0 - 0x1p-49 * (fma(64.0 * R, R, 36.0 * R) + 10.0 * ((double)nhist + 1.0));
 }
 
-// Groups of 8 J atoms (not the like-element diagonal tile): 8 independent
-// distance chains and no per-pair branch on the common path.
-//  * FP64: the estimate takes s = |ri'|^2 + |rj'|^2 - 2 ri'.rj' from the
-//    tile-centred coordinates (it only has to lie within the band of the
-//    IEEE value); a pair whose estimate is too close to a bin edge, or whose
-//    s is not positive, recomputes s from the original coordinates and its
-//    bin exactly as the reference does.
-//  * SAFE: the J tile's bin range lies inside the window and below nhist
-//    (kl >= B, ku < B + WCAP, ku < nhist from the tile boxes) and maps to
-//    slots without a wrap, so every pair goes to slot k + off unchecked;
-//    otherwise each pair is tested.
-template <class T, int WCAP, bool SAFE>
-__device__ __forceinline__ void pair_group8(T xi, T yi, T zi, const IAtom &ia, const T *sx,
-                                            const T *sy, const T *sz, const double *tx,
-                                            const double *ty, const double *tz, const double *tn,
-                                            uint32_t ng, T bin, double inv_bin, double band,
-                                            uint32_t nhist, uint32_t B, uint32_t wlim, uint32_t D,
-                                            uint32_t off, uint32_t *win,
-                                            unsigned long long *__restrict__ hp,
-                                            unsigned long long *__restrict__ ovf) {
+// FP32 groups of 8 J atoms (not the like-element diagonal tile): the
+// reference GPU expression for 8 independent pairs, no per-pair branch.
+// SAFE: the J tile's bin range lies inside the window and below nhist
+// (kl >= B, ku < B + WCAP, ku < nhist from the tile boxes) and maps to slots
+// without a wrap, so every pair goes to slot k + off unchecked; otherwise
+// each pair is tested.
+template <int WCAP, bool SAFE>
+__device__ __forceinline__ void pair_group8_f32(float xi, float yi, float zi, const float *sx,
+                                                const float *sy, const float *sz, uint32_t ng,
+                                                float bin, uint32_t nhist, uint32_t B,
+                                                uint32_t wlim, uint32_t D, uint32_t off,
+                                                uint32_t *win, unsigned long long *__restrict__ hp,
+                                                unsigned long long *__restrict__ ovf) {
 #pragma clang fp contract(off)
-  constexpr bool F64 = sizeof(T) == 8;
   for (uint32_t j0 = 0; j0 < ng; j0 += 8) {
     uint32_t k[8];
-    bool bad[8], any = false;
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
-      if constexpr (F64) {
-        double s = ia.n + tn[j0 + p];
-        s = __builtin_fma(ia.az, tz[j0 + p], s);
-        s = __builtin_fma(ia.ay, ty[j0 + p], s);
-        s = __builtin_fma(ia.ax, tx[j0 + p], s);
-        k[p] = fast_bin<T>(s, bin, inv_bin, band, nhist, bad[p]);
-      } else {
-        const T dx = xi - sx[j0 + p];
-        const T dy = yi - sy[j0 + p];
-        const T dz = zi - sz[j0 + p];
-        T s = dx * dx + dy * dy;
-        s = s + dz * dz;
-        k[p] = fast_bin<T>(s, bin, inv_bin, band, nhist, bad[p]);
-      }
-      any |= bad[p];
-    }
-    if constexpr (F64) {
-      if (__builtin_expect(any, 0)) {
-#pragma unroll
-        for (int p = 0; p < 8; ++p) {
-          if (bad[p]) {
-            const T dx = xi - sx[j0 + p];
-            const T dy = yi - sy[j0 + p];
-            const T dz = zi - sz[j0 + p];
-            T s = dx * dx + dy * dy;
-            s = s + dz * dz;
-            k[p] = exact_bin<T>(s, bin, nhist);
-          }
-        }
-      }
+      const float dx = xi - sx[j0 + p];
+      const float dy = yi - sy[j0 + p];
+      const float dz = zi - sz[j0 + p];
+      float s = dx * dx + dy * dy;
+      s = s + dz * dz;
+      k[p] = exact_bin<float>(s, bin, nhist);
     }
     if constexpr (SAFE) {
 #pragma unroll
@@ -536,14 +490,14 @@ __device__ __forceinline__ void pair_block(T xi, T yi, T zi, uint32_t il, const 
   }
 }
 
-template <class T, int WCAP>
-__global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
-  constexpr bool F64 = sizeof(T) == 8;
-  constexpr int kShift = F64 ? kStage : 1;
+// FP32 binning: one work item = (pair type, I tile, run of J tiles), one I
+// atom per thread.
+template <int WCAP>
+__global__ __launch_bounds__(kTile) void hist_kernel_f32(Args<float> A) {
+  using T = float;
   __shared__ uint32_t win[WCAP];
   __shared__ unsigned long long keys[kSortCap];
   __shared__ T sx[kStage], sy[kStage], sz[kStage];
-  __shared__ double tx[kShift], ty[kShift], tz[kShift], tn[kShift];
 
   const Item it = A.items[blockIdx.x];
   const TypeDesc td = A.types[it.type];
@@ -561,15 +515,6 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
   const T bin = (T)td.bin;
   const uint32_t nhist = td.nhist;
   const double *bi = A.boxes + 6 * (size_t)tiI;
-  // FP64 fast path: coordinates relative to the I tile's box centre
-  double c[3] = {0.0, 0.0, 0.0};
-  double band = td.band;
-  IAtom ia{0.0, 0.0, 0.0, 0.0};
-  if constexpr (F64) {
-    band = tile_band(bi, td.band, td.inv_bin, nhist, c);
-    const double u = (double)xi - c[0], v = (double)yi - c[1], w = (double)zi - c[2];
-    ia = IAtom{-2.0 * u, -2.0 * v, -2.0 * w, fma(w, w, fma(v, v, u * u))};
-  }
   const uint32_t n = it.jcount;
   uint32_t npow = 1;
   while (npow < n) npow <<= 1;
@@ -642,17 +587,9 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
       const uint32_t cn = min((uint32_t)kStage, J.cnt - s0);
       if (tid < cn) {
         const size_t g = (size_t)J.off + s0 + tid;
-        const T x = A.x[g], y = A.y[g], z = A.z[g];
-        sx[tid] = x;
-        sy[tid] = y;
-        sz[tid] = z;
-        if constexpr (F64) {
-          const double u = x - c[0], v = y - c[1], w = z - c[2];
-          tx[tid] = u;
-          ty[tid] = v;
-          tz[tid] = w;
-          tn[tid] = fma(w, w, fma(v, v, u * u));
-        }
+        sx[tid] = A.x[g];
+        sy[tid] = A.y[g];
+        sz[tid] = A.z[g];
       }
       __syncthreads();
       if (ivalid) {
@@ -662,11 +599,11 @@ __global__ __launch_bounds__(kTile) void hist_kernel(Args<T> A) {
         } else {
           const uint32_t ng = cn & ~7u;
           if (safe)
-            pair_group8<T, WCAP, true>(xi, yi, zi, ia, sx, sy, sz, tx, ty, tz, tn, ng, bin,
-                                       td.inv_bin, band, nhist, B, wlim, D, off, win, hp, ovf);
+            pair_group8_f32<WCAP, true>(xi, yi, zi, sx, sy, sz, ng, bin, nhist, B, wlim, D, off, win,
+                                        hp, ovf);
           else
-            pair_group8<T, WCAP, false>(xi, yi, zi, ia, sx, sy, sz, tx, ty, tz, tn, ng, bin,
-                                        td.inv_bin, band, nhist, B, wlim, D, off, win, hp, ovf);
+            pair_group8_f32<WCAP, false>(xi, yi, zi, sx, sy, sz, ng, bin, nhist, B, wlim, D, off, win,
+                                         hp, ovf);
           pair_block<T, WCAP, false>(xi, yi, zi, tid, sx + ng, sy + ng, sz + ng, cn - ng, s0 + ng, bin,
                                      td.inv_bin, td.band, nhist, B, wlim, D, win, hp, ovf);
         }
@@ -702,12 +639,12 @@ __device__ __forceinline__ void pair_group4x2(const IAtom &ia, const IAtom &ib, 
       s = __builtin_fma(ia.az, rz, s);
       s = __builtin_fma(ia.ay, ry, s);
       s = __builtin_fma(ia.ax, rx, s);
-      k[p] = fast_bin<double>(s, bin, inv_bin, band, nhist, bad[p]);
+      k[p] = fast_bin(s, inv_bin, band, bad[p]);
       double t = ib.n + rn;
       t = __builtin_fma(ib.az, rz, t);
       t = __builtin_fma(ib.ay, ry, t);
       t = __builtin_fma(ib.ax, rx, t);
-      k[4 + p] = fast_bin<double>(t, bin, inv_bin, band, nhist, bad[4 + p]);
+      k[4 + p] = fast_bin(t, inv_bin, band, bad[4 + p]);
       any |= bad[p] | bad[4 + p];
     }
     if (__builtin_expect(any, 0)) {
@@ -757,7 +694,7 @@ __device__ __forceinline__ void pair_group4x2(const IAtom &ia, const IAtom &ib, 
 // (j > i) and J tile 2 ti + 1 with I tile 2 ti fully and with 2 ti + 1 for
 // j > i; those two J tiles take the per-pair loop.
 template <int WCAP>
-__global__ __launch_bounds__(kTile) void hist_kernel_pair(Args<double> A) {
+__global__ __launch_bounds__(kTile) void hist_kernel_f64(Args<double> A) {
   __shared__ uint32_t win[WCAP];
   __shared__ unsigned long long keys[kSortCap];
   __shared__ double sx[kStage], sy[kStage], sz[kStage];
@@ -1210,14 +1147,14 @@ static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wc
   const dim3 grid((unsigned)nitems), block(kTile);
   if constexpr (sizeof(T) == 8) {
     if (wcap == 8192)
-      hipLaunchKernelGGL((hist_kernel_pair<8192>), grid, block, 0, ctx->stream, A);
+      hipLaunchKernelGGL((hist_kernel_f64<8192>), grid, block, 0, ctx->stream, A);
     else
-      hipLaunchKernelGGL((hist_kernel_pair<32768>), grid, block, 0, ctx->stream, A);
+      hipLaunchKernelGGL((hist_kernel_f64<32768>), grid, block, 0, ctx->stream, A);
   } else {
     if (wcap == 8192)
-      hipLaunchKernelGGL((hist_kernel<T, 8192>), grid, block, 0, ctx->stream, A);
+      hipLaunchKernelGGL((hist_kernel_f32<8192>), grid, block, 0, ctx->stream, A);
     else
-      hipLaunchKernelGGL((hist_kernel<T, 32768>), grid, block, 0, ctx->stream, A);
+      hipLaunchKernelGGL((hist_kernel_f32<32768>), grid, block, 0, ctx->stream, A);
   }
   XS_HIP_CHECK(hipGetLastError());
   if (prof) XS_TRY(prof_end(ctx, "hist", ev));
