@@ -8,17 +8,17 @@
 // CalcFunctions.cpp:662-703; the plain sum is the one-block case).
 // A work group of 4 waves first computes the item's pair distances in FP64
 // into LDS (r/2pi, 1/r and, on a uniform q mesh, sin/cos of the phase step
-// dq r), then each lane owns 16 consecutive q and each wave a quarter of the
-// pairs: sin(q r) is seeded once per pair and lane from the series in
-// xs_sincos.h, advanced one q by the stored rotation and then by the
-// three-term recurrence s_{k+1} = 2 cos(dq r) s_k - s_{k-1} (one FMA; a
-// rounding error made at step j grows by |sin((k-j) dq r)/sin(dq r)| <= k-j,
-// so over 16 q it stays near 1e-14), so a pair-q term costs about 3 FP64
-// instructions.  FP64 because the intensity at
-// low q is a strongly cancelling difference (see xs_iq.hip).  The 1e-8 of the
+// dq r), then each lane owns 32 consecutive q and each half-wave every other
+// pair of its wave's quarter: sin(q r) is seeded once per pair and lane from
+// the series in xs_sincos.h, advanced one q by the stored rotation and then
+// by the three-term recurrence s_{k+1} = 2 cos(dq r) s_k - s_{k-1} (one FMA;
+// a rounding error made at step j grows by |sin((k-j) dq r)/sin(dq r)| <=
+// k-j, so over 32 q it stays near 1e-14), so a pair-q term costs about 3 FP64
+// instructions plus 1/32 of the seed.  FP64 because the intensity at low q
+// is a strongly cancelling difference (see xs_iq.hip).  The 1e-8 of the
 // denominator is applied to first order through a second accumulator of
-// sin(q r)/r^2.  The four waves' partial sums are combined in LDS and added to
-// S with FP64 atomics.
+// sin(q r)/r^2.  The half-waves' sums are added by a cross-lane shuffle, the
+// four waves' in LDS, and the result to S with FP64 atomics.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -31,11 +31,12 @@ namespace xs {
 namespace debye {
 
 constexpr int kTT = 32;       // atoms per tile
-constexpr int kQPT = 16;      // q per lane
+constexpr int kQPT = 32;      // q per lane
 constexpr int kWaves = 4;
 constexpr int kBlock = 64 * kWaves;
-constexpr int kQBlock = 64 * kQPT;  // q per work group
+constexpr int kQBlock = 32 * kQPT;  // q per work group (a half-wave covers them)
 constexpr double kTwoPi = 6.283185307179586476925286766559;
+static_assert((kWaves - 1) * kQPT * 32 <= 4 * kTT * kTT, "wave sums must fit the pair buffer");
 
 struct Tile {
   uint32_t off, cnt, el, blk;
@@ -79,14 +80,17 @@ __global__ __launch_bounds__(kBlock) void debye_kernel(const double *__restrict_
   }
   __syncthreads();
 
-  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const unsigned iq0 = blockIdx.y * kQBlock + lane * kQPT;
+  // lane: 32 consecutive q of the block (lane & 31) for every other pair of
+  // its wave's quarter (lane >> 5), so that the seed of the recurrence is
+  // spread over 32 q
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5;
+  const unsigned iq0 = blockIdx.y * kQBlock + (lane & 31) * kQPT;
   double acc[kQPT], corr[kQPT];
 #pragma unroll
   for (int m = 0; m < kQPT; ++m) acc[m] = corr[m] = 0.0;
   const double q0 = iq0 < nq ? q[iq0] : 0.0;
   const unsigned pstart = wave * (kTT * kTT / kWaves), pend = pstart + kTT * kTT / kWaves;
-  for (unsigned m = pstart; m < pend; ++m) {
+  for (unsigned m = pstart + half; m < pend; m += 2) {
     const double4 v = pr[m];
     if (v.y == 0.0) continue;
     const double ir = v.y, ir2 = v.y * v.y;
@@ -120,25 +124,28 @@ __global__ __launch_bounds__(kBlock) void debye_kernel(const double *__restrict_
       }
     }
   }
-  // sum over the waves: waves 1..3 park sin/(q r) - 1e-8 sin/(q r)^2 per q
+  // sum over the half-waves (lane l + 32 onto lane l, a cross-lane shuffle)
+  // and the waves: waves 1..3 park their 32 x 32 sums of
+  // sin/(q r) - 1e-8 sin/(q r)^2 in LDS (3 x 32 x 32 doubles, 24 KiB)
   double res[kQPT];
 #pragma unroll
   for (int k = 0; k < kQPT; ++k) {
     const double qk = iq0 + k < nq ? q[iq0 + k] : 0.0;
     res[k] = qk == 0.0 ? 0.0 : (acc[k] - (1e-8 / qk) * corr[k]) / qk;
+    res[k] += __shfl_down(res[k], 32, 64);
   }
   __syncthreads();
   double *park = reinterpret_cast<double *>(pr);
-  if (wave > 0) {
+  if (wave > 0 && half == 0) {
 #pragma unroll
-    for (int k = 0; k < kQPT; ++k) park[((wave - 1) * kQPT + k) * 64 + lane] = res[k];
+    for (int k = 0; k < kQPT; ++k) park[((wave - 1) * kQPT + k) * 32 + lane] = res[k];
   }
   __syncthreads();
-  if (wave == 0) {
+  if (wave == 0 && half == 0) {
 #pragma unroll
     for (int k = 0; k < kQPT; ++k) {
       double t = res[k];
-      for (int w = 0; w < kWaves - 1; ++w) t += park[(w * kQPT + k) * 64 + lane];
+      for (int w = 0; w < kWaves - 1; ++w) t += park[(w * kQPT + k) * 32 + lane];
       if (iq0 + k < nq && t != 0.0)
         atomicAdd(&S[((size_t)it.p * nparts + it.part) * nq + iq0 + k], t);
     }

@@ -4,6 +4,14 @@ import sys
 import numpy as np
 import pytest
 
+try:
+    # torch first: the HIP runtime it initialises must not find the device
+    # already set up by libxs_hip, or torch reports no GPU (the GPU tests that
+    # hand device buffers to the library use torch allocations)
+    import torch  # noqa: F401
+except ImportError:
+    torch = None
+
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(REPO, "softx-d-16-00083_amd")
 for p in (PKG, os.path.join(REPO, "oracle")):
