@@ -971,13 +971,13 @@ static void add_items(const std::vector<TypeDesc> &types, uint32_t first, uint32
           if ((nt & 1) && (nu - 1 + nu - u) % nu <= h) --total;
         }
         const uint32_t chunk =
-            std::min<uint32_t>(paired ? (total + 1) / 2 : total, (uint32_t)kSortCap);
+            std::min<uint32_t>(paired ? (total + 3) / 4 : total, (uint32_t)kSortCap);
         for (uint32_t c = 0; c < total; c += chunk)
           items.push_back(Item{t, u, (start + c) % nt, std::min<uint32_t>(chunk, total - c)});
       }
     } else {
       const uint32_t nu = paired ? (d.a_n + 1) / 2 : d.a_n;
-      const uint32_t parts = std::min<uint32_t>(d.b_n, paired ? 4u : 2u);
+      const uint32_t parts = std::min<uint32_t>(d.b_n, paired ? 8u : 2u);
       const uint32_t per = std::min<uint32_t>((d.b_n + parts - 1) / parts, kSortCap);
       for (uint32_t u = 0; u < nu; ++u)
         for (uint32_t j0 = 0; j0 < d.b_n; j0 += per)
