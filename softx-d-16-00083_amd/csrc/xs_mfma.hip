@@ -9,7 +9,7 @@
 // split exactly into NB 7-bit slices, H = sum_j b_j 2^(7j), b_j in [0, 127].
 // Every slice product sum_k a_i b_j is computed exactly by the int8 MFMA in
 // int32.  Products with the same d = j - i share the scale 2^(7d-6) and one
-// int32 accumulator; accumulators are moved into FP64 every 32,768 bins,
+// int32 accumulator; accumulators are added into S (FP64) every 32,768 bins,
 // before any can overflow (|acc| <= 5 * 64 * 127 * 32768 < 2^31).  All
 // NA * NB products are kept: dropping a_4 b_0 would leave an error of
 // 2^-28 per count on bins with small counts.  With NA = 5 the result differs
@@ -129,7 +129,7 @@ __global__ void slice_counts(const unsigned long long *__restrict__ hist, unsign
 }
 
 template <int NA, int NB, int NT>
-__global__ __launch_bounds__(256) void pairsum_mfma_kernel(const v4i *__restrict__ As,
+__global__ __launch_bounds__(256, 2) void pairsum_mfma_kernel(const v4i *__restrict__ As,
                                                            const v4i *__restrict__ Bs, unsigned MT,
                                                            unsigned NTn, unsigned KT, unsigned nq,
                                                            unsigned ncols, double *__restrict__ S) {
@@ -146,12 +146,10 @@ __global__ __launch_bounds__(256) void pairsum_mfma_kernel(const v4i *__restrict
   const v4i *ap = As + (size_t)mt * KT * 64 + lane;
   const v4i *bp = Bs + (size_t)ng * NT * KT * 64 + lane;
 
-  double out[NT][16];
-#pragma unroll
-  for (int t = 0; t < NT; ++t)
-#pragma unroll
-    for (int e = 0; e < 16; ++e) out[t][e] = 0.0;
-
+  // The int32 accumulators go to S (FP64) every kFlush k tiles: the wave owns
+  // its output elements, so the first flush stores and later ones add.  No
+  // FP64 output registers and no operand double buffer, so that two waves fit
+  // on a SIMD and hide each other's operand loads.
   for (unsigned kb = 0; kb < KT; kb += kFlush) {
     const unsigned ke = min(KT, kb + kFlush);
     v16i acc[NG][NT];
@@ -159,23 +157,14 @@ __global__ __launch_bounds__(256) void pairsum_mfma_kernel(const v4i *__restrict
     for (int g = 0; g < NG; ++g)
 #pragma unroll
       for (int t = 0; t < NT; ++t) acc[g][t] = v16i{0};
-    // operands of k tile kt + 1 are loaded while the products of kt run
-    v4i a[NA], b[NB][NT];
-#pragma unroll
-    for (int i = 0; i < NA; ++i) a[i] = ap[i * sA + (size_t)kb * 64];
-#pragma unroll
-    for (int j = 0; j < NB; ++j)
-#pragma unroll
-      for (int t = 0; t < NT; ++t) b[j][t] = bp[j * sB + ((size_t)t * KT + kb) * 64];
     for (unsigned kt = kb; kt < ke; ++kt) {
-      v4i an[NA], bn[NB][NT];
-      const unsigned kn = kt + 1 < ke ? kt + 1 : kt;
+      v4i a[NA], b[NB][NT];
 #pragma unroll
-      for (int i = 0; i < NA; ++i) an[i] = ap[i * sA + (size_t)kn * 64];
+      for (int i = 0; i < NA; ++i) a[i] = ap[i * sA + (size_t)kt * 64];
 #pragma unroll
       for (int j = 0; j < NB; ++j)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) bn[j][t] = bp[j * sB + ((size_t)t * KT + kn) * 64];
+        for (int t = 0; t < NT; ++t) b[j][t] = bp[j * sB + ((size_t)t * KT + kt) * 64];
 #pragma unroll
       for (int i = 0; i < NA; ++i)
 #pragma unroll
@@ -184,30 +173,21 @@ __global__ __launch_bounds__(256) void pairsum_mfma_kernel(const v4i *__restrict
           for (int t = 0; t < NT; ++t)
             acc[j - i - DMIN][t] =
                 __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j][t], acc[j - i - DMIN][t], 0, 0, 0);
-#pragma unroll
-      for (int i = 0; i < NA; ++i) a[i] = an[i];
-#pragma unroll
-      for (int j = 0; j < NB; ++j)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) b[j][t] = bn[j][t];
     }
 #pragma unroll
-    for (int g = 0; g < NG; ++g) {
-      const double sc = ldexp(1.0, 7 * (g + DMIN) - 6);
+    for (int t = 0; t < NT; ++t) {
+      const unsigned col = (ng * NT + t) * 32 + (lane & 31);
+      if (col >= ncols) continue;
 #pragma unroll
-      for (int t = 0; t < NT; ++t)
+      for (int e = 0; e < 16; ++e) {
+        const unsigned row = mt * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+        if (row >= nq) continue;
+        double v = 0.0;
 #pragma unroll
-        for (int e = 0; e < 16; ++e) out[t][e] = fma((double)acc[g][t][e], sc, out[t][e]);
-    }
-  }
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const unsigned col = (ng * NT + t) * 32 + (lane & 31);
-    if (col >= ncols) continue;
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const unsigned row = mt * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-      if (row < nq) S[(size_t)col * nq + row] = out[t][e];
+        for (int g = 0; g < NG; ++g) v = fma((double)acc[g][t][e], ldexp(1.0, 7 * (g + DMIN) - 6), v);
+        double *o = S + (size_t)col * nq + row;
+        *o = kb == 0 ? v : *o + v;
+      }
     }
   }
 }
@@ -270,7 +250,7 @@ int pairsum_mfma(xs_ctx *ctx, const unsigned long long *hist, unsigned ncols, un
   if ((cmax >> (7 * nb)) != 0ull)
     return fail(XS_ERR_RANGE, "xs_hist_pairsum: a bin count exceeds 2^35 (matrix-core path)");
   // two column tiles per wave unless the accumulators would not fit
-  const int NT = (ncols > 32 && nb <= 3) ? 2 : 1;
+  const int NT = (ncols > 32 && nb <= 2) ? 2 : 1;  // nb = 3 with NT = 2 would spill
   const unsigned NTn = (ncols + 32 * NT - 1) / (32 * NT) * NT;
 
   hipEvent_t ev;
