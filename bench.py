@@ -253,6 +253,33 @@ def ensemble_leg(ctx, count, steps, warmup):
     hist_eq = bool(torch.equal(hist, hist2))
     pdf_eq = bool(torch.equal(pdf, pdf2))
     s_rel = float(((S - S2).abs().max() / S2.abs().max()).item())
+
+    # structure by structure with the reference GPU binning (FP32 coordinates,
+    # IEEE float sqrt and division), the plain FP32 form: one timed pass
+    xyz32_d = [torch.from_numpy(np.ascontiguousarray(p.xyz)).to(dev) for p in parts]
+    hist3 = torch.empty_like(hist)
+
+    def plain_f32():
+        e0 = 0
+        for s, p in enumerate(parts):
+            P = p.n_el * (p.n_el + 1) // 2
+            nh = int(b.nhist[s])
+            hp = hist3.data_ptr() + 8 * int(b.hist_off[s])
+            bctx.hist_dev(xyz32_d[s].data_ptr(), p.n, p.el_count, bin_, nh, hp, None, f64=False)
+            bctx.hist_pairsum_dev(hp, P, nh, bin_, qd.data_ptr(), nq,
+                                  S2.data_ptr() + 8 * int(b.type_first[s]) * nq)
+            bctx.pdf_dev(hp, p.el_count, nh, bin_, "rPDF", rho[s], sl[e0:e0 + p.n_el],
+                         pdf2.data_ptr() + 8 * int(b.pdf_off[s]), cpu_radius=True)
+            e0 += p.n_el
+
+    plain_f32()
+    bctx.synchronize()
+    t0 = time.perf_counter()
+    plain_f32()
+    bctx.synchronize()
+    t_p32 = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    moved32 = int((hist - hist3).abs().sum().item()) // 2
     ovf = int(b.overflow().sum())
     b.close()
     bctx.close()
@@ -265,6 +292,11 @@ def ensemble_leg(ctx, count, steps, warmup):
         "patterns_per_s_batched": round(count / t_b, 2),
         "patterns_per_s_plain": round(count / t_p, 2),
         "speedup": round(t_p / t_b, 3),
+        "patterns_per_s_plain_f32": round(count / t_p32, 2),
+        "speedup_vs_plain_f32": round(t_p32 / t_b, 3),
+        "plain_f32_note": "structure by structure with FP32 coordinates and the reference GPU binning "
+                          "(IEEE float sqrt and division); the batched form bins in FP64 (the reference "
+                          "CPU expression)",
         "ms_batched": round(t_b * 1e3, 2),
         "ms_plain": round(t_p * 1e3, 2),
         "gpairs_per_s_batched": round(pairs / t_b / 1e9, 2),
@@ -274,7 +306,8 @@ def ensemble_leg(ctx, count, steps, warmup):
                           "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                           "frac": round(OPS_PER_PAIR * pairs / (hist_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)},
         "checks": {"hist_equal_plain": hist_eq, "pdf_equal_plain": pdf_eq,
-                   "pairsum_max_rel_diff_vs_plain": s_rel, "overflow": ovf},
+                   "pairsum_max_rel_diff_vs_plain": s_rel, "overflow": ovf,
+                   "f32_binning_moved_pairs": moved32},
         "setup_s": {"generate": round(t_gen, 2), "plan": round(t_plan, 2)},
     }
 
