@@ -28,8 +28,10 @@
 // Counts equal the oracle's bit for bit in both modes.
 //
 // Pipelines.
-//   xs_hist, device input: bbox and per-cell counts are computed on the GPU and
-//     read back (two small synchronisations) to build tiles and work items.
+//   xs_hist, device input: the bbox is computed on the GPU and read back; the
+//     equal-count sorts run on the GPU and the tiles follow from the counts
+//     (one more read-back of the tile boxes when the window caps their width;
+//     the cell-grid fallback reads back per-cell counts).
 //   xs_hist, host input, and xs_batch_plan: the sort, tiles, boxes and work
 //     items are built on the host and uploaded.
 //   xs_batch_run (xs_batch.hip): zero fills, histograms, pair sums and PDFs
@@ -868,9 +870,10 @@ __global__ __launch_bounds__(kTile) void hist_kernel_f64(Args<double> A) {
 
 // ---------------------------------------------------------------- host planning
 
-// Cell sizing.  Lanes of a work group hold the atoms of one I tile, so a
-// tile should be nearly full: cells are sized for kFill atoms in the fullest
-// cell.  The first guess assumes uniform density over the bounding box; one
+// Cell-grid fallback (device input whose equal-count tiles would be wider
+// than the window allows).  Cell sizing: lanes of a work group hold the
+// atoms of one I tile, so a tile should be nearly full: cells are sized for
+// kFill atoms in the fullest cell.  The first guess assumes uniform density over the bounding box; one
 // refinement rescales the edge by cbrt(kFill / fullest count) after counting
 // (spheres are ~2x denser than their box).  When the histogram exceeds the
 // window, the edge is capped at (WCAP - 8) bin / (2 sqrt 3) so that the bin
