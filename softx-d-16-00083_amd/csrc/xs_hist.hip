@@ -362,12 +362,26 @@ __device__ __forceinline__ uint32_t bin_index<double>(double s, double bin, doub
   return t < (double)nhist ? (uint32_t)t : nhist;
 }
 
+// float: (unsigned)(sqrtf(s) / bin) of the reference GPU kernel in IEEE
+// single precision.  sqrtf is the IEEE one; the division is evaluated as
+// (float)(r * inv_bin) in double, inv_bin = RN(1 / (double)bin).  That
+// product lies within 2^-51.9 (relative) of r / bin, while a quotient of two
+// floats lies at least 2^-49 from any midpoint m between floats: r - m bin
+// is a non-zero multiple of ulp(m) ulp(bin) (m has 25 significant bits and
+// an odd last one, so m bin has more than 24), i.e. |r/bin - m| >=
+// ulp(m) ulp(bin) / bin >= 2^-49 m.  Rounding the product to float
+// therefore gives the IEEE quotient.  inv_bin must be 1 / (double)(float)bin.
+// sqrtf(s) / bin in IEEE single precision (before the conversion to a bin).
+__device__ __forceinline__ float dist_over_bin_f32(float s, double inv_bin) {
+  return (float)((double)__builtin_sqrtf(s) * inv_bin);
+}
+
 template <>
 __device__ __forceinline__ uint32_t bin_index<float>(float s, float bin, double inv_bin,
                                                      double band, uint32_t nhist) {
-  (void)inv_bin;
+  (void)bin;
   (void)band;
-  const float t = __builtin_sqrtf(s) / bin;
+  const float t = dist_over_bin_f32(s, inv_bin);
   return t < (float)nhist ? (uint32_t)t : nhist;
 }
 
@@ -383,12 +397,6 @@ template <class T>
 __device__ __forceinline__ uint32_t exact_bin(T s, T bin, uint32_t nhist) {
   const T t = (T)__builtin_sqrt((double)s) / bin;
   return t < (T)nhist ? (uint32_t)t : nhist;
-}
-
-template <>
-__device__ __forceinline__ uint32_t exact_bin<float>(float s, float bin, uint32_t nhist) {
-  const float t = __builtin_sqrtf(s) / bin;
-  return t < (float)nhist ? (uint32_t)t : nhist;
 }
 
 // FP64 fast path of one I atom: its coordinates relative to a centre c of
@@ -432,7 +440,7 @@ __device__ __forceinline__ double tile_band(const double *bi, double band0, doub
 template <int WCAP, bool SAFE>
 __device__ __forceinline__ void pair_group8_f32(float xi, float yi, float zi, const float *sx,
                                                 const float *sy, const float *sz, uint32_t ng,
-                                                float bin, uint32_t nhist, uint32_t B,
+                                                float bin, double inv_bin, uint32_t nhist, uint32_t B,
                                                 uint32_t wlim, uint32_t D, uint32_t off,
                                                 uint32_t *win, unsigned long long *__restrict__ hp,
                                                 unsigned long long *__restrict__ ovf) {
@@ -446,7 +454,9 @@ __device__ __forceinline__ void pair_group8_f32(float xi, float yi, float zi, co
       const float dz = zi - sz[j0 + p];
       float s = dx * dx + dy * dy;
       s = s + dz * dz;
-      k[p] = exact_bin<float>(s, bin, nhist);
+      // no clamp: a bin at or past nhist (t >= nhist, converted with
+      // saturation) goes to the overflow counter below either way
+      k[p] = (uint32_t)dist_over_bin_f32(s, inv_bin);
     }
     if constexpr (SAFE) {
 #pragma unroll
@@ -515,6 +525,7 @@ __global__ __launch_bounds__(kTile) void hist_kernel_f32(Args<float> A) {
     zi = A.z[gi];
   }
   const T bin = (T)td.bin;
+  const double inv_bin32 = 1.0 / (double)bin;  // for bin_index<float>
   const uint32_t nhist = td.nhist;
   const double *bi = A.boxes + 6 * (size_t)tiI;
   const uint32_t n = it.jcount;
@@ -596,18 +607,18 @@ __global__ __launch_bounds__(kTile) void hist_kernel_f32(Args<float> A) {
       __syncthreads();
       if (ivalid) {
         if (diag) {
-          pair_block<T, WCAP, true>(xi, yi, zi, tid, sx, sy, sz, cn, s0, bin, td.inv_bin, td.band,
+          pair_block<T, WCAP, true>(xi, yi, zi, tid, sx, sy, sz, cn, s0, bin, inv_bin32, td.band,
                                     nhist, B, wlim, D, win, hp, ovf);
         } else {
           const uint32_t ng = cn & ~7u;
           if (safe)
-            pair_group8_f32<WCAP, true>(xi, yi, zi, sx, sy, sz, ng, bin, nhist, B, wlim, D, off, win,
-                                        hp, ovf);
+            pair_group8_f32<WCAP, true>(xi, yi, zi, sx, sy, sz, ng, bin, inv_bin32, nhist, B, wlim, D,
+                                        off, win, hp, ovf);
           else
-            pair_group8_f32<WCAP, false>(xi, yi, zi, sx, sy, sz, ng, bin, nhist, B, wlim, D, off, win,
-                                         hp, ovf);
+            pair_group8_f32<WCAP, false>(xi, yi, zi, sx, sy, sz, ng, bin, inv_bin32, nhist, B, wlim, D,
+                                         off, win, hp, ovf);
           pair_block<T, WCAP, false>(xi, yi, zi, tid, sx + ng, sy + ng, sz + ng, cn - ng, s0 + ng, bin,
-                                     td.inv_bin, td.band, nhist, B, wlim, D, win, hp, ovf);
+                                     inv_bin32, td.band, nhist, B, wlim, D, win, hp, ovf);
         }
       }
       __syncthreads();
