@@ -503,13 +503,17 @@ def amp2d_leg(ctx, nrot=16):
 def debye_leg(ctx):
     """Direct Debye pair sums (every pair, every q, FP64 terms): the 1,985-atom
     fcc sphere on the reference q mesh (1,024 q over 0.5-10 A^-1; 2.0e9
-    pair-q) and the 50,000-atom three-element disordered solid on the same
-    1,024 q (1.28e12 pair-q).  Gpair-q/s from the kernel's HIP events; the small case is
-    checked against an FP64 numpy sum over all pairs."""
+    pair-q), the user manual's NaCl 10x10x10 example (8,000 atoms, two
+    elements, 1,024 q from 0.5 A^-1 in steps of 6.5/1,024; 3.3e10 pair-q)
+    and the 50,000-atom three-element disordered solid on 1,024 q over
+    0.5-10 A^-1 (1.28e12 pair-q).  Gpair-q/s from the kernel's HIP events;
+    the small case is checked against an FP64 numpy sum over all pairs."""
     import torch
     dev = torch.device("cuda", 0)
     out = {}
     for name, ens, q in (("fcc_1985", st.fcc_sphere(), st.q_mesh(0.5, 10.0, 1024)),
+                         ("nacl_10x10x10_8000", st.rocksalt_cube(ncell=10, vacancy=0.0, sigma=0.0),
+                          st.q_mesh(0.5, 7.0, 1024)),
                          ("disordered_50000", st.disordered_cubic(), st.q_mesh(0.5, 10.0, 1024))):
         x64 = ens.xyz64()
         xd = torch.from_numpy(x64).to(dev)
