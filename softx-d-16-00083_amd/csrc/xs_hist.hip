@@ -371,9 +371,25 @@ __device__ __forceinline__ uint32_t bin_index<double>(double s, double bin, doub
 // an odd last one, so m bin has more than 24), i.e. |r/bin - m| >=
 // ulp(m) ulp(bin) / bin >= 2^-49 m.  Rounding the product to float
 // therefore gives the IEEE quotient.  inv_bin must be 1 / (double)(float)bin.
+// IEEE sqrtf(s) for s >= 2^-96: v_sqrt_f32 is within one ulp there, and
+// Tuckerman's rounding test picks the correctly rounded root among y and its
+// two float neighbours y-, y+: RN(sqrt s) = y- if s <= y y-, y+ if s > y y+,
+// else y (the fma residuals' signs are exact).  The general sqrtf adds a
+// rescaling of s below 2^-96 and a zero/inf/NaN select; the s here are
+// finite sums of squares, and below 2^-96 the root (< 2^-47) lies far below
+// one bin, so the bin index is 0 either way.
+__device__ __forceinline__ float sqrt_rn_f32(float s) {
+  const float y = __builtin_amdgcn_sqrtf(s);
+  const float dn = __int_as_float(__float_as_int(y) - 1);
+  const float up = __int_as_float(__float_as_int(y) + 1);
+  float r = fmaf(-dn, y, s) <= 0.0f ? dn : y;
+  r = fmaf(-up, y, s) > 0.0f ? up : r;
+  return r;
+}
+
 // sqrtf(s) / bin in IEEE single precision (before the conversion to a bin).
 __device__ __forceinline__ float dist_over_bin_f32(float s, double inv_bin) {
-  return (float)((double)__builtin_sqrtf(s) * inv_bin);
+  return (float)((double)sqrt_rn_f32(s) * inv_bin);
 }
 
 template <>

@@ -101,6 +101,29 @@ def test_bin_edges_integer_lattice(ctx):
         assert_same(r32, f"lattice bin {b} f32")
 
 
+def test_f32_root_rounding_one_ulp_bins(ctx):
+    """FP32 binning with bin = 2^-23 A and all distances below 2 A: every
+    distance in [1, 2) A then has a bin of its own float value (r / bin is an
+    exact integer there), so the counts record each pair's rounded sqrtf(s) bit
+    for bit: 4.2 million cross-cluster pairs, plus near-coincident atoms with
+    s down in the denormal range, whose bin is 0."""
+    rng = np.random.Generator(np.random.PCG64(21))
+    a = rng.uniform(-0.15, 0.15, size=(3, 2040))
+    b = rng.uniform(-0.15, 0.15, size=(3, 2048)) + np.array([[1.15], [0.3], [0.1]])
+    tiny = np.array([[0.0, 1e-20, 3e-21, 1e-19, -2e-22, 5e-24, 1e-18, 0.0],
+                     [0.0, 0.0, 2e-21, 0.0, 1e-22, 0.0, -1e-18, 4e-23],
+                     [0.0, 0.0, 0.0, 3e-20, 0.0, 5e-24, 0.0, 0.0]])
+    xyz = np.ascontiguousarray(np.concatenate([tiny, a, b], axis=1).astype(np.float32))
+    ens = st.Ensemble(xyz, [xyz.shape[1]], ["A"])
+    bin_, nh = 2.0 ** -23, 1 << 24
+    g, ov = ctx.hist(ens.xyz, ens.el_count, bin_, nh, f64=False)
+    o, oo = O.hist_f32(ens.xyz, ens.el_count, bin_, nh)
+    ulp_bins = int(np.count_nonzero(o[..., 1 << 23:]))
+    print(f"distinct roots in [1, 2): {ulp_bins}", flush=True)
+    assert ulp_bins > 1_000_000
+    assert_same((g, ov, o, oo), "one-ulp bins f32")
+
+
 def test_device_pointers_match_host(ctx):
     import torch
     ens = st.fcc_sphere(radius=12.0, seed=4)
