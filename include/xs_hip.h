@@ -47,6 +47,8 @@ extern "C" {
 #define XS_PDF_CPU_RADIUS 0x4u  /* xs_pdf: radius offset of the CPU path for p >= 1 */
 #define XS_BATCH_GRAPH 0x8u     /* xs_batch_run: capture once, replay as a hipGraph */
 #define XS_PAIRSUM_MFMA 0x10u   /* xs_hist_pairsum: int8 matrix-core product */
+#define XS_F32_DOUBLE_QUOTIENT 0x20u /* xs_hist, xs_batch_plan, FP32 binning: skip the
+                                        FP32 quotient form (see xs_hist) */
 
 /* PDF types, typedefs.h:54-56 */
 #define XS_PDF_RDF 0
@@ -99,8 +101,13 @@ int xs_profile_read(xs_ctx *ctx, const char *kernel, double *total_ms,
  * Same-element pairs are j > i, cross-element pairs all (i, j)
  * (CalcFunctions.cpp:782).  hist (ntypes*nhist) is overwritten; pairs whose
  * index is >= nhist go to *overflow (may be NULL; with XS_DEVICE_PTRS a
- * device pointer).  Replaces calcHist (CalcFunctions.cpp:757-809) and
- * calcHistCuda (CalcFunctionsCUDA.cu:855-920). */
+ * device pointer).  FP32 binning evaluates the IEEE quotient by bin either as
+ * a rounded double product or, for a bin width that a check over all 2^23
+ * significands has passed (once per bin and context; it synchronises), with
+ * FP32 operations only, which is faster; XS_F32_DOUBLE_QUOTIENT keeps the
+ * double product.  Both give the IEEE quotient, so the counts are the same.
+ * Replaces calcHist (CalcFunctions.cpp:757-809) and calcHistCuda
+ * (CalcFunctionsCUDA.cu:855-920). */
 int xs_hist(xs_ctx *ctx, const void *xyz, size_t n_atoms,
             const unsigned *el_count, unsigned n_el, double bin, unsigned nhist,
             uint64_t *hist, uint64_t *overflow, unsigned flags);
@@ -203,7 +210,8 @@ int xs_pairlist_pairsum(xs_ctx *ctx, const double *r, const unsigned *type, size
  * follow those of structure s-1 in el_count; xyz[s] is its element-major
  * double SoA (3 * N_s values, host memory).  Coordinates are binned in double
  * with XS_F64_BINNING (calcHist, CalcFunctions.cpp:784-788), else rounded to
- * float and binned as calcHistKernel (CalcFunctionsCUDA.cu:832-833).
+ * float and binned as calcHistKernel (CalcFunctionsCUDA.cu:832-833), with
+ * the quotient form chosen as in xs_hist (XS_F32_DOUBLE_QUOTIENT applies).
  * nhist[s] = 0 (or nhist == NULL) takes GetHistSize's bin count
  * (CalcFunctions.cpp:224-243).  Each structure needs at least one atom. */
 int xs_batch_plan(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el,

@@ -171,6 +171,7 @@ int xs_free(xs_ctx *ctx) {
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->aux) (void)hipFree(ctx->aux);
   if (ctx->mat) (void)hipFree(ctx->mat);
+  if (ctx->quot_check) (void)hipFree(ctx->quot_check);
   if (ctx->pin) (void)hipHostFree(ctx->pin);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;

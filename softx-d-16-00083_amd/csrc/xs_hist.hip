@@ -92,6 +92,7 @@ struct Args {
   const Item *items;
   unsigned long long *hist;
   unsigned long long *overflow;
+  uint32_t quot_f32;  // FP32 binning: the bin passed quot_f32_check_kernel
 };
 
 // ---------------------------------------------------------------- prep kernels
@@ -392,6 +393,40 @@ __device__ __forceinline__ float dist_over_bin_f32(float s, double inv_bin) {
   return (float)((double)sqrt_rn_f32(s) * inv_bin);
 }
 
+// RN(1 / bin) in single precision (IEEE division).
+__device__ __forceinline__ float recip_f32(float bin) { return 1.0f / bin; }
+
+// r / bin in FP32 operations only: q0 = r yb with yb = RN(1 / bin), the
+// residual r - q0 bin by one FMA, and Markstein's correction q0 + e yb by
+// another.  Three FP32 instructions against the FP64 multiply and two
+// conversions of dist_over_bin_f32, which issue at the FP64 rate.  The
+// correction is proven to give the IEEE quotient when q0 lies within one ulp
+// of r / bin, but q0 can lie up to 1.5 ulp away, so the kernels use this form
+// only for bins that quot_f32_check_kernel has passed.
+__device__ __forceinline__ float quot_f32(float r, float bin, float yb) {
+#pragma clang fp contract(off)
+  const float q0 = r * yb;
+  return fmaf(fmaf(-q0, bin, r), yb, q0);
+}
+
+// Counts the r in [1, 2) (all 2^23 significands, one per thread) for which
+// quot_f32 differs from the double-product quotient, which equals the IEEE one
+// for every r.  For 2^-60 <= bin <= 2^60 no count means quot_f32 is the IEEE
+// quotient for every r the histogram kernels form:
+// - q0, the residual and the correction scale exactly with the exponent of r
+//   while they stay normal.  That holds wherever r / bin >= 1/4: then q0 is
+//   normal, r >= 2^-62, and a non-zero residual, a multiple of
+//   ulp(q0) ulp(bin), is at least 2^-109.  So one binade decides them all.
+// - Where r / bin < 1/4, q0 stays below 1/4 (1 + 2^-23) and the correction
+//   below 2^-24, so both quotients are below 1 and both bins are 0.
+// The quotient stays below 2^31 (the host rejects wider spans), far from
+// overflow.
+__global__ void quot_f32_check_kernel(float bin, double inv_bin, unsigned long long *bad) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;  // grid of exactly 2^23 threads
+  const float r = __uint_as_float(0x3f800000u | u);
+  if (quot_f32(r, bin, recip_f32(bin)) != (float)((double)r * inv_bin)) atomicAdd(bad, 1ull);
+}
+
 template <>
 __device__ __forceinline__ uint32_t bin_index<float>(float s, float bin, double inv_bin,
                                                      double band, uint32_t nhist) {
@@ -449,15 +484,17 @@ __device__ __forceinline__ double tile_band(const double *bi, double band0, doub
 
 // FP32 groups of 8 J atoms (not the like-element diagonal tile): the
 // reference GPU expression for 8 independent pairs, no per-pair branch.
+// FQ: the quotient by bin as quot_f32 (yb = recip_f32(bin)), else as the
+// rounded double product.
 // SAFE: the J tile's bin range lies inside the window and below nhist
 // (kl >= B, ku < B + WCAP, ku < nhist from the tile boxes) and maps to slots
 // without a wrap, so every pair goes to slot k + off unchecked; otherwise
 // each pair is tested.
-template <int WCAP, bool SAFE>
+template <int WCAP, bool SAFE, bool FQ>
 __device__ __forceinline__ void pair_group8_f32(float xi, float yi, float zi, const float *sx,
                                                 const float *sy, const float *sz, uint32_t ng,
-                                                float bin, double inv_bin, uint32_t nhist, uint32_t B,
-                                                uint32_t wlim, uint32_t D, uint32_t off,
+                                                float bin, double inv_bin, float yb, uint32_t nhist,
+                                                uint32_t B, uint32_t wlim, uint32_t D, uint32_t off,
                                                 uint32_t *win, unsigned long long *__restrict__ hp,
                                                 unsigned long long *__restrict__ ovf) {
 #pragma clang fp contract(off)
@@ -472,7 +509,10 @@ __device__ __forceinline__ void pair_group8_f32(float xi, float yi, float zi, co
       s = s + dz * dz;
       // no clamp: a bin at or past nhist (t >= nhist, converted with
       // saturation) goes to the overflow counter below either way
-      k[p] = (uint32_t)dist_over_bin_f32(s, inv_bin);
+      if constexpr (FQ)
+        k[p] = (uint32_t)quot_f32(sqrt_rn_f32(s), bin, yb);
+      else
+        k[p] = (uint32_t)dist_over_bin_f32(s, inv_bin);
     }
     if constexpr (SAFE) {
 #pragma unroll
@@ -520,7 +560,7 @@ __device__ __forceinline__ void pair_block(T xi, T yi, T zi, uint32_t il, const 
 
 // FP32 binning: one work item = (pair type, I tile, run of J tiles), one I
 // atom per thread.
-template <int WCAP>
+template <int WCAP, bool FQ>
 __global__ __launch_bounds__(kTile) void hist_kernel_f32(Args<float> A) {
   using T = float;
   __shared__ uint32_t win[WCAP];
@@ -542,6 +582,7 @@ __global__ __launch_bounds__(kTile) void hist_kernel_f32(Args<float> A) {
   }
   const T bin = (T)td.bin;
   const double inv_bin32 = 1.0 / (double)bin;  // for bin_index<float>
+  const float yb = recip_f32(bin);              // for quot_f32
   const uint32_t nhist = td.nhist;
   const double *bi = A.boxes + 6 * (size_t)tiI;
   const uint32_t n = it.jcount;
@@ -628,11 +669,11 @@ __global__ __launch_bounds__(kTile) void hist_kernel_f32(Args<float> A) {
         } else {
           const uint32_t ng = cn & ~7u;
           if (safe)
-            pair_group8_f32<WCAP, true>(xi, yi, zi, sx, sy, sz, ng, bin, inv_bin32, nhist, B, wlim, D,
-                                        off, win, hp, ovf);
+            pair_group8_f32<WCAP, true, FQ>(xi, yi, zi, sx, sy, sz, ng, bin, inv_bin32, yb, nhist, B,
+                                            wlim, D, off, win, hp, ovf);
           else
-            pair_group8_f32<WCAP, false>(xi, yi, zi, sx, sy, sz, ng, bin, inv_bin32, nhist, B, wlim, D,
-                                         off, win, hp, ovf);
+            pair_group8_f32<WCAP, false, FQ>(xi, yi, zi, sx, sy, sz, ng, bin, inv_bin32, yb, nhist, B,
+                                             wlim, D, off, win, hp, ovf);
           pair_block<T, WCAP, false>(xi, yi, zi, tid, sx + ng, sy + ng, sz + ng, cn - ng, s0 + ng, bin,
                                      inv_bin32, td.band, nhist, B, wlim, D, win, hp, ovf);
         }
@@ -1181,13 +1222,46 @@ static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wc
     else
       hipLaunchKernelGGL((hist_kernel_f64<32768>), grid, block, 0, ctx->stream, A);
   } else {
-    if (wcap == 8192)
-      hipLaunchKernelGGL((hist_kernel_f32<8192>), grid, block, 0, ctx->stream, A);
-    else
-      hipLaunchKernelGGL((hist_kernel_f32<32768>), grid, block, 0, ctx->stream, A);
+    if (wcap == 8192) {
+      if (A.quot_f32)
+        hipLaunchKernelGGL((hist_kernel_f32<8192, true>), grid, block, 0, ctx->stream, A);
+      else
+        hipLaunchKernelGGL((hist_kernel_f32<8192, false>), grid, block, 0, ctx->stream, A);
+    } else {
+      if (A.quot_f32)
+        hipLaunchKernelGGL((hist_kernel_f32<32768, true>), grid, block, 0, ctx->stream, A);
+      else
+        hipLaunchKernelGGL((hist_kernel_f32<32768, false>), grid, block, 0, ctx->stream, A);
+    }
   }
   XS_HIP_CHECK(hipGetLastError());
   if (prof) XS_TRY(prof_end(ctx, "hist", ev));
+  return XS_OK;
+}
+
+// Whether FP32 binning may take quot_f32 for this bin: quot_f32_check_kernel
+// over all 2^23 significands, once per bin and context (synchronises).
+static int quot_f32_exact(xs_ctx *ctx, double bin_d, bool *ok) {
+  *ok = false;
+  const float bin = (float)bin_d;
+  if (!(bin >= 0x1p-60f && bin <= 0x1p60f)) return XS_OK;
+  uint32_t key;
+  std::memcpy(&key, &bin, sizeof key);
+  const auto it = ctx->quot_f32_ok.find(key);
+  if (it != ctx->quot_f32_ok.end()) {
+    *ok = it->second;
+    return XS_OK;
+  }
+  if (!ctx->quot_check) XS_HIP_CHECK(hipMalloc((void **)&ctx->quot_check, 8));
+  XS_TRY(ensure_pinned(ctx, 64));
+  hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->quot_check, (size_t)1);
+  hipLaunchKernelGGL(quot_f32_check_kernel, dim3(1u << 15), dim3(256), 0, ctx->stream, bin,
+                     1.0 / (double)bin, ctx->quot_check);
+  XS_HIP_CHECK(hipGetLastError());
+  XS_HIP_CHECK(hipMemcpyAsync(ctx->pin, ctx->quot_check, 8, hipMemcpyDeviceToHost, ctx->stream));
+  XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  *ok = *static_cast<const unsigned long long *>(ctx->pin) == 0;
+  ctx->quot_f32_ok[key] = *ok;
   return XS_OK;
 }
 
@@ -1332,7 +1406,7 @@ static int eq_tiles_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *
 template <class T>
 static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_count,
                       unsigned n_el, double bin, unsigned nhist, unsigned long long *hist_d,
-                      unsigned long long *ovf_user, bool equal_count = true) {
+                      unsigned long long *ovf_user, bool quot, bool equal_count = true) {
   const unsigned ntypes = ntypes_of(n_el);
   const size_t hsize = (size_t)ntypes * nhist;
   const uint32_t wcap = wcap_for(nhist);
@@ -1474,17 +1548,18 @@ static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_co
         if (bx[6 * t + 3 + a] - bx[6 * t + a] > cap) wide = true;
     if (wide) {
       XS_TRY(prof_end(ctx, "hist_prep", ev));
-      return run_device<T>(ctx, xyz, n, el_count, n_el, bin, nhist, hist_d, ovf_user, false);
+      return run_device<T>(ctx, xyz, n, el_count, n_el, bin, nhist, hist_d, ovf_user, quot, false);
     }
   }
   XS_TRY(prof_end(ctx, "hist_prep", ev));
   Args<T> A{sxp, syp, szp, tiles_d, boxes, types_d, items_d, hist_d, ovf_d};
+  A.quot_f32 = quot;
   return launch_hist<T>(ctx, A, items.size(), wcap, true);
 }
 
 template <class T>
 static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_count, unsigned n_el,
-                    double bin, unsigned nhist, uint64_t *hist_out, uint64_t *ovf_out) {
+                    double bin, unsigned nhist, uint64_t *hist_out, uint64_t *ovf_out, bool quot) {
   const unsigned ntypes = ntypes_of(n_el);
   const size_t hsize = (size_t)ntypes * nhist;
   {
@@ -1536,6 +1611,7 @@ static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_coun
   hipLaunchKernelGGL(fill_u64, dim3(std::max(zb, 1u)), dim3(256), 0, ctx->stream, hist_d, hsize);
   hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, ctx->stream, ovf_d, (size_t)1);
   Args<T> A{sorted, sorted + n, sorted + 2 * n, tiles_d, boxes_d, types_d, items_d, hist_d, ovf_d};
+  A.quot_f32 = quot;
   XS_TRY(launch_hist<T>(ctx, A, items.size(), wcap_for(nhist), true));
   XS_HIP_CHECK(hipMemcpyAsync(hist_out, hist_d, hsize * 8, hipMemcpyDeviceToHost, ctx->stream));
   XS_HIP_CHECK(hipMemcpyAsync(ctx->pin, ovf_d, 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -1564,10 +1640,11 @@ static void free_plan(xs_batch *b) {
 template <class T>
 static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const unsigned *el_count,
                       const double *const *xyz, double bin, const unsigned *nhist_in, bool f64,
-                      xs_batch **out) {
+                      bool quot, xs_batch **out) {
   auto *b = new xs_batch();
   b->ctx = ctx;
   b->f64 = f64;
+  b->quot_f32 = quot;
   b->bin = bin;
   b->nstruct = nstruct;
   std::vector<T> sx, sy, sz, tmp;
@@ -1703,6 +1780,7 @@ static int plan_hist(xs_batch *b, unsigned long long *hist_d, bool prof) {
             items,
             hist_d,
             b->ovf};
+  A.quot_f32 = b->quot_f32;
   XS_TRY(launch_hist<T>(ctx, A, b->items8192, 8192, prof));
   A.items = items + b->items8192;
   XS_TRY(launch_hist<T>(ctx, A, b->items32768, 32768, prof));
@@ -1751,18 +1829,20 @@ extern "C" int xs_hist(xs_ctx *ctx, const void *xyz, size_t n_atoms, const unsig
   }
   auto *h = reinterpret_cast<unsigned long long *>(hist);
   auto *o = reinterpret_cast<unsigned long long *>(overflow);
+  bool quot = false;
+  if (!f64 && !(flags & XS_F32_DOUBLE_QUOTIENT)) XS_TRY(hist::quot_f32_exact(ctx, bin, &quot));
   if (device) {
     if (f64)
       return hist::run_device<double>(ctx, static_cast<const double *>(xyz), n_atoms, el_count, n_el,
-                                      bin, nhist, h, o);
+                                      bin, nhist, h, o, false);
     return hist::run_device<float>(ctx, static_cast<const float *>(xyz), n_atoms, el_count, n_el, bin,
-                                   nhist, h, o);
+                                   nhist, h, o, quot);
   }
   if (f64)
     return hist::run_host<double>(ctx, static_cast<const double *>(xyz), n_atoms, el_count, n_el, bin,
-                                  nhist, hist, overflow);
+                                  nhist, hist, overflow, false);
   return hist::run_host<float>(ctx, static_cast<const float *>(xyz), n_atoms, el_count, n_el, bin,
-                               nhist, hist, overflow);
+                               nhist, hist, overflow, quot);
 }
 
 extern "C" int xs_batch_plan(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el,
@@ -1776,8 +1856,10 @@ extern "C" int xs_batch_plan(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el
   for (unsigned s = 0; s < nstruct; ++s)
     if (!xyz[s]) return fail(XS_ERR_ARG, "xs_batch_plan: null structure");
   if (flags & XS_F64_BINNING)
-    return hist::plan_build<double>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, true, plan);
-  return hist::plan_build<float>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, false, plan);
+    return hist::plan_build<double>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, true, false, plan);
+  bool quot = false;
+  if (!(flags & XS_F32_DOUBLE_QUOTIENT)) XS_TRY(hist::quot_f32_exact(ctx, bin, &quot));
+  return hist::plan_build<float>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, false, quot, plan);
 }
 
 extern "C" int xs_batch_free(xs_batch *plan) {

@@ -61,6 +61,7 @@ struct BatchRunState {
 struct xs_batch {
   xs_ctx *ctx = nullptr;
   bool f64 = true;
+  bool quot_f32 = false;  // FP32 binning with the checked FP32 quotient form
   double bin = 0;
   unsigned nstruct = 0;
   std::vector<unsigned> n_el, el_count, nhist, ntot;
@@ -114,6 +115,11 @@ struct xs_ctx {
   std::set<xs_batch *> plans;
   // Built ensembles on this context (freed by xs_free if still alive).
   std::set<xs_atoms *> atom_sets;
+
+  // FP32 binning: bins (float bits) checked for the FP32 quotient form, and
+  // the device counter of that check (8 bytes, allocated on first use).
+  std::map<uint32_t, bool> quot_f32_ok;
+  unsigned long long *quot_check = nullptr;
 
   bool profiling = false;
   std::vector<xs::ProfRecord> prof_pending;

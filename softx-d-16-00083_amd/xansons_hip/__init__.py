@@ -28,6 +28,7 @@ XS_F64_BINNING = 0x2
 XS_PDF_CPU_RADIUS = 0x4
 XS_BATCH_GRAPH = 0x8
 XS_PAIRSUM_MFMA = 0x10
+XS_F32_DOUBLE_QUOTIENT = 0x20
 PDF_TYPES = {"RDF": 0, "PDF": 1, "rPDF": 2}
 
 _lib = None
@@ -150,23 +151,27 @@ class Context:
         return ms.value, n.value
 
     # ---------------------------------------------------------------- H1
-    def hist(self, xyz, el_count, bin_: float, nhist: int, f64: bool = False):
+    def hist(self, xyz, el_count, bin_: float, nhist: int, f64: bool = False,
+             double_quotient: bool = False):
         """Histogram from host coordinates (3, N).  f64=True bins in double as
-        the reference CPU path (xyz promoted to float64), else in float."""
+        the reference CPU path (xyz promoted to float64), else in float;
+        double_quotient keeps the FP32 binning's quotient as a rounded double
+        product (same counts)."""
         ec = _u32(el_count)
         xyz = np.ascontiguousarray(xyz, dtype=np.float64 if f64 else np.float32)
         n = int(xyz.shape[-1]) if xyz.ndim > 1 else xyz.size // 3
         h = np.zeros(ntypes(len(ec)) * int(nhist), dtype=np.uint64)
         ov = np.zeros(1, dtype=np.uint64)
-        flags = XS_F64_BINNING if f64 else 0
+        flags = (XS_F64_BINNING if f64 else 0) | (XS_F32_DOUBLE_QUOTIENT if double_quotient else 0)
         _check(lib().xs_hist(self.h, _p(xyz), n, _p(ec), len(ec), float(bin_), int(nhist), _p(h),
                              _p(ov), flags))
         return h.reshape(-1, int(nhist)), int(ov[0])
 
     def hist_dev(self, xyz_ptr: int, n: int, el_count, bin_: float, nhist: int, hist_ptr: int,
-                 ovf_ptr: Optional[int] = None, f64: bool = False):
+                 ovf_ptr: Optional[int] = None, f64: bool = False, double_quotient: bool = False):
         ec = _u32(el_count)
-        flags = XS_DEVICE_PTRS | (XS_F64_BINNING if f64 else 0)
+        flags = XS_DEVICE_PTRS | (XS_F64_BINNING if f64 else 0) | \
+            (XS_F32_DOUBLE_QUOTIENT if double_quotient else 0)
         _check(lib().xs_hist(self.h, C.c_void_p(xyz_ptr), int(n), _p(ec), len(ec), float(bin_),
                              int(nhist), C.c_void_p(hist_ptr),
                              C.c_void_p(ovf_ptr) if ovf_ptr else None, flags))
@@ -299,7 +304,8 @@ class Batch:
     ``structures`` is a list of (xyz (3, N) array, el_count) pairs, each
     element-major.  ``f64`` bins in double as the reference CPU path."""
 
-    def __init__(self, ctx: Context, structures, bin_: float, nhist=None, f64: bool = True):
+    def __init__(self, ctx: Context, structures, bin_: float, nhist=None, f64: bool = True,
+                 double_quotient: bool = False):
         self.ctx = ctx
         self.bin = float(bin_)
         n_el = _u32([len(ec) for _, ec in structures])
@@ -308,7 +314,7 @@ class Batch:
         ptrs = (C.c_void_p * len(structures))(*[x.ctypes.data for x in self._xyz])
         nh = None if nhist is None else _u32(nhist)
         self.h = C.c_void_p()
-        flags = XS_F64_BINNING if f64 else 0
+        flags = (XS_F64_BINNING if f64 else 0) | (XS_F32_DOUBLE_QUOTIENT if double_quotient else 0)
         _check(lib().xs_batch_plan(ctx.h, len(structures), _p(n_el), _p(ecs), ptrs, self.bin,
                                    _p(nh), flags, C.byref(self.h)))
         self._xyz = None

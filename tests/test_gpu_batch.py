@@ -83,9 +83,13 @@ def test_batch_f64_matches_oracle(bctx, ensemble):
     b.close()
 
 
-def test_batch_f32_matches_oracle(bctx, ensemble):
+@pytest.mark.parametrize("double_quotient", [False, True])
+def test_batch_f32_matches_oracle(bctx, ensemble, double_quotient):
+    """FP32 binning over the ensemble, with the checked FP32 quotient form and
+    with the double product: both equal the oracle."""
     import xansons_hip as xh
-    b = xh.Batch(bctx, [(e.xyz, e.el_count) for e in ensemble], BIN, f64=False)
+    b = xh.Batch(bctx, [(e.xyz, e.el_count) for e in ensemble], BIN, f64=False,
+                 double_quotient=double_quotient)
     hist, _, _ = b.run()
     for s, (e, (h, _, _)) in enumerate(zip(ensemble, split(b, ensemble, hist, None, None, 0))):
         ho, _ = O.hist_f32(e.xyz, e.el_count, BIN, int(b.nhist[s]))

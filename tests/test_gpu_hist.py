@@ -124,6 +124,37 @@ def test_f32_root_rounding_one_ulp_bins(ctx):
     assert_same((g, ov, o, oo), "one-ulp bins f32")
 
 
+@pytest.mark.parametrize("double_quotient", [False, True])
+def test_f32_quotient_forms(ctx, double_quotient):
+    """FP32 binning with the FP32 quotient form (taken when the bin passes
+    the device's check over all significands) and with the double product
+    (XS_F32_DOUBLE_QUOTIENT): both equal the oracle bit for bit on bin-edge
+    distances (integer lattice) and a jittered rock-salt block with
+    vacancies on the host path, and on a sphere on the device path."""
+    import torch
+    r = np.arange(12, dtype=np.float64)
+    g = np.stack(np.meshgrid(r, r, r, indexing="ij"), axis=0).reshape(3, -1)
+    lat = st.Ensemble(np.ascontiguousarray(g.astype(np.float32)), [g.shape[1] // 2, g.shape[1] - g.shape[1] // 2],
+                      ["A", "B"])
+    cl = st.rocksalt_cube(ncell=8, vacancy=0.05, sigma=0.1, seed_vac=41, seed_jit=42)
+    for ens, b in ((lat, 0.5), (lat, 0.1), (cl, 0.0123), (cl, 1.0 / 3.0)):
+        nh = O.hist_size(ens.xyz64(), b)
+        gh, gov = ctx.hist(ens.xyz, ens.el_count, b, nh, double_quotient=double_quotient)
+        assert_same((gh, gov, *O.hist_f32(ens.xyz, ens.el_count, b, nh)), f"quotient {double_quotient} bin {b}")
+    ens = st.fcc_sphere(radius=22.0, seed=6)
+    nh = O.hist_size(ens.xyz64(), 0.004)
+    dev = torch.device("cuda", 0)
+    x = torch.from_numpy(np.ascontiguousarray(ens.xyz)).to(dev)
+    h = torch.empty(nh, dtype=torch.int64, device=dev)
+    ov = torch.empty(1, dtype=torch.int64, device=dev)
+    ctx.hist_dev(x.data_ptr(), ens.n, ens.el_count, 0.004, nh, h.data_ptr(), ov.data_ptr(),
+                 double_quotient=double_quotient)
+    ctx.synchronize()
+    ho, oo = O.hist_f32(ens.xyz, ens.el_count, 0.004, nh)
+    assert np.array_equal(h.cpu().numpy().astype(np.uint64), ho[0])
+    assert int(ov.item()) == oo
+
+
 def test_device_pointers_match_host(ctx):
     import torch
     ens = st.fcc_sphere(radius=12.0, seed=4)
