@@ -51,6 +51,12 @@ OPS_PER_PAIR = 11                 # 3 sub, 3 mul, 2 add, sqrt, div, cvt
 # spends issuing VALU work is SLOTS_PER_PAIR * pairs / 64 / (256 * 2.4e9 * t).
 SLOTS_PER_PAIR = 17.69
 CU_ISSUE_PER_S = 256 * 2.4e9
+# FP32 binning kernel (hist_kernel_f32<32768, true>) on the same workload:
+# SQ_ACTIVE_INST_VALU reads 23.86 wave-instructions per 64 pairs.  MI355X's
+# FP32 vector peak (157.3 TFLOP/s) is two 64-lane FP32 instructions per CU
+# per cycle at 2.4 GHz, so the kernel's share of that issue rate is
+# F32_ACTIVE_PER_64_PAIRS * pairs / 64 / (2 * 256 * 2.4e9 * t).
+F32_ACTIVE_PER_64_PAIRS = 23.86
 # HBM bytes per launch of hist_kernel_f64<32768> on this workload, from
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch), scaled by
 # the counters' reading for a 1 GiB device-to-device copy (FETCH_SIZE reads
@@ -812,6 +818,7 @@ def main():
     del step32, hist32
 
     hist_ms = kern["hist"][0] / max(kern["hist"][1], 1)
+    hist32_ms = kern32["hist"][0] / max(kern32["hist"][1], 1)
     achieved = OPS_PER_PAIR * pairs / (hist_ms * 1e-3) / 1e12
     line = {
         "metric": METRIC,
@@ -845,7 +852,11 @@ def main():
         "kernels_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in kern.items()},
         "f32_binning": {"ms_per_step": round(ms32, 3),
                         "gpairs_per_s": round(pairs / (ms32 * 1e-3) / 1e9, 3),
-                        "hist_kernel_ms": round(kern32["hist"][0] / max(kern32["hist"][1], 1), 3)},
+                        "hist_kernel_ms": round(hist32_ms, 3),
+                        "kernel": "hist_kernel_f32<32768, true>",
+                        "valu_active_per_64_pairs": F32_ACTIVE_PER_64_PAIRS,
+                        "valu_issue_frac_of_fp32_peak": round(
+                            F32_ACTIVE_PER_64_PAIRS * pairs / 64 / (2 * CU_ISSUE_PER_S * hist32_ms * 1e-3), 4)},
         "checks": {"pair_totals_exact": totals_ok, "overflow": int(ovf.item())},
     }
     progress(f"headline done: {line['value']} Gpairs/s")
