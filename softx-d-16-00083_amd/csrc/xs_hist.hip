@@ -44,38 +44,16 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include "xs_hist_plan.h"
 #include "xs_internal.h"
 
 namespace xs {
 namespace hist {
 
-constexpr int kTile = 1024;      // atoms per tile == threads per work group
 constexpr int kStage = 256;      // J atoms staged in LDS per step
-constexpr int kSortCap = 2048;   // J tiles sorted per work item
-constexpr int kMaxEl = 16;
 constexpr int kBoxBlocks = 512;  // partial bounding boxes
 constexpr double kFill = 0.94 * kTile;  // target atoms in the fullest cell
 constexpr uint32_t kMaxCellsPerAxis = 256;
-
-struct Tile {
-  uint32_t off, cnt;  // atom range in the sorted arrays
-};
-
-// One pair type of one structure.
-struct TypeDesc {
-  unsigned long long hist_off;  // offset of its nhist bins in the output
-  uint32_t nhist;
-  uint32_t a_base, a_n;         // I tiles: global tile index base and count
-  uint32_t b_base, b_n;         // J tiles
-  uint32_t same;                // like-element pair type: j > i within a tile
-  uint32_t ovf;                 // overflow counter index
-  uint32_t pad;
-  double bin, inv_bin, band;
-};
-
-struct Item {
-  uint32_t type, ti, jstart, jcount;
-};
 
 struct Grid {
   double lo[3];
@@ -97,17 +75,26 @@ struct Args {
 
 // ---------------------------------------------------------------- prep kernels
 
+// Bounding boxes.  fmin / fmax drop a NaN operand, so a NaN coordinate is
+// tracked by its own flag and turns the whole box into NaN, which the host's
+// finiteness check (check_span) rejects before any pair kernel runs.
 template <class T>
 __global__ void hist_bbox_partial(const T *__restrict__ x, const T *__restrict__ y,
                                   const T *__restrict__ z, size_t n, double *__restrict__ part) {
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  int nan = 0;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
     const double v[3] = {(double)x[i], (double)y[i], (double)z[i]};
     for (int a = 0; a < 3; ++a) {
+      nan |= v[a] != v[a];
       lo[a] = fmin(lo[a], v[a]);
       hi[a] = fmax(hi[a], v[a]);
     }
+  }
+  if (__syncthreads_or(nan)) {
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = NAN;
+    return;
   }
   __shared__ double s[6][256];
   for (int a = 0; a < 3; ++a) {
@@ -130,11 +117,18 @@ __global__ void hist_bbox_reduce(const double *__restrict__ part, int nparts,
                                  double *__restrict__ bbox) {
   __shared__ double s[6][256];
   double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  int nan = 0;
   for (int i = threadIdx.x; i < nparts; i += blockDim.x)
     for (int a = 0; a < 3; ++a) {
-      v[a] = fmin(v[a], part[i * 6 + a]);
-      v[3 + a] = fmax(v[3 + a], part[i * 6 + 3 + a]);
+      const double l = part[i * 6 + a], h = part[i * 6 + 3 + a];
+      nan |= (l != l) | (h != h);
+      v[a] = fmin(v[a], l);
+      v[3 + a] = fmax(v[3 + a], h);
     }
+  if (__syncthreads_or(nan)) {
+    if (threadIdx.x < 6) bbox[threadIdx.x] = NAN;
+    return;
+  }
   for (int a = 0; a < 6; ++a) s[a][threadIdx.x] = v[a];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
@@ -327,15 +321,19 @@ __device__ __forceinline__ void rotate_window(uint32_t *win, uint32_t B, uint32_
 // Bin index of a squared distance s.
 // double: the reference CPU expression (unsigned)(sqrt(s)/bin) in IEEE double.
 //   Fast path: sqrt(s)/bin estimated with v_rsq_f64 and one Newton step times
-//   1/bin.  Measured on gfx950 over 2^24 arguments spanning 20 decades,
-//   v_rsq_f64 is within 2^-24.2 and the Newton-refined square root within
-//   2^-47.8 relative; with the roundings of s, of the product by 1/bin and of
-//   the IEEE sqrt and division, estimate and IEEE quotient differ by less than
-//   2^-46 relative, i.e. nhist*2^-46 absolute.  So when the estimate's
-//   fractional part is more than nhist*2^-42 (16x that) away from an integer,
-//   its floor is the IEEE result.  Otherwise -- about nhist*2^-41 of the
-//   pairs, and every NaN -- the IEEE sqrt and division are evaluated.
-//   `band` = 1/2 - nhist*2^-42 - 2^-50.
+//   1/bin.  The ISA reference guides of CDNA3 and CDNA4 document V_RSQ_F64 to
+//   (2**29) ULP, i.e. y = (1 + e)/sqrt(s) with |e| <= 2^29 2^-52 = 2^-23
+//   (relative to y's own ulp, which is at most 2^-52 y).  One Newton step
+//   g (3 - t)/2, g = s y, t = g y, leaves sqrt(s)(1 + e)(1 - e - e^2/2)
+//   = sqrt(s)(1 - 3e^2/2 - e^3/2): relative error below 1.5 2^-46 = 2^-45.42.
+//   Its four FP64 roundings (g, t, the FMA, the product) add below 2^-50.4,
+//   the IEEE sqrt and division of the reference below 2^-52, so estimate and
+//   IEEE quotient differ by less than 2^-45.3 relative, nhist 2^-45.3 bins.
+//   (Measured on gfx950 over 2^24 arguments: v_rsq_f64 within 2^-24.2.)
+//   When the estimate's fractional part is more than nhist 2^-41 (over 16x
+//   that) away from an integer, its floor is the IEEE result.  Otherwise --
+//   about nhist 2^-40 of the pairs, and every NaN -- the IEEE sqrt and
+//   division are evaluated.  `band` = 1/2 - nhist 2^-41 - 2^-50 (band_for).
 //   The estimate stays below 2^31: the host rejects structures whose
 //   bounding-box diagonal spans 2^31 bins.
 // float: the IEEE single-precision expression of the reference GPU kernel,
@@ -427,6 +425,42 @@ __global__ void quot_f32_check_kernel(float bin, double inv_bin, unsigned long l
   if (quot_f32(r, bin, recip_f32(bin)) != (float)((double)r * inv_bin)) atomicAdd(bad, 1ull);
 }
 
+// Exhaustive check of sqrt_rn_f32 against the IEEE root, one pass over every
+// positive finite float s.  y is the IEEE root iff m-^2 < s < m+^2 with m-,
+// m+ the midpoints between y and its float neighbours (a float s is never a
+// midpoint's square: that square has more than 24 significant bits).  m+-
+// have 25 significant bits and y >= 2^-75, so m+-^2 is exact in double and
+// the test is exact.  bad[0] counts wrong roots for s >= 2^-96; bad[1] counts
+// s < 2^-96 whose result is not in [0, 2^-47); bad[2] wrong roots for
+// s < 2^-96.  The FP32 binning relies on bad[0] = bad[1] = 0 and, unless
+// bad[2] = 0 too, on bins of at least 2^-40 (then every s < 2^-96 bins to 0
+// with the IEEE root and with sqrt_rn_f32 alike).
+__global__ void sqrt_f32_check_kernel(unsigned long long *bad) {
+  unsigned long long b0 = 0, b1 = 0, b2 = 0;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t u = 1u + blockIdx.x * blockDim.x + threadIdx.x; u < 0x7f800000u; u += stride) {
+    const float s = __uint_as_float(u);
+    const float y = sqrt_rn_f32(s);
+    const uint32_t yb = __float_as_uint(y);
+    bool ok = false;
+    if (y > 0.0f && y < INFINITY) {
+      const double yd = y;
+      const double lo = 0.5 * (yd + (double)__uint_as_float(yb - 1u));
+      const double hi = 0.5 * (yd + (double)__uint_as_float(yb + 1u));
+      ok = lo * lo < (double)s && (double)s < hi * hi;
+    }
+    if (u >= 0x0f800000u) {  // s >= 2^-96
+      b0 += !ok;
+    } else {
+      b1 += !(y >= 0.0f && y < 0x1p-47f);
+      b2 += !ok;
+    }
+  }
+  if (b0) atomicAdd(&bad[0], b0);
+  if (b1) atomicAdd(&bad[1], b1);
+  if (b2) atomicAdd(&bad[2], b2);
+}
+
 template <>
 __device__ __forceinline__ uint32_t bin_index<float>(float s, float bin, double inv_bin,
                                                      double band, uint32_t nhist) {
@@ -460,7 +494,7 @@ struct IAtom {
 
 // Centre c of an I box (bi, 6 doubles) and the half-width of the FP64 fast
 // path's band for I atoms in that box taken relative to c.
-// Relative to IEEE sqrt(s)/bin the estimate errs by (a) below nhist 2^-46
+// Relative to IEEE sqrt(s)/bin the estimate errs by (a) below nhist 2^-45.3
 // bins from v_rsq_f64, the Newton step and the roundings of the reference
 // expression, and (b) below u (64 R^2 + 36 R + 10 T) bins from forming s as
 // |ri'|^2 + |rj'|^2 - 2 ri'.rj' with shifted, rounded coordinates, where
@@ -1000,10 +1034,25 @@ static void cell_tiles(uint32_t pos, uint32_t c, std::vector<Tile> &tiles) {
   }
 }
 
-static uint32_t wcap_for(uint32_t nhist) { return nhist <= 8192 ? 8192u : 32768u; }
-
-static double band_for(uint32_t nhist) {
-  return 0.5 - std::ldexp((double)nhist, -42) - std::ldexp(1.0, -50);
+// Bounding box of n atoms (SoA xyz) on the host.  std::min / std::max drop a
+// NaN, so NaN is tested for explicitly and makes the whole box NaN, which
+// check_span rejects.
+template <class T>
+static void host_bbox(const T *xyz, size_t n, double bbox[6]) {
+  bool nan = false;
+  for (int a = 0; a < 3; ++a) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (size_t i = 0; i < n; ++i) {
+      const double v = (double)xyz[a * n + i];
+      nan |= v != v;
+      lo = std::min(lo, v);
+      hi = std::max(hi, v);
+    }
+    bbox[a] = lo;
+    bbox[3 + a] = hi;
+  }
+  if (nan)
+    for (int a = 0; a < 6; ++a) bbox[a] = NAN;
 }
 
 static int check_span(const double bbox[6], double bin) {
@@ -1014,93 +1063,6 @@ static int check_span(const double bbox[6], double bin) {
   if (std::sqrt(d2) / bin >= 2147483648.0)
     return fail(XS_ERR_RANGE, "histogram: the structure spans more than 2^31 bins");
   return XS_OK;
-}
-
-// Work items of pair types [first, last) (appended).  The I side of an item
-// is one tile, or with `paired` (FP64 binning) the tile pair {2 ti, 2 ti + 1}
-// (an I unit).  Like-element pairs: I unit ti with itself and the next h
-// units cyclically (h = (n-1)/2, or n/2 for the first half when n is even,
-// n = units), so each unordered unit pair is visited once and items have
-// equal length; the J run is the tiles of those units.  Cross-element pairs:
-// each I unit against the J tiles in parts.  Paired items are split twice as
-// finely, so that there are as many items as unpaired ones.
-static void add_items(const std::vector<TypeDesc> &types, uint32_t first, uint32_t last,
-                      std::vector<Item> &items, bool paired) {
-  for (uint32_t t = first; t < last; ++t) {
-    const TypeDesc &d = types[t];
-    if (!d.a_n || !d.b_n) continue;
-    if (d.same) {
-      const uint32_t nt = d.a_n;
-      const uint32_t nu = paired ? (nt + 1) / 2 : nt;
-      for (uint32_t u = 0; u < nu; ++u) {
-        const uint32_t h = (nu & 1) ? (nu - 1) / 2 : (u < nu / 2 ? nu / 2 : nu / 2 - 1);
-        uint32_t start = u, total = h + 1;
-        if (paired) {
-          start = 2 * u;
-          total = 2 * (h + 1);
-          // the last unit holds one tile when nt is odd
-          if ((nt & 1) && (nu - 1 + nu - u) % nu <= h) --total;
-        }
-        const uint32_t chunk =
-            std::min<uint32_t>(paired ? (total + 3) / 4 : total, (uint32_t)kSortCap);
-        for (uint32_t c = 0; c < total; c += chunk)
-          items.push_back(Item{t, u, (start + c) % nt, std::min<uint32_t>(chunk, total - c)});
-      }
-    } else {
-      const uint32_t nu = paired ? (d.a_n + 1) / 2 : d.a_n;
-      const uint32_t parts = std::min<uint32_t>(d.b_n, paired ? 8u : 2u);
-      const uint32_t per = std::min<uint32_t>((d.b_n + parts - 1) / parts, kSortCap);
-      for (uint32_t u = 0; u < nu; ++u)
-        for (uint32_t j0 = 0; j0 < d.b_n; j0 += per)
-          items.push_back(Item{t, u, j0, std::min(per, d.b_n - j0)});
-    }
-  }
-}
-
-// Longest first (estimated pairs: I unit atoms x atoms of its J run).
-static void order_items(std::vector<Item> &items, const std::vector<TypeDesc> &types,
-                        const std::vector<Tile> &tiles, bool paired) {
-  std::vector<std::pair<double, uint32_t>> cost(items.size());
-  for (size_t i = 0; i < items.size(); ++i) {
-    const Item &it = items[i];
-    const TypeDesc &d = types[it.type];
-    double jn = 0;
-    for (uint32_t m = 0; m < it.jcount; ++m) jn += tiles[d.b_base + (it.jstart + m) % d.b_n].cnt;
-    double in = 0;
-    if (paired) {
-      for (uint32_t a = 2 * it.ti; a < std::min(2 * it.ti + 2, d.a_n); ++a) in += tiles[d.a_base + a].cnt;
-    } else {
-      in = tiles[d.a_base + it.ti].cnt;
-    }
-    cost[i] = {-in * jn, (uint32_t)i};
-  }
-  std::stable_sort(cost.begin(), cost.end());
-  std::vector<Item> out(items.size());
-  for (size_t i = 0; i < items.size(); ++i) out[i] = items[cost[i].second];
-  items.swap(out);
-}
-
-// Type descriptors of one structure whose element e has tiles [ebase[e], ebase[e]+en[e]).
-static void add_types(unsigned n_el, const uint32_t *ebase, const uint32_t *en, uint32_t nhist,
-                      double bin, unsigned long long hist_off, uint32_t ovf,
-                      std::vector<TypeDesc> &types) {
-  unsigned long long off = hist_off;
-  for (unsigned a = 0; a < n_el; ++a)
-    for (unsigned b = a; b < n_el; ++b, off += nhist) {
-      TypeDesc d{};
-      d.hist_off = off;
-      d.nhist = nhist;
-      d.a_base = ebase[a];
-      d.a_n = en[a];
-      d.b_base = ebase[b];
-      d.b_n = en[b];
-      d.same = a == b;
-      d.ovf = ovf;
-      d.bin = bin;
-      d.inv_bin = 1.0 / bin;
-      d.band = band_for(nhist);
-      types.push_back(d);
-    }
 }
 
 // Equal-count k-d tiles of one element (indices idx[0, ne) into x, y, z): the
@@ -1239,6 +1201,36 @@ static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wc
   return XS_OK;
 }
 
+static int check_counters(xs_ctx *ctx) {
+  if (!ctx->quot_check) XS_HIP_CHECK(hipMalloc((void **)&ctx->quot_check, 32));
+  return ensure_pinned(ctx, 64);
+}
+
+// FP32 binning computes the IEEE sqrtf with sqrt_rn_f32, whose correction
+// step needs v_sqrt_f32 within one ulp of the root's own binade.  Once per
+// context, sqrt_f32_check_kernel compares it with the IEEE root on every
+// positive float (about 2 ms, synchronises); binning in FP32 is refused if
+// any root from 2^-96 up differs, and for bins below 2^-40 unless the check
+// also held below 2^-96.
+static int sqrt_f32_checked(xs_ctx *ctx, double bin) {
+  if (ctx->sqrt_f32_state < 0) {
+    XS_TRY(check_counters(ctx));
+    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->quot_check, (size_t)4);
+    hipLaunchKernelGGL(sqrt_f32_check_kernel, dim3(8192), dim3(256), 0, ctx->stream, ctx->quot_check);
+    XS_HIP_CHECK(hipGetLastError());
+    XS_HIP_CHECK(hipMemcpyAsync(ctx->pin, ctx->quot_check, 24, hipMemcpyDeviceToHost, ctx->stream));
+    XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const auto *bad = static_cast<const unsigned long long *>(ctx->pin);
+    ctx->sqrt_f32_state = (bad[0] || bad[1]) ? 0 : (bad[2] ? 2 : 1);
+  }
+  if (ctx->sqrt_f32_state == 0)
+    return fail(XS_ERR_DEVICE, "FP32 binning: this device's v_sqrt_f32 fails the exhaustive check "
+                               "of the correctly rounded square root; use XS_F64_BINNING");
+  if (ctx->sqrt_f32_state == 2 && !((float)bin >= 0x1p-40f))
+    return fail(XS_ERR_RANGE, "FP32 binning: bins below 2^-40 need XS_F64_BINNING on this device");
+  return XS_OK;
+}
+
 // Whether FP32 binning may take quot_f32 for this bin: quot_f32_check_kernel
 // over all 2^23 significands, once per bin and context (synchronises).
 static int quot_f32_exact(xs_ctx *ctx, double bin_d, bool *ok) {
@@ -1252,8 +1244,7 @@ static int quot_f32_exact(xs_ctx *ctx, double bin_d, bool *ok) {
     *ok = it->second;
     return XS_OK;
   }
-  if (!ctx->quot_check) XS_HIP_CHECK(hipMalloc((void **)&ctx->quot_check, 8));
-  XS_TRY(ensure_pinned(ctx, 64));
+  XS_TRY(check_counters(ctx));
   hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->quot_check, (size_t)1);
   hipLaunchKernelGGL(quot_f32_check_kernel, dim3(1u << 15), dim3(256), 0, ctx->stream, bin,
                      1.0 / (double)bin, ctx->quot_check);
@@ -1266,72 +1257,6 @@ static int quot_f32_exact(xs_ctx *ctx, double bin_d, bool *ok) {
 }
 
 // ---------------------------------------------------------------- xs_hist
-
-// Equal-count partition of one element for the device path: gx slabs along
-// x and gy columns along y in each slab, all of equal count, so every column
-// holds ne / (gx gy) atoms (+-1) whatever the shape; columns are cut along z
-// into ceil(count / kTile) (at least nz) equal tiles.  gx, gy make tiles
-// about cubic and as full as possible; nx, ny, nz make slabs, columns and
-// tiles of a uniform-density structure narrower than 0.9 of the window cap
-// (the cap itself is checked on the tile boxes).
-static void choose_equal_count(uint32_t ne, const double bbox[6], double cap, bool paired,
-                               uint32_t &gx, uint32_t &gy, uint32_t &nz) {
-  double ext[3];
-  for (int a = 0; a < 3; ++a) ext[a] = std::max(bbox[3 + a] - bbox[a], 1e-6);
-  auto need = [&](double e) -> uint32_t {
-    return std::isfinite(cap) ? (uint32_t)std::max(1.0, std::ceil(e / (0.9 * cap))) : 1u;
-  };
-  const uint32_t nx = need(ext[0]), ny = need(ext[1]);
-  nz = need(ext[2]);
-  if (paired) nz += nz & 1u;
-  const double T = std::max(1.0, std::ceil(ne / (double)kTile));
-  const double L = std::cbrt(ext[0] * ext[1] * ext[2] / T);
-  const long cx = std::lround(ext[0] / L), cy = std::lround(ext[1] / L);
-  double best = -INFINITY;
-  gx = std::max(nx, 1u);
-  gy = std::max(ny, 1u);
-  for (long a = cx - 2; a <= cx + 2; ++a)
-    for (long b = cy - 2; b <= cy + 2; ++b) {
-      const uint64_t ga = std::max<uint64_t>(std::max<long>(a, 1), nx);
-      const uint64_t gb = std::max<uint64_t>(std::max<long>(b, 1), ny);
-      if (ga * gb > std::max<uint64_t>(ne, 1) || ga * gb > 65536) continue;
-      const double col = (double)ne / (double)(ga * gb);
-      double tpc = std::max(std::ceil(col / kTile), (double)nz);
-      if (paired) tpc = 2.0 * std::ceil(tpc / 2.0);
-      const double fill = ne / ((double)(ga * gb) * tpc * kTile);
-      const double shape = std::fabs(std::log(ext[0] / ga / L)) + std::fabs(std::log(ext[1] / gb / L)) +
-                           std::fabs(std::log(ext[2] / tpc / L));
-      const double score = fill - 0.01 * shape;
-      if (score > best) {
-        best = score;
-        gx = (uint32_t)ga;
-        gy = (uint32_t)gb;
-      }
-    }
-}
-
-// Tiles of the equal-count partition (positions from `pos`), column by column.
-// paired: an even number of runs per column, so that tiles 2u and 2u + 1 are
-// neighbouring runs of one column (the I tile pairs of the FP64 kernel).
-static void equal_count_tiles(uint32_t pos, uint32_t ne, uint32_t gx, uint32_t gy, uint32_t nz,
-                              bool paired, std::vector<Tile> &tiles) {
-  for (uint64_t sl = 0; sl < gx; ++sl) {
-    const uint64_t S = (sl + 1) * ne / gx - sl * ne / gx;
-    for (uint64_t c = 0; c < gy; ++c) {
-      const uint32_t C = (uint32_t)((c + 1) * S / gy - c * S / gy);
-      if (!C) continue;
-      uint32_t runs = std::max((C + kTile - 1) / (uint32_t)kTile, nz);
-      if (paired) runs += runs & 1u;
-      const uint32_t nt = std::min(C, runs);
-      for (uint32_t t = 0, o = 0; t < nt; ++t) {
-        const uint32_t len = (C - o) / (nt - t);
-        tiles.push_back(Tile{pos + o, len});
-        o += len;
-      }
-      pos += C;
-    }
-  }
-}
 
 // Scratch of eq_tiles_device for elements of up to maxne atoms.
 template <class T>
@@ -1563,12 +1488,8 @@ static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_coun
   const unsigned ntypes = ntypes_of(n_el);
   const size_t hsize = (size_t)ntypes * nhist;
   {
-    double bbox[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    for (int a = 0; a < 3; ++a)
-      for (size_t i = 0; i < n; ++i) {
-        bbox[a] = std::min(bbox[a], (double)xyz[a * n + i]);
-        bbox[3 + a] = std::max(bbox[3 + a], (double)xyz[a * n + i]);
-      }
+    double bbox[6];
+    host_bbox<T>(xyz, n, bbox);
     XS_TRY(check_span(bbox, bin));
   }
   std::vector<T> sx, sy, sz;
@@ -1667,12 +1588,7 @@ static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const
     }
     const double *X = xyz[s];
     double bbox[6];
-    for (int a = 0; a < 3; ++a) bbox[a] = bbox[3 + a] = X[a * n];
-    for (int a = 0; a < 3; ++a)
-      for (size_t i = 0; i < n; ++i) {
-        bbox[a] = std::min(bbox[a], X[a * n + i]);
-        bbox[3 + a] = std::max(bbox[3 + a], X[a * n + i]);
-      }
+    host_bbox<double>(X, n, bbox);
     const int rc = check_span(bbox, bin);
     if (rc != XS_OK) {
       free_plan(b);
@@ -1690,6 +1606,16 @@ static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const
     }
     tmp.resize(3 * n);
     for (size_t i = 0; i < 3 * n; ++i) tmp[i] = (T)X[i];
+    if (sizeof(T) < sizeof(double)) {
+      // coordinates beyond the float range become infinite in FP32 binning
+      double fb[6];
+      host_bbox<T>(tmp.data(), n, fb);
+      const int rf = check_span(fb, bin);
+      if (rf != XS_OK) {
+        free_plan(b);
+        return rf;
+      }
+    }
     uint32_t ebase[kMaxEl], en[kMaxEl];
     sort_structure_host<T>(tmp.data(), n, el_count + ec, ne, bin, nh, sx, sy, sz, tiles, boxes, ebase,
                            en);
@@ -1830,6 +1756,7 @@ extern "C" int xs_hist(xs_ctx *ctx, const void *xyz, size_t n_atoms, const unsig
   auto *h = reinterpret_cast<unsigned long long *>(hist);
   auto *o = reinterpret_cast<unsigned long long *>(overflow);
   bool quot = false;
+  if (!f64) XS_TRY(hist::sqrt_f32_checked(ctx, bin));
   if (!f64 && !(flags & XS_F32_DOUBLE_QUOTIENT)) XS_TRY(hist::quot_f32_exact(ctx, bin, &quot));
   if (device) {
     if (f64)
@@ -1858,6 +1785,7 @@ extern "C" int xs_batch_plan(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el
   if (flags & XS_F64_BINNING)
     return hist::plan_build<double>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, true, false, plan);
   bool quot = false;
+  XS_TRY(hist::sqrt_f32_checked(ctx, bin));
   if (!(flags & XS_F32_DOUBLE_QUOTIENT)) XS_TRY(hist::quot_f32_exact(ctx, bin, &quot));
   return hist::plan_build<float>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, false, quot, plan);
 }

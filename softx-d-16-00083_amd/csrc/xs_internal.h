@@ -116,9 +116,13 @@ struct xs_ctx {
   // Built ensembles on this context (freed by xs_free if still alive).
   std::set<xs_atoms *> atom_sets;
 
-  // FP32 binning: bins (float bits) checked for the FP32 quotient form, and
-  // the device counter of that check (8 bytes, allocated on first use).
+  // FP32 binning: bins (float bits) checked for the FP32 quotient form; the
+  // outcome of the exhaustive check of the FP32 square root (-1 not run yet,
+  // 0 failed, 1 IEEE for every positive float, 2 IEEE from 2^-96 up and
+  // below 2^-47 under it); the device counters of both checks (4 x 8 bytes,
+  // allocated on first use).
   std::map<uint32_t, bool> quot_f32_ok;
+  int sqrt_f32_state = -1;
   unsigned long long *quot_check = nullptr;
 
   bool profiling = false;

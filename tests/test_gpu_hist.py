@@ -265,3 +265,49 @@ def test_fp64_tile_centred_distances(ctx, case):
     assert ov_h == oo and ov_d == oo
     assert np.array_equal(host, want)
     assert np.array_equal(devh, want)
+
+
+@pytest.mark.parametrize("f64", [False, True])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_nan_coordinate_rejected(ctx, f64, axis):
+    """A NaN anywhere (not only in the first atom) is rejected with
+    XS_ERR_ARG before any pair kernel runs: host input, device input and
+    batch plans.  fmin/fmax and std::min/max would drop it from the
+    bounding box and let a NaN distance reach an unchecked LDS slot."""
+    import xansons_hip as xh
+    ens = st.random_cluster([300, 200], box=14.0, min_dist=0.5, seed=5)
+    xyz = ens.xyz64().copy()
+    xyz[axis, 337] = np.nan
+    nh = 4000
+    print(f"nan test f64={f64} axis={axis}", flush=True)
+    with pytest.raises(xh.XSError) as e:
+        ctx.hist(xyz, ens.el_count, 0.01, nh, f64=f64)
+    assert e.value.code == -1 and "non-finite" in str(e.value)
+    dev = torch.device("cuda", 0)
+    xd = torch.from_numpy(np.ascontiguousarray(xyz if f64 else xyz.astype(np.float32))).to(dev)
+    h = torch.zeros(3 * nh, dtype=torch.int64, device=dev)
+    with pytest.raises(xh.XSError) as e:
+        ctx.hist_dev(xd.data_ptr(), ens.n, ens.el_count, 0.01, nh, h.data_ptr(), f64=f64)
+    assert e.value.code == -1 and "non-finite" in str(e.value)
+    ok = ens.xyz64()
+    with pytest.raises(xh.XSError) as e:
+        xh.Batch(ctx, [(ok, ens.el_count), (xyz, ens.el_count), (ok, ens.el_count)], 0.01, f64=f64)
+    assert e.value.code == -1 and "non-finite" in str(e.value)
+    # the same structure without the NaN still runs and equals the oracle
+    g, _ = ctx.hist(ok, ens.el_count, 0.01, nh, f64=True)
+    o, _ = O.hist_f64(ok, ens.el_count, 0.01, nh)
+    assert np.array_equal(g, o)
+
+
+def test_f32_binning_overflowing_coordinates_rejected(ctx):
+    """Coordinates beyond the float range become infinite when a batch plan
+    (double input) rounds them to float for FP32 binning: rejected as
+    non-finite.  The bin of 1e31 A keeps the double span itself within 2^31
+    bins, so it is the float check that refuses."""
+    import xansons_hip as xh
+    ens = st.random_cluster([300], box=10.0, min_dist=0.5, seed=6)
+    xyz = ens.xyz64().copy()
+    xyz[0, 17] = 1e39
+    with pytest.raises(xh.XSError) as e:
+        xh.Batch(ctx, [(xyz, ens.el_count)], 1e31, f64=False)
+    assert e.value.code == -1 and "non-finite" in str(e.value)

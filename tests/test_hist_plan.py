@@ -1,0 +1,25 @@
+"""Host planning of the histogram kernels (no GPU): the work items keep every
+item below 2^32 pairs, the uint32 LDS slots' limit, and visit every tile
+pair exactly once.  The checks are C++ (tests/cpp/hist_plan_check.cpp)
+compiled with g++ against the library's own planning header."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "..", "softx-d-16-00083_amd", "csrc")
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+def test_item_tables_bound_and_coverage(tmp_path):
+    exe = tmp_path / "hist_plan_check"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-I", CSRC,
+                    os.path.join(HERE, "cpp", "hist_plan_check.cpp"), "-o", str(exe)],
+                   check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("ok")
+    # the longest paired runs are capped at 2047 tiles: 2 * 1024 * 2047 * 1024 pairs
+    assert "max_pairs=4292870144" in r.stdout
