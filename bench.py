@@ -52,11 +52,22 @@ OPS_PER_PAIR = 11                 # 3 sub, 3 mul, 2 add, sqrt, div, cvt
 SLOTS_PER_PAIR = 17.69
 CU_ISSUE_PER_S = 256 * 2.4e9
 # FP32 binning kernel (hist_kernel_f32<32768, true>) on the same workload:
-# SQ_ACTIVE_INST_VALU reads 23.86 wave-instructions per 64 pairs.  MI355X's
-# FP32 vector peak (157.3 TFLOP/s) is two 64-lane FP32 instructions per CU
-# per cycle at 2.4 GHz, so the kernel's share of that issue rate is
-# F32_ACTIVE_PER_64_PAIRS * pairs / 64 / (2 * 256 * 2.4e9 * t).
+# SQ_ACTIVE_INST_VALU reads 23.86 wave-instructions per 64 pairs.  Its
+# roofline fraction is the algorithmic one of the FP64 line: 11 ops per pair
+# x pairs / t against MI355X's FP32 vector peak (157.3 TFLOP/s).  Beside it,
+# the issue rate it reaches in wave-instructions per CU-cycle at 2.4 GHz:
+# non-packed v_fma_f32 issues at 1.93x the rate of v_fma_f64 on this GPU
+# (measured), i.e. up to two FP32 wave-instructions per CU-cycle against one
+# for FP64, so each kernel's issue rate is read against the peak of its own
+# instruction type (2 for this one, 1 for the FP64 kernel).
 F32_ACTIVE_PER_64_PAIRS = 23.86
+FP32_PEAK_TFLOPS = 157.3
+# Direct Debye and 2D units (one op per sin, cos, add, multiply): 4 ops per
+# pair-q term (sin, q r, x 1/(q r), accumulate) and 7 per complex-exponential
+# term (sin, cos, 3 for q.r, 2 accumulations).  Both kernels compute in FP64,
+# so the peak is the FP64 vector one.
+DEBYE_OPS_PER_PAIR_Q = 4
+AMP2D_OPS_PER_TERM = 7
 # HBM bytes per launch of hist_kernel_f64<32768> on this workload, from
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch), scaled by
 # the counters' reading for a 1 GiB device-to-device copy (FETCH_SIZE reads
@@ -173,11 +184,14 @@ def copy_probe(nbytes=1 << 30):
     print(json.dumps({"copy_probe_bytes": nbytes}))
 
 
-def ensemble_leg(ctx, count, steps, warmup):
+def ensemble_leg(ctx, count, steps, warmup, use_graph=True):
     """Batched (one plan, graph replays) vs structure-by-structure runs of the
     same kernels on `count` nanoparticles: histograms (FP64 binning), pair
     sums on 2,048 q and rPDF per particle.  Both write the same layout, which
-    is compared after the timed runs."""
+    is compared after the timed runs.  A fresh ensemble costs its plan (host
+    tiling and upload) plus one run; replays of the same coordinates cost the
+    run alone; both are reported.  use_graph=False replays eagerly (for
+    kernel traces)."""
     import torch
     import xansons_hip as xh
     from xansons_hip import structures as st2
@@ -196,8 +210,11 @@ def ensemble_leg(ctx, count, steps, warmup):
     rho = np.array([density(p.xyz64()) for p in parts])
     sl = np.concatenate([np.linspace(2.0, 9.0, p.n_el) for p in parts])
     bctx = xh.Context(0)          # own non-default stream (graph capture)
+    structs = [(np.ascontiguousarray(p.xyz64()), p.el_count) for p in parts]
+    # the plan of a fresh ensemble (host k-d tiling on worker threads, upload
+    # of the sorted coordinates and tables) is part of its first pattern
     t0 = time.perf_counter()
-    b = xh.Batch(bctx, [(p.xyz64(), p.el_count) for p in parts], bin_, f64=True)
+    b = xh.Batch(bctx, structs, bin_, f64=True)
     t_plan = time.perf_counter() - t0
     dev = torch.device("cuda", 0)
     qd = torch.from_numpy(q).to(dev)
@@ -208,8 +225,14 @@ def ensemble_leg(ctx, count, steps, warmup):
 
     def batched():
         b.run_dev(hist.data_ptr(), qd.data_ptr(), nq, S.data_ptr(), "rPDF", rho, sl,
-                  pdf.data_ptr(), graph=True)
+                  pdf.data_ptr(), graph=use_graph)
 
+    # a fresh ensemble: plan + first run (eager launches, as a new plan's
+    # first run would be), timed end to end
+    t0 = time.perf_counter()
+    b.run_dev(hist.data_ptr(), qd.data_ptr(), nq, S.data_ptr(), "rPDF", rho, sl, pdf.data_ptr())
+    bctx.synchronize()
+    t_first = time.perf_counter() - t0
     for _ in range(warmup):
         batched()
     bctx.synchronize()
@@ -290,23 +313,33 @@ def ensemble_leg(ctx, count, steps, warmup):
     b.close()
     bctx.close()
     hist_ms = split_ms["hist"]
+    t_fresh = t_plan + t_first
     return {
         "workload": f"{count} nanoparticles, N log-uniform 500-200,000 (mean {sum(p.n for p in parts) // count}), "
                     "1-4 elements, fcc/bcc sphere, cube, core-shell; bin 0.01 A; 2,048 q 0.2-25 A^-1; "
                     "histograms + pair sums + rPDF per particle",
         "pairs": pairs,
-        "patterns_per_s_batched": round(count / t_b, 2),
+        "patterns_per_s_batched": round(count / t_fresh, 2),
         "patterns_per_s_plain": round(count / t_p, 2),
-        "speedup": round(t_p / t_b, 3),
+        "speedup": round(t_p / t_fresh, 3),
         "patterns_per_s_plain_f32": round(count / t_p32, 2),
-        "speedup_vs_plain_f32": round(t_p32 / t_b, 3),
+        "speedup_vs_plain_f32": round(t_p32 / t_fresh, 3),
+        "batched_note": "fresh ensemble: plan (host k-d tiling on worker threads + upload of sorted "
+                        "coordinates and tables) + first run; the plain forms start from coordinates "
+                        "already on the device",
+        "ms_plan": round(t_plan * 1e3, 2),
+        "ms_first_run": round(t_first * 1e3, 2),
+        "patterns_per_s_batched_replay": round(count / t_b, 2),
+        "speedup_replay": round(t_p / t_b, 3),
+        "replay_note": "the same coordinates again: graph replays of the finished plan" if use_graph
+                       else "the same coordinates again: eager runs of the finished plan",
         "plain_f32_note": "structure by structure with FP32 coordinates and the reference GPU binning "
                           "(IEEE float sqrt and division); the batched form bins in FP64 (the reference "
                           "CPU expression)",
         "ms_batched": round(t_b * 1e3, 2),
         "ms_plain": round(t_p * 1e3, 2),
-        "gpairs_per_s_batched": round(pairs / t_b / 1e9, 2),
-        "graph_ms": round(g_ms[0] / max(g_ms[1], 1), 3),
+        "gpairs_per_s_batched_replay": round(pairs / t_b / 1e9, 2),
+        "graph_ms": round(g_ms[0] / max(g_ms[1], 1), 3) if use_graph else None,
         "kernels_ms_eager_run": split_ms,
         "hist_roofline": {"bound": "valu", "achieved": round(OPS_PER_PAIR * pairs / (hist_ms * 1e-3) / 1e12, 3),
                           "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -314,7 +347,7 @@ def ensemble_leg(ctx, count, steps, warmup):
         "checks": {"hist_equal_plain": hist_eq, "pdf_equal_plain": pdf_eq,
                    "pairsum_max_rel_diff_vs_plain": s_rel, "overflow": ovf,
                    "f32_binning_moved_pairs": moved32},
-        "setup_s": {"generate": round(t_gen, 2), "plan": round(t_plan, 2)},
+        "setup_s": {"generate": round(t_gen, 2)},
     }
 
 
@@ -360,9 +393,13 @@ def pairsum_matrix_leg(ctx, steps):
     tm, km = timed(lambda: ctx.hist_pairsum_dev(hist.data_ptr(), ncols, nh, bin_, qd.data_ptr(), nq,
                                                 Sm.data_ptr(), mfma=True),
                    ["pairsum_slices", "pairsum_mfma"])
+    Sf = torch.empty_like(Sv)
+    tf, kf = timed(lambda: ctx.hist_pairsum_dev(hist.data_ptr(), ncols, nh, bin_, qd.data_ptr(), nq,
+                                                Sf.data_ptr(), f32=True), ["hist_pairsum"])
     H = hist.view(ncols, nh).cpu().numpy().view(np.uint64)
     Svh = Sv.view(ncols, nq).cpu().numpy()
     Smh = Sm.view(ncols, nq).cpu().numpy()
+    Sfh = Sf.view(ncols, nq).cpu().numpy()
     # FP64 reference on the first 16 particles (48 columns)
     cols = 48
     r = (np.arange(nh) + 0.5) * bin_
@@ -404,28 +441,36 @@ def pairsum_matrix_leg(ctx, steps):
         "columns": ncols, "nhist": nh, "nq": nq, "max_count": cmax, "count_slices": nb,
         "overflow": ovf,
         "ms_fp64_vector": round(tv * 1e3, 3), "ms_mfma": round(tm * 1e3, 3),
+        "ms_plain_fp32": round(tf * 1e3, 3),
         "kernel_ms": {"hist_pairsum": round(kv["hist_pairsum"], 3),
                       "pairsum_slices": round(km["pairsum_slices"], 3),
-                      "pairsum_mfma": round(mm_ms, 3)},
+                      "pairsum_mfma": round(mm_ms, 3),
+                      "hist_pairsum_f32": round(kf["hist_pairsum"], 3)},
         "speedup": round(tv / tm, 2),
+        "speedup_vs_plain_fp32": round(tf / tm, 2),
+        "plain_fp32_note": "FP32 terms with IEEE sinf and FP32 sums over 2048-bin chunks, one thread "
+                           "per q (the arithmetic of the reference GPU kernels)",
         "fp64_equiv_tflops_mfma": round(flop / tm / 1e12, 2),
         "roofline_mfma": {"bound": "mfma", "achieved": round(int8_ops / (mm_ms * 1e-3) / 1e12, 1),
                           "peak": 5000.0, "unit": "TOP/s (int8)",
                           "frac": round(int8_ops / (mm_ms * 1e-3) / 1e12 / 5000.0, 4),
                           "int8_products": 5 * nb},
-        "max_rel_err_S_vs_fp64": {"fp64_vector": s_err(Svh), "mfma": s_err(Smh)},
-        "max_rel_err_I_vs_fp64": {"fp64_vector": i_err(Svh), "mfma": i_err(Smh)},
+        "max_rel_err_S_vs_fp64": {"fp64_vector": s_err(Svh), "mfma": s_err(Smh),
+                                  "plain_fp32": s_err(Sfh)},
+        "max_rel_err_I_vs_fp64": {"fp64_vector": i_err(Svh), "mfma": i_err(Smh),
+                                  "plain_fp32": i_err(Sfh)},
         "error_sample": f"first {cols // 3} particles ({cols} histograms), all {nq} q; "
                         "S: max |dS| / max |S| per histogram; I: pointwise where I >= 1e-3 max I",
     }
 
 
-def amp2d_leg(ctx, nrot=16):
+def amp2d_leg(ctx, nrot=256, nrot_list=16):
     """2D pattern throughput: the 50,000-atom three-element disordered solid,
     a 512 x 512 Cartesian q plane (|q_x|, |q_y| <= 8 A^-1, q_z = 0) and
-    `nrot` of the 256 seeded uniform rotations; complex-exponential terms per
-    second of the grid kernel (xs_amp2d_grid), with the list kernel
-    (xs_amp2d, same points as an explicit list) timed beside it.  Checks:
+    all 256 seeded uniform rotations; complex-exponential terms per second of
+    the grid kernel (xs_amp2d_grid) over the whole workload, with the list
+    kernel (xs_amp2d, same points as an explicit list) timed beside it on
+    the first `nrot_list` rotations.  Checks:
     Friedel symmetry I(q) = I(-q) over the plane, grid vs list, and a sample
     of 64 q points x 2 rotations against an FP64 numpy sum."""
     import torch
@@ -447,6 +492,7 @@ def amp2d_leg(ctx, nrot=16):
     fd = torch.from_numpy(np.ascontiguousarray(ff)).to(dev)
     Id = torch.empty(nqp, dtype=torch.float64, device=dev)
     Il = torch.empty(nqp, dtype=torch.float64, device=dev)
+    Ig = torch.empty(nqp, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
 
     def grid():
@@ -454,8 +500,12 @@ def amp2d_leg(ctx, nrot=16):
                            nrot, fd.data_ptr(), Id.data_ptr())
 
     def listed():
-        ctx.amp2d_dev(xd.data_ptr(), ens.n, ens.el_count, qd.data_ptr(), nqp, rd.data_ptr(), nrot,
+        ctx.amp2d_dev(xd.data_ptr(), ens.n, ens.el_count, qd.data_ptr(), nqp, rd.data_ptr(), nrot_list,
                       fd.data_ptr(), Il.data_ptr())
+
+    def grid_first():
+        ctx.amp2d_grid_dev(xd.data_ptr(), ens.n, ens.el_count, q0, du, dv, n, n, rd.data_ptr(),
+                           nrot_list, fd.data_ptr(), Ig.data_ptr())
 
     def timed(fn):
         fn()
@@ -471,10 +521,13 @@ def amp2d_leg(ctx, nrot=16):
 
     t, kms = timed(grid)
     _, kms_list = timed(listed)
+    grid_first()
+    ctx.synchronize()
     I = Id.cpu().numpy()
     fried = float(np.max(np.abs(I - I[::-1]) / np.maximum(np.abs(I), 1e-300)))
     IL = Il.cpu().numpy()
-    vs_list = float(np.max(np.abs(I - IL)) / np.abs(IL).max())
+    IG = Ig.cpu().numpy()
+    vs_list = float(np.max(np.abs(IG - IL)) / np.abs(IL).max())
     # FP64 numpy evaluation of 64 q points x 2 rotations
     idx = np.random.Generator(np.random.PCG64(1)).choice(nqp, 64, replace=False)
     Ie = np.zeros(64)
@@ -492,17 +545,23 @@ def amp2d_leg(ctx, nrot=16):
     ctx.synchronize()
     err = float(np.max(np.abs(Id.cpu().numpy()[idx] - Ie) / Ie))
     terms = ens.n * nqp * nrot
+    terms_list = ens.n * nqp * nrot_list
+    achieved = AMP2D_OPS_PER_TERM * terms / (kms * 1e-3) / 1e12
     return {
         "workload": f"{ens.n}-atom 3-element disordered solid, 512 x 512 q plane (|q| components <= 8), "
-                    f"{nrot} of 256 seeded uniform rotations",
+                    f"all {nrot} seeded uniform rotations",
         "terms": terms, "ms": round(t * 1e3, 2), "kernel_ms": round(kms, 2),
         "gterms_per_s": round(terms / (kms * 1e-3) / 1e9, 1),
-        "full_256_rotations_s_extrapolated": round(kms * 1e-3 * 256 / nrot, 3),
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "ops_per_term": AMP2D_OPS_PER_TERM, "kernel": "amp2d_grid_kernel"},
+        "list_kernel_rotations": nrot_list,
         "list_kernel_ms": round(kms_list, 2),
-        "list_gterms_per_s": round(terms / (kms_list * 1e-3) / 1e9, 1),
+        "list_gterms_per_s": round(terms_list / (kms_list * 1e-3) / 1e9, 1),
         "checks": {"friedel_max_rel": fried, "grid_vs_list_max_rel": vs_list,
                    "fp64_sample_max_rel": err,
-                   "fp64_sample": "64 q points x 2 rotations, FP64 numpy sum"},
+                   "fp64_sample": "64 q points x 2 rotations, FP64 numpy sum",
+                   "grid_vs_list": f"first {nrot_list} rotations, all q points"},
     }
 
 
@@ -537,8 +596,12 @@ def debye_leg(ctx):
         kms = ctx.profile_read("debye")[0]
         ctx.profile(False)
         pairs = ens.n * (ens.n - 1) // 2
+        achieved = DEBYE_OPS_PER_PAIR_Q * pairs * q.size / (kms * 1e-3) / 1e12
         rec = {"atoms": ens.n, "nq": int(q.size), "pair_q": pairs * int(q.size), "ms": round(t * 1e3, 3),
-               "kernel_ms": round(kms, 3), "gpair_q_per_s": round(pairs * q.size / (kms * 1e-3) / 1e9, 1)}
+               "kernel_ms": round(kms, 3), "gpair_q_per_s": round(pairs * q.size / (kms * 1e-3) / 1e9, 1),
+               "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                            "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                            "ops_per_pair_q": DEBYE_OPS_PER_PAIR_Q, "kernel": "debye_kernel<true>"}}
         if ens.n <= 2000:
             d = np.sqrt(((x64[:, :, None] - x64[:, None, :]) ** 2).sum(axis=0))
             iu = np.triu_indices(ens.n, 1)
@@ -772,6 +835,8 @@ def main():
     ap.add_argument("--no-rearr", action="store_true", help="skip the RearrangementInt leg")
     ap.add_argument("--only", choices=["matrix", "2d", "ensemble", "debye", "atoms", "rearr"],
                     help="run one extra leg alone and print its object (for profiling)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="ensemble leg: replay the plan with eager launches instead of a hipGraph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported: "
@@ -790,7 +855,8 @@ def main():
                "debye": lambda: debye_leg(ctx),
                "atoms": lambda: atoms_leg(ctx, cpu=not args.no_cpu_baseline),
                "rearr": lambda: rearr_leg(ctx, cpu=not args.no_cpu_baseline),
-               "ensemble": lambda: ensemble_leg(ctx, args.ensemble, max(1, args.steps // 2), 1)}
+               "ensemble": lambda: ensemble_leg(ctx, args.ensemble, max(1, args.steps // 2), 1,
+                                                use_graph=not args.no_graph)}
         print(json.dumps({args.only: leg[args.only]()}))
         ctx.close()
         return
@@ -854,9 +920,16 @@ def main():
                         "gpairs_per_s": round(pairs / (ms32 * 1e-3) / 1e9, 3),
                         "hist_kernel_ms": round(hist32_ms, 3),
                         "kernel": "hist_kernel_f32<32768, true>",
+                        "roofline": {"bound": "valu",
+                                     "achieved": round(OPS_PER_PAIR * pairs / (hist32_ms * 1e-3) / 1e12, 3),
+                                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                     "frac": round(OPS_PER_PAIR * pairs / (hist32_ms * 1e-3) / 1e12
+                                                   / FP32_PEAK_TFLOPS, 4),
+                                     "ops_per_pair": OPS_PER_PAIR},
                         "valu_active_per_64_pairs": F32_ACTIVE_PER_64_PAIRS,
-                        "valu_issue_frac_of_fp32_peak": round(
-                            F32_ACTIVE_PER_64_PAIRS * pairs / 64 / (2 * CU_ISSUE_PER_S * hist32_ms * 1e-3), 4)},
+                        "valu_wave_instr_per_cu_cycle": round(
+                            F32_ACTIVE_PER_64_PAIRS * pairs / 64 / (CU_ISSUE_PER_S * hist32_ms * 1e-3), 4),
+                        "valu_issue_peak_wave_instr_per_cu_cycle": 2},
         "checks": {"pair_totals_exact": totals_ok, "overflow": int(ovf.item())},
     }
     progress(f"headline done: {line['value']} Gpairs/s")
@@ -872,7 +945,8 @@ def main():
     if args.ensemble > 0:
         del hist, ovf
         torch.cuda.empty_cache()
-        line["ensemble"] = ensemble_leg(ctx, args.ensemble, max(1, args.steps // 2), 1)
+        line["ensemble"] = ensemble_leg(ctx, args.ensemble, max(1, args.steps // 2), 1,
+                                        use_graph=not args.no_graph)
         progress("ensemble leg done")
     if not args.no_atoms:
         line["atoms_builder"] = atoms_leg(ctx, cpu=not args.no_cpu_baseline)

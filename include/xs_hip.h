@@ -49,6 +49,7 @@ extern "C" {
 #define XS_PAIRSUM_MFMA 0x10u   /* xs_hist_pairsum: int8 matrix-core product */
 #define XS_F32_DOUBLE_QUOTIENT 0x20u /* xs_hist, xs_batch_plan, FP32 binning: skip the
                                         FP32 quotient form (see xs_hist) */
+#define XS_PAIRSUM_F32 0x40u    /* xs_hist_pairsum: plain FP32 terms and sums (comparison) */
 
 /* PDF types, typedefs.h:54-56 */
 #define XS_PDF_RDF 0
@@ -122,7 +123,11 @@ int xs_hist(xs_ctx *ctx, const void *xyz, size_t n_atoms,
  * counts into exact 7-bit slices, int32 accumulation; the result differs
  * from the FP64 sum by about 2^-35 sum_k h[k] (needs counts < 2^35 and a
  * workspace of ~5 nq nhist bytes).  It pays off for many histograms (pair
- * types of many structures) on one bin grid. */
+ * types of many structures) on one bin grid.
+ * XS_PAIRSUM_F32 evaluates the terms and the sums over 2048-bin chunks in
+ * FP32 (IEEE sinf), the arithmetic of the reference GPU kernels; its error
+ * against FP64 reaches ~1e-4 of I(q) at low q, where I(q) is a cancelling
+ * difference, so it is there for comparison, not for results. */
 int xs_hist_pairsum(xs_ctx *ctx, const uint64_t *hist, unsigned ntypes,
                     unsigned nhist, double bin, const double *q, unsigned nq,
                     double *S, unsigned flags);

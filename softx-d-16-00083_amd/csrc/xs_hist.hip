@@ -37,9 +37,12 @@
 //   xs_batch_run (xs_batch.hip): zero fills, histograms, pair sums and PDFs
 //     are launches only, optionally captured once into a hipGraph.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include <hipcub/hipcub.hpp>
@@ -1125,17 +1128,20 @@ static void kd_tiles(const T *x, const T *y, const T *z, uint32_t *idx, uint32_t
   }
 }
 
-// Host-side sort of one structure into the sorted arrays (appended), with its
-// tiles (equal-count k-d tiles per element) and boxes.  xyz: SoA of the
-// structure, element-major.
+// Host-side sort of one structure: its atoms in tile order into sx, sy, sz
+// (n each), its tiles (equal-count k-d tiles per element, atom offsets from
+// atom_base) and their boxes appended to tiles / boxes; element e's tiles
+// are [ebase[e], ebase[e] + en[e]) counted from the first tile appended.
+// xyz: SoA of the structure, element-major.
 template <class T>
 static void sort_structure_host(const T *xyz, size_t n, const unsigned *el_count, unsigned n_el,
-                                double bin, uint32_t nhist, std::vector<T> &sx, std::vector<T> &sy,
-                                std::vector<T> &sz, std::vector<Tile> &tiles,
-                                std::vector<double> &boxes, uint32_t *ebase, uint32_t *en) {
+                                double bin, uint32_t nhist, T *sx, T *sy, T *sz, uint32_t atom_base,
+                                std::vector<Tile> &tiles, std::vector<double> &boxes, uint32_t *ebase,
+                                uint32_t *en) {
   const double cap = cell_cap(bin, nhist, wcap_for(nhist));
   std::vector<uint32_t> idx;
   std::vector<std::pair<uint32_t, uint32_t>> leaves;
+  const size_t tile0 = tiles.size();
   size_t off = 0;
   for (unsigned e = 0; e < n_el; ++e) {
     const uint32_t ne = el_count[e];
@@ -1144,25 +1150,22 @@ static void sort_structure_host(const T *xyz, size_t n, const unsigned *el_count
     leaves.clear();
     kd_tiles<T>(xyz + off, xyz + n + off, xyz + 2 * n + off, idx.data(), ne, cap, sizeof(T) == 8,
                 leaves);
-    const size_t base = sx.size();
-    ebase[e] = (uint32_t)tiles.size();
-    for (const auto &l : leaves) tiles.push_back(Tile{(uint32_t)(base + l.first), l.second});
-    en[e] = (uint32_t)tiles.size() - ebase[e];
-    sx.resize(base + ne);
-    sy.resize(base + ne);
-    sz.resize(base + ne);
+    ebase[e] = (uint32_t)(tiles.size() - tile0);
+    for (const auto &l : leaves) tiles.push_back(Tile{(uint32_t)(atom_base + off + l.first), l.second});
+    en[e] = (uint32_t)(tiles.size() - tile0) - ebase[e];
     for (uint32_t d = 0; d < ne; ++d) {
       const uint32_t i = idx[d];
-      sx[base + d] = xyz[off + i];
-      sy[base + d] = xyz[n + off + i];
-      sz[base + d] = xyz[2 * n + off + i];
+      sx[off + d] = xyz[off + i];
+      sy[off + d] = xyz[n + off + i];
+      sz[off + d] = xyz[2 * n + off + i];
     }
     off += ne;
   }
-  for (size_t t = boxes.size() / 6; t < tiles.size(); ++t) {
+  for (size_t t = tile0; t < tiles.size(); ++t) {
     double b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = tiles[t].off; i < tiles[t].off + tiles[t].cnt; ++i) {
-      const double v[3] = {(double)sx[i], (double)sy[i], (double)sz[i]};
+      const uint32_t li = i - atom_base;
+      const double v[3] = {(double)sx[li], (double)sy[li], (double)sz[li]};
       for (int a = 0; a < 3; ++a) {
         b[a] = std::min(b[a], v[a]);
         b[3 + a] = std::max(b[3 + a], v[a]);
@@ -1492,11 +1495,12 @@ static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_coun
     host_bbox<T>(xyz, n, bbox);
     XS_TRY(check_span(bbox, bin));
   }
-  std::vector<T> sx, sy, sz;
+  std::vector<T> sx(n), sy(n), sz(n);
   std::vector<Tile> tiles;
   std::vector<double> boxes;
   uint32_t ebase[kMaxEl], en[kMaxEl];
-  sort_structure_host<T>(xyz, n, el_count, n_el, bin, nhist, sx, sy, sz, tiles, boxes, ebase, en);
+  sort_structure_host<T>(xyz, n, el_count, n_el, bin, nhist, sx.data(), sy.data(), sz.data(), 0u,
+                         tiles, boxes, ebase, en);
   std::vector<TypeDesc> types;
   add_types(n_el, ebase, en, nhist, bin, 0ull, 0u, types);
   std::vector<Item> items;
@@ -1558,69 +1562,133 @@ static void free_plan(xs_batch *b) {
   delete b;
 }
 
+// Per-structure part of a batch plan, built on a worker thread.
+struct StructPart {
+  int rc = XS_OK;
+  std::string err;
+  uint32_t nh = 0;
+  std::vector<Tile> tiles;  // atom offsets already global
+  std::vector<double> boxes;
+  uint32_t ebase[kMaxEl], en[kMaxEl];
+};
+
+// Threads of the host planning: the machine's, at most 16.
+static unsigned plan_threads() {
+  const unsigned h = std::thread::hardware_concurrency();
+  return std::max(1u, std::min(h ? h : 1u, 16u));
+}
+
 template <class T>
 static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const unsigned *el_count,
                       const double *const *xyz, double bin, const unsigned *nhist_in, bool f64,
                       bool quot, xs_batch **out) {
+  // 1. sizes and validation (serial, cheap)
+  std::vector<size_t> nat(nstruct), ecf(nstruct), abase(nstruct);
+  size_t ec = 0, ntot_all = 0;
+  for (unsigned s = 0; s < nstruct; ++s) {
+    const unsigned ne = n_el[s];
+    if (ne == 0 || ne > (unsigned)kMaxEl)
+      return fail(XS_ERR_RANGE, "xs_batch_plan: n_el must be in [1, 16]");
+    size_t n = 0;
+    for (unsigned e = 0; e < ne; ++e) n += el_count[ec + e];
+    if (n == 0 || n >= (size_t(1) << 31))
+      return fail(XS_ERR_RANGE, "xs_batch_plan: a structure must hold 1 to 2^31-1 atoms");
+    nat[s] = n;
+    ecf[s] = ec;
+    abase[s] = ntot_all;
+    ntot_all += n;
+    ec += ne;
+  }
+  if (ntot_all >= (size_t(1) << 32))
+    return fail(XS_ERR_RANGE, "xs_batch_plan: an ensemble must hold fewer than 2^32 atoms");
+  const size_t nc = std::max<size_t>(ntot_all, 1);
+  std::vector<T> sorted(3 * nc);
+
+  // 2. per structure, on worker threads (largest first): bounding box, bin
+  //    count, k-d tiles; sorted coordinates straight into their place
+  std::vector<StructPart> parts(nstruct);
+  std::vector<unsigned> order(nstruct);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](unsigned a, unsigned c) { return nat[a] > nat[c]; });
+  std::atomic<unsigned> next{0};
+  auto worker = [&]() {
+    std::vector<T> tmp;
+    for (unsigned k; (k = next.fetch_add(1)) < nstruct;) {
+      const unsigned s = order[k];
+      StructPart &P = parts[s];
+      const size_t n = nat[s];
+      const double *X = xyz[s];
+      double bbox[6];
+      host_bbox<double>(X, n, bbox);
+      P.rc = check_span(bbox, bin);
+      if (P.rc != XS_OK) {
+        P.err = xs_last_error();
+        continue;
+      }
+      // bin count as the reference's GetHistSize (CalcFunctions.cpp:224-243) unless given
+      uint32_t nh = nhist_in ? nhist_in[s] : 0u;
+      if (!nh) {
+        const double dx = bbox[3] - bbox[0], dy = bbox[4] - bbox[1], dz = bbox[5] - bbox[2];
+        nh = (uint32_t)(std::sqrt(dx * dx + dy * dy + dz * dz) / bin) + 1;
+      }
+      if (!f64 && nh > (1u << 24)) {
+        P.rc = XS_ERR_RANGE;
+        P.err = "xs_batch_plan: nhist > 2^24 needs XS_F64_BINNING";
+        continue;
+      }
+      P.nh = nh;
+      tmp.resize(3 * n);
+      for (size_t i = 0; i < 3 * n; ++i) tmp[i] = (T)X[i];
+      if (sizeof(T) < sizeof(double)) {
+        // coordinates beyond the float range become infinite in FP32 binning
+        double fb[6];
+        host_bbox<T>(tmp.data(), n, fb);
+        P.rc = check_span(fb, bin);
+        if (P.rc != XS_OK) {
+          P.err = xs_last_error();
+          continue;
+        }
+      }
+      const size_t a0 = abase[s];
+      sort_structure_host<T>(tmp.data(), n, el_count + ecf[s], n_el[s], bin, nh, sorted.data() + a0,
+                             sorted.data() + nc + a0, sorted.data() + 2 * nc + a0, (uint32_t)a0, P.tiles,
+                             P.boxes, P.ebase, P.en);
+    }
+  };
+  {
+    const unsigned nt = std::min(plan_threads(), nstruct);
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto &th : pool) th.join();
+  }
+  for (unsigned s = 0; s < nstruct; ++s)
+    if (parts[s].rc != XS_OK) return fail(parts[s].rc, parts[s].err);
+
+  // 3. tables in structure order (serial)
   auto *b = new xs_batch();
   b->ctx = ctx;
   b->f64 = f64;
   b->quot_f32 = quot;
   b->bin = bin;
   b->nstruct = nstruct;
-  std::vector<T> sx, sy, sz, tmp;
   std::vector<Tile> tiles;
   std::vector<double> boxes;
   std::vector<TypeDesc> types;
-  size_t ec = 0;
   unsigned long long hoff = 0;
   for (unsigned s = 0; s < nstruct; ++s) {
+    StructPart &P = parts[s];
     const unsigned ne = n_el[s];
-    if (ne == 0 || ne > (unsigned)kMaxEl) {
-      free_plan(b);
-      return fail(XS_ERR_RANGE, "xs_batch_plan: n_el must be in [1, 16]");
-    }
-    size_t n = 0;
-    for (unsigned e = 0; e < ne; ++e) n += el_count[ec + e];
-    if (n == 0 || n >= (size_t(1) << 31)) {
-      free_plan(b);
-      return fail(XS_ERR_RANGE, "xs_batch_plan: a structure must hold 1 to 2^31-1 atoms");
-    }
-    const double *X = xyz[s];
-    double bbox[6];
-    host_bbox<double>(X, n, bbox);
-    const int rc = check_span(bbox, bin);
-    if (rc != XS_OK) {
-      free_plan(b);
-      return rc;
-    }
-    // bin count as the reference's GetHistSize (CalcFunctions.cpp:224-243) unless given
-    uint32_t nh = nhist_in ? nhist_in[s] : 0u;
-    if (!nh) {
-      const double dx = bbox[3] - bbox[0], dy = bbox[4] - bbox[1], dz = bbox[5] - bbox[2];
-      nh = (uint32_t)(std::sqrt(dx * dx + dy * dy + dz * dz) / bin) + 1;
-    }
-    if (!f64 && nh > (1u << 24)) {
-      free_plan(b);
-      return fail(XS_ERR_RANGE, "xs_batch_plan: nhist > 2^24 needs XS_F64_BINNING");
-    }
-    tmp.resize(3 * n);
-    for (size_t i = 0; i < 3 * n; ++i) tmp[i] = (T)X[i];
-    if (sizeof(T) < sizeof(double)) {
-      // coordinates beyond the float range become infinite in FP32 binning
-      double fb[6];
-      host_bbox<T>(tmp.data(), n, fb);
-      const int rf = check_span(fb, bin);
-      if (rf != XS_OK) {
-        free_plan(b);
-        return rf;
-      }
-    }
-    uint32_t ebase[kMaxEl], en[kMaxEl];
-    sort_structure_host<T>(tmp.data(), n, el_count + ec, ne, bin, nh, sx, sy, sz, tiles, boxes, ebase,
-                           en);
+    const uint32_t nh = P.nh;
+    const uint32_t tbase = (uint32_t)tiles.size();
+    tiles.insert(tiles.end(), P.tiles.begin(), P.tiles.end());
+    boxes.insert(boxes.end(), P.boxes.begin(), P.boxes.end());
+    std::vector<Tile>().swap(P.tiles);
+    std::vector<double>().swap(P.boxes);
+    uint32_t ebase[kMaxEl];
+    for (unsigned e = 0; e < ne; ++e) ebase[e] = tbase + P.ebase[e];
     const size_t t0 = types.size();
-    add_types(ne, ebase, en, nh, bin, hoff, s, types);
+    add_types(ne, ebase, P.en, nh, bin, hoff, s, types);
     b->type_first.push_back(t0);
     for (size_t t = t0; t < types.size(); ++t) {
       b->type_hist_off.push_back(types[t].hist_off);
@@ -1629,13 +1697,12 @@ static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const
     b->pdf_off.push_back(b->pdf_len);
     b->pdf_len += (unsigned long long)(1 + ntypes_of(ne)) * nh;
     b->n_el.push_back(ne);
-    b->el_first.push_back(ec);
+    b->el_first.push_back(ecf[s]);
     b->nhist.push_back(nh);
-    b->ntot.push_back((unsigned)n);
+    b->ntot.push_back((unsigned)nat[s]);
     b->hist_off.push_back(hoff);
-    for (unsigned e = 0; e < ne; ++e) b->el_count.push_back(el_count[ec + e]);
+    for (unsigned e = 0; e < ne; ++e) b->el_count.push_back(el_count[ecf[s] + e]);
     hoff += (unsigned long long)ntypes_of(ne) * nh;
-    ec += ne;
   }
   b->hist_len = hoff;
   b->ntypes_total = types.size();
@@ -1648,7 +1715,6 @@ static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const
   b->items8192 = small.size();
   b->items32768 = large.size();
   small.insert(small.end(), large.begin(), large.end());
-  const size_t nc = std::max<size_t>(sx.size(), 1);
   b->natoms = nc;
   const bool ok =
       hipMalloc(&b->coords, 3 * nc * sizeof(T)) == hipSuccess &&
@@ -1664,11 +1730,7 @@ static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const
   }
   T *c = static_cast<T *>(b->coords);
   hipError_t e = hipStreamSynchronize(ctx->stream);
-  if (e == hipSuccess && !sx.empty()) {
-    e = hipMemcpy(c, sx.data(), sx.size() * sizeof(T), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(c + nc, sy.data(), sy.size() * sizeof(T), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(c + 2 * nc, sz.data(), sz.size() * sizeof(T), hipMemcpyHostToDevice);
-  }
+  if (e == hipSuccess) e = hipMemcpy(c, sorted.data(), 3 * nc * sizeof(T), hipMemcpyHostToDevice);
   if (e == hipSuccess && !tiles.empty())
     e = hipMemcpy(b->tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice);
   if (e == hipSuccess && !boxes.empty())

@@ -82,6 +82,41 @@ __global__ __launch_bounds__(kQBlock) void pairsum_kernel(const unsigned long lo
   if (acc != 0.0) atomicAdd(&S[(size_t)p * nq + iq], acc / (qv * bin));
 }
 
+// The plain FP32 form (XS_PAIRSUM_F32), for comparison: every term
+// h sin(q r)/(q r + 1e-8) evaluated in FP32 with the IEEE-accurate sinf, as
+// the reference GPU kernels evaluate theirs in FP32 (CalcFunctionsCUDA.cu:
+// 923-985; they take __sinf and 1e-6f, here sinf and the CPU path's 1e-8 so
+// that the difference to FP64 is the FP32 arithmetic alone), summed in FP32
+// over each 2048-bin chunk; chunk sums go to S with atomics.  Counts are
+// rounded to float (exact below 2^24).
+__global__ __launch_bounds__(kQBlock) void pairsum_f32_kernel(const unsigned long long *__restrict__ hist,
+                                                              unsigned nhist, const double *__restrict__ q,
+                                                              unsigned nq, double bin,
+                                                              double *__restrict__ S) {
+  __shared__ float2 bins[kChunk];
+  const unsigned p = blockIdx.z;
+  const unsigned k0 = blockIdx.y * kChunk;
+  const unsigned kn = min((unsigned)kChunk, nhist - k0);
+  const unsigned long long *h = hist + (size_t)p * nhist + k0;
+  const float binf = (float)bin;
+  for (unsigned b = threadIdx.x; b < kn; b += kQBlock)
+    bins[b] = make_float2((float)h[b], ((float)(k0 + b) + 0.5f) * binf);
+  __syncthreads();
+  const unsigned iq = blockIdx.x * kQBlock + threadIdx.x;
+  if (iq >= nq) return;
+  const float qv = (float)q[iq];
+  if (qv == 0.0f) return;
+  float acc = 0.0f;
+  for (unsigned b = 0; b < kn; ++b) {
+    const float2 e = bins[b];
+    if (e.x != 0.0f) {
+      const float x = qv * e.y;
+      acc += e.x * sinf(x) / (x + 1e-8f);
+    }
+  }
+  if (acc != 0.0f) atomicAdd(&S[(size_t)p * nq + iq], (double)acc);
+}
+
 // Many histograms: blockIdx.x = chunk, blockIdx.y = block of 256 q.
 __global__ __launch_bounds__(kQBlock) void pairsum_batch_kernel(
     const unsigned long long *__restrict__ hist, const PairsumChunk *__restrict__ chunks,
@@ -234,6 +269,8 @@ extern "C" int xs_hist_pairsum(xs_ctx *ctx, const uint64_t *hist, unsigned ntype
     hd = h2;
     qd = q2;
   }
+  if ((flags & XS_PAIRSUM_MFMA) && (flags & XS_PAIRSUM_F32))
+    return fail(XS_ERR_ARG, "xs_hist_pairsum: XS_PAIRSUM_MFMA and XS_PAIRSUM_F32 exclude each other");
   if (flags & XS_PAIRSUM_MFMA) {
     XS_TRY(pairsum_mfma(ctx, hd, ntypes, nhist, bin, qd, nq, Sd));
   } else {
@@ -242,8 +279,12 @@ extern "C" int xs_hist_pairsum(xs_ctx *ctx, const uint64_t *hist, unsigned ntype
     hipEvent_t ev;
     XS_TRY(prof_begin(ctx, "hist_pairsum", &ev));
     dim3 grid((nq + iq::kQBlock - 1) / iq::kQBlock, (nhist + iq::kChunk - 1) / iq::kChunk, ntypes);
-    hipLaunchKernelGGL(iq::pairsum_kernel, grid, dim3(iq::kQBlock), 0, ctx->stream, hd, nhist, qd,
-                       nq, bin, Sd);
+    if (flags & XS_PAIRSUM_F32)
+      hipLaunchKernelGGL(iq::pairsum_f32_kernel, grid, dim3(iq::kQBlock), 0, ctx->stream, hd, nhist,
+                         qd, nq, bin, Sd);
+    else
+      hipLaunchKernelGGL(iq::pairsum_kernel, grid, dim3(iq::kQBlock), 0, ctx->stream, hd, nhist, qd,
+                         nq, bin, Sd);
     XS_HIP_CHECK(hipGetLastError());
     XS_TRY(prof_end(ctx, "hist_pairsum", ev));
   }

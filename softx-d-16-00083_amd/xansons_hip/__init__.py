@@ -29,6 +29,7 @@ XS_PDF_CPU_RADIUS = 0x4
 XS_BATCH_GRAPH = 0x8
 XS_PAIRSUM_MFMA = 0x10
 XS_F32_DOUBLE_QUOTIENT = 0x20
+XS_PAIRSUM_F32 = 0x40
 PDF_TYPES = {"RDF": 0, "PDF": 1, "rPDF": 2}
 
 _lib = None
@@ -177,20 +178,24 @@ class Context:
                              C.c_void_p(ovf_ptr) if ovf_ptr else None, flags))
 
     # ---------------------------------------------------------------- H3
-    def hist_pairsum(self, hist, bin_: float, q, mfma: bool = False):
-        """mfma=True: the int8 matrix-core product (many histograms)."""
+    def hist_pairsum(self, hist, bin_: float, q, mfma: bool = False, f32: bool = False):
+        """mfma=True: the int8 matrix-core product (many histograms); f32=True:
+        the plain FP32 form (comparison only)."""
         hist = np.ascontiguousarray(hist, dtype=np.uint64)
         q = np.ascontiguousarray(q, dtype=np.float64)
         nt, nh = hist.shape
         S = np.zeros(nt * q.size)
         _check(lib().xs_hist_pairsum(self.h, _p(hist), nt, nh, float(bin_), _p(q), q.size, _p(S),
-                                     XS_PAIRSUM_MFMA if mfma else 0))
+                                     (XS_PAIRSUM_MFMA if mfma else 0) | (XS_PAIRSUM_F32 if f32 else 0)))
         return S.reshape(nt, q.size)
 
-    def hist_pairsum_dev(self, hist_ptr, ntypes_, nhist, bin_, q_ptr, nq, S_ptr, mfma=False):
+    def hist_pairsum_dev(self, hist_ptr, ntypes_, nhist, bin_, q_ptr, nq, S_ptr, mfma=False,
+                         f32=False):
+        """f32=True: the plain FP32 form (comparison only)."""
         _check(lib().xs_hist_pairsum(self.h, C.c_void_p(hist_ptr), int(ntypes_), int(nhist),
                                      float(bin_), C.c_void_p(q_ptr), int(nq), C.c_void_p(S_ptr),
-                                     XS_DEVICE_PTRS | (XS_PAIRSUM_MFMA if mfma else 0)))
+                                     XS_DEVICE_PTRS | (XS_PAIRSUM_MFMA if mfma else 0) |
+                                     (XS_PAIRSUM_F32 if f32 else 0)))
 
     # ---------------------------------------------------------------- H4
     def pdf(self, hist, el_count, bin_, pdf_type, rho0, sl, cpu_radius=False):

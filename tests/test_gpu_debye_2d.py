@@ -139,3 +139,28 @@ def test_amp2d_grid_equals_list_and_oracle(ctx, nu, nv):
     print(f"2d grid {nu}x{nv}: vs oracle {pw:.3e} {l2:.3e}, vs list {pl:.3e}", flush=True)
     assert pw <= TOL and l2 <= TOL
     assert pl <= TOL
+
+
+def test_amp2d_grid_c3_rotation_set(ctx):
+    """The 2D workload's own inputs at reduced size: the seeded uniform
+    rotation set (st.uniform_rotations(256, seed=8); every 16th of them), the
+    512 x 512 plane's q grid subsampled to 64 x 64 points with the same
+    q0 and the full plane's step times 8, and the three-element disordered
+    solid cut to 3,000 atoms, against the FP64 oracle."""
+    ens = st.disordered_cubic(natoms=3000, n=15)
+    x64 = ens.xyz64()
+    n = 512
+    step = 2.0 * 8.0 / (n - 1)
+    q0 = np.array([-8.0, -8.0, 0.0])
+    du, dv = np.array([0.0, 8 * step, 0.0]), np.array([8 * step, 0.0, 0.0])
+    nu = nv = 64
+    iv, iu = np.meshgrid(np.arange(nv), np.arange(nu), indexing="ij")
+    qv = q0[None] + iu.reshape(-1, 1) * du[None] + iv.reshape(-1, 1) * dv[None]
+    ff = st.gaussian_ff(np.linalg.norm(qv, axis=1), st.gaussian_ff_coeffs(3, seed=9))
+    rot = st.uniform_rotations(256, seed=8)[::16].reshape(-1, 9)
+    assert rot.shape[0] == 16
+    Ig = ctx.amp2d_grid(x64, ens.el_count, q0, du, dv, nu, nv, rot, ff)
+    Io = O.amp2d(x64, ens.el_count, qv, rot, ff)
+    pw, l2 = rel_err(Ig, Io)
+    print(f"2d C3 rotations: {pw:.3e} {l2:.3e}", flush=True)
+    assert pw <= TOL and l2 <= TOL

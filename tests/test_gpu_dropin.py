@@ -108,7 +108,7 @@ def test_partial_intensity_two_blocks(tmp):
 
 
 @needs_exe
-@pytest.mark.parametrize("conv", [0, 7, 11])
+@pytest.mark.parametrize("conv", list(range(12)))
 def test_2d(tmp, conv):
     g = golden("amp2d_small")
     ens = ens_of(g, ["Si", "O"])
@@ -122,6 +122,24 @@ def test_2d(tmp, conv):
     I2 = np.loadtxt(os.path.join(tmp, "D_neut_2D.txt"), ndmin=2)
     assert rel_err(I2, g[f"I2D_{conv}"])[0] < 2e-6
     assert rel_err(col(tmp, "D_neut_1D.txt"), g[f"I1D_{conv}"])[0] < 2e-6
+
+
+@needs_exe
+def test_2d_xray_polar(tmp):
+    """The 2D polarisation branch (x-ray form factors, PolarFactor with a
+    given wavelength) against the reference's 2D and 1D files."""
+    g = golden("amp2d_small")
+    ens = ens_of(g, ["Si", "O"])
+    mg.write_atoms(os.path.join(tmp, "atoms.txt"), ens)
+    mg.write_ff(os.path.join(tmp, "ff.txt"), ens.names, g["q"], g["ff"])
+    run(tmp, mg.xml(dict(name="DX", source="xray", scenario="2D", FFfilename="ff.txt",
+                         PolarFactor="yes", wavelength="1.0"),
+                    ['<q min="0.5" max="6" N="10" Nfi="12"/>',
+                     '<Euler convention="ZXZ" min="10 20 30" max="100 80 200" N="2 2 3"/>'],
+                    ["atoms.txt"]))
+    I2 = np.loadtxt(os.path.join(tmp, "DX_xray_2D.txt"), ndmin=2)
+    assert rel_err(I2, g["I2D_xray_polar"])[0] < 2e-6
+    assert rel_err(col(tmp, "DX_xray_1D.txt"), g["I1D_xray_polar"])[0] < 2e-6
 
 
 @needs_exe

@@ -303,11 +303,11 @@ def test_f32_binning_overflowing_coordinates_rejected(ctx):
     """Coordinates beyond the float range become infinite when a batch plan
     (double input) rounds them to float for FP32 binning: rejected as
     non-finite.  The bin of 1e31 A keeps the double span itself within 2^31
-    bins, so it is the float check that refuses."""
+    bins and the bin count is given, so it is the float check that refuses."""
     import xansons_hip as xh
     ens = st.random_cluster([300], box=10.0, min_dist=0.5, seed=6)
     xyz = ens.xyz64().copy()
     xyz[0, 17] = 1e39
     with pytest.raises(xh.XSError) as e:
-        xh.Batch(ctx, [(xyz, ens.el_count)], 1e31, f64=False)
+        xh.Batch(ctx, [(xyz, ens.el_count)], 1e31, nhist=[1000], f64=False)
     assert e.value.code == -1 and "non-finite" in str(e.value)

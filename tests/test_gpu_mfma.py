@@ -87,3 +87,43 @@ def test_mfma_rejects_counts_beyond_2_35(ctx):
     H[1, 3] = np.uint64(1) << np.uint64(35)
     with pytest.raises(xh.XSError):
         mfma_pairsum(ctx, H, 0.01, st.q_mesh(0.5, 5.0, 8))
+
+
+def test_mfma_full_precision_form_workload(ctx):
+    """The precision-form workload at its full size: 256 two-element
+    particles, 768 histograms on one 16,384-bin grid (0.004 A), 8,192 q over
+    0.2-25 A^-1.  Every column against the FP64 vector kernel (itself pinned
+    to the oracle), and 24 columns against an FP64 numpy product."""
+    import torch
+    import xansons_hip as xh
+    nh, bin_, nq = 16384, 0.004, 8192
+    parts = st.c4_particles()
+    b = xh.Batch(ctx, [(p.xyz64(), p.el_count) for p in parts], bin_, nhist=[nh] * len(parts))
+    hist, _, _ = b.run()
+    assert not b.overflow().any()
+    b.close()
+    H = hist.reshape(-1, nh)
+    assert H.shape == (768, nh)
+    q = st.q_mesh(0.2, 25.0, nq)
+    dev = torch.device("cuda", 0)
+    hd = torch.from_numpy(H.view(np.int64)).to(dev)
+    qd = torch.from_numpy(q).to(dev)
+    Sv = torch.empty(768 * nq, dtype=torch.float64, device=dev)
+    Sm = torch.empty_like(Sv)
+    torch.cuda.synchronize()
+    print("c4 full: 768 x 16384 x 8192", flush=True)
+    ctx.hist_pairsum_dev(hd.data_ptr(), 768, nh, bin_, qd.data_ptr(), nq, Sv.data_ptr())
+    ctx.hist_pairsum_dev(hd.data_ptr(), 768, nh, bin_, qd.data_ptr(), nq, Sm.data_ptr(), mfma=True)
+    ctx.synchronize()
+    Sv = Sv.view(768, nq).cpu().numpy()
+    Sm = Sm.view(768, nq).cpu().numpy()
+    check_s(Sm, Sv, "c4 all columns, MFMA vs FP64 vector")
+    cols = np.arange(0, 768, 32)
+    r = (np.arange(nh) + 0.5) * bin_
+    qr = np.outer(q, r)
+    A = np.sin(qr) / (qr + 1e-8)
+    del qr
+    Se = (A @ H[cols].astype(np.float64).T).T
+    del A
+    check_s(Sm[cols], Se, "c4 24 columns, MFMA vs FP64 numpy")
+    check_s(Sv[cols], Se, "c4 24 columns, FP64 vector vs FP64 numpy")
