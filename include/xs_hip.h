@@ -151,6 +151,23 @@ int xs_debye_pairsum(xs_ctx *ctx, const double *xyz, size_t n_atoms,
                      const unsigned *el_count, unsigned n_el, const double *q,
                      unsigned nq, double *S, unsigned flags);
 
+/* Allocation-free variant of xs_debye_pairsum, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * xyz, q and S are device pointers (flags must hold XS_DEVICE_PTRS); the call
+ * enqueues on the context stream and returns without waiting.  It calls no
+ * hipMalloc/hipFree, no memset and no host copy: the kernels' tile and
+ * work-item tables are built on the device in a 32 MiB scratch array that
+ * the library holds on every device it runs on until the process ends.
+ * Limit: the tables of the structure must fit the scratch -- 16 bytes per
+ * tile of 32 atoms and per tile pair, about 65,000 atoms; larger inputs get
+ * XS_ERR_RANGE (use xs_debye_pairsum).  Every call on a device shares its
+ * scratch: calls that overlap on one device (several streams or threads)
+ * race, so serialise them. */
+int xs_debye_pairsum_noalloc(xs_ctx *ctx, const double *xyz, size_t n_atoms,
+                             const unsigned *el_count, unsigned n_el, const double *q,
+                             unsigned nq, double *S, unsigned flags);
+
 /* H6: block-pair partial Debye sums.  blk_count[el*nblocks+b] atoms of
  * element el belong to block b (block-major inside each element).
  * S[(p*nparts + part)*nq + iq], part(iB<=jB) = nblocks*iB - iB*(iB+1)/2 + jB.

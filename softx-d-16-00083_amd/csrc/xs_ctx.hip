@@ -153,6 +153,9 @@ int xs_init(int device, xs_ctx **out) {
     return fail(XS_ERR_HIP, "hipStreamCreateWithFlags failed");
   }
   ctx->stream = ctx->own_stream;
+  // the allocation-free entry point's scratch address on this device; a
+  // failure here is reported by that entry point alone (it asks again)
+  if (debye_scratch_init(device) != XS_OK) set_error("");
   *out = ctx;
   return XS_OK;
 }

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "xs_internal.h"
@@ -38,6 +39,9 @@ constexpr int kQBlock = 32 * kQPT;  // q per work group (a half-wave covers them
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 static_assert((kWaves - 1) * kQPT * 32 <= 4 * kTT * kTT, "wave sums must fit the pair buffer");
 
+constexpr int kMaxElem = 16;
+constexpr int kMaxTypes = kMaxElem * (kMaxElem + 1) / 2;
+
 struct Tile {
   uint32_t off, cnt, el, blk;
 };
@@ -45,6 +49,16 @@ struct Item {
   uint32_t ti, tj, p, part;
 };
 
+// The q mesh as found on the device (allocation-free entry point): its step
+// and whether it is uniform (the test of uniform_mesh below).
+struct MeshInfo {
+  double dq;
+  uint32_t uniform, pad;
+};
+
+// mesh == nullptr: the host decided UNIFORM and passed dq.  Otherwise both
+// variants are launched and the one that does not match mesh->uniform
+// returns at once (the flag is the same for the whole grid).
 template <bool UNIFORM>
 __global__ __launch_bounds__(kBlock) void debye_kernel(const double *__restrict__ x,
                                                        const double *__restrict__ y,
@@ -53,7 +67,12 @@ __global__ __launch_bounds__(kBlock) void debye_kernel(const double *__restrict_
                                                        const Item *__restrict__ items,
                                                        const double *__restrict__ q, unsigned nq,
                                                        double dq, unsigned nparts,
-                                                       double *__restrict__ S) {
+                                                       double *__restrict__ S,
+                                                       const MeshInfo *__restrict__ mesh) {
+  if (mesh) {
+    if ((mesh->uniform != 0) != UNIFORM) return;
+    dq = mesh->dq;
+  }
   __shared__ double4 pr[kTT * kTT];
   const Item it = items[blockIdx.x];
   const Tile A = tiles[it.ti], B = tiles[it.tj];
@@ -235,10 +254,10 @@ static int run(xs_ctx *ctx, const double *xyz_in, size_t n, const unsigned *el_c
     dim3 grid((unsigned)items.size(), (nq + kQBlock - 1) / kQBlock);
     if (uniform)
       hipLaunchKernelGGL(debye_kernel<true>, grid, dim3(kBlock), 0, ctx->stream, xyz, xyz + n,
-                         xyz + 2 * n, td, idv, qd, nq, dq, nparts, Sd);
+                         xyz + 2 * n, td, idv, qd, nq, dq, nparts, Sd, nullptr);
     else
       hipLaunchKernelGGL(debye_kernel<false>, grid, dim3(kBlock), 0, ctx->stream, xyz, xyz + n,
-                         xyz + 2 * n, td, idv, qd, nq, dq, nparts, Sd);
+                         xyz + 2 * n, td, idv, qd, nq, dq, nparts, Sd, nullptr);
     XS_HIP_CHECK(hipGetLastError());
     XS_TRY(prof_end(ctx, "debye", ev));
   }
@@ -252,6 +271,126 @@ static int run(xs_ctx *ctx, const double *xyz_in, size_t n, const unsigned *el_c
   return XS_OK;
 }
 
+// ---------------------------------------------------------------- allocation-free entry point
+//
+// xs_debye_pairsum_noalloc runs the same kernels with their tile and item
+// tables built on the device in `scratch`, a file-scope device array, so it
+// allocates nothing, copies nothing to or from the host and never waits:
+// mesh_info_kernel finds dq and whether the mesh is uniform, table_kernel
+// writes the tables, fill_kernel clears S, and both debye_kernel variants are
+// launched (the one that does not match the mesh returns at once).  Host code
+// must not pass `scratch` itself to a kernel (that is a host-side stand-in,
+// not a device address): scratch_ptr() gives the device address, taken once
+// per device with hipGetSymbolAddress.  Every call on a device shares its
+// scratch, so calls that overlap on one device race.
+
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// The device address of `scratch` on the current device (which must be
+// `device`), looked up on the first call for that device.
+static int scratch_ptr(int device, void **out) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(XS_ERR_DEVICE, "xs_debye_pairsum_noalloc: device id beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(XS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return XS_OK;
+}
+
+// Tables of one structure (one structural block), in the order run() builds
+// them on the host: tiles of 32 atoms per element; items per pair type
+// p = (a <= b), I tile major, J tiles from the I tile on (a == b) or all.
+struct TableArgs {
+  uint32_t n_el, ntiles;
+  unsigned long long nitems;
+  uint32_t el_off[kMaxElem + 1];     // first atom of element e
+  uint32_t tile_base[kMaxElem + 1];  // first tile of element e
+  unsigned long long item_off[kMaxTypes + 1];  // first item of pair type p
+};
+
+__global__ void mesh_info_kernel(const double *__restrict__ q, unsigned nq, MeshInfo *mesh) {
+  __shared__ double smax[256];
+  __shared__ int sbad[256];
+  const double dq = nq > 1 ? (q[nq - 1] - q[0]) / (double)(nq - 1) : 0.0;
+  double m = 0.0;
+  for (unsigned k = threadIdx.x; k < nq; k += blockDim.x) m = fmax(m, fabs(q[k]));
+  smax[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+    __syncthreads();
+  }
+  const double scale = smax[0];
+  int bad = 0;
+  for (unsigned k = threadIdx.x; k < nq; k += blockDim.x)
+    bad |= fabs(q[k] - (q[0] + (double)k * dq)) > 1e-13 * scale;
+  sbad[threadIdx.x] = bad;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int any = 0;
+    for (int t = 0; t < 256; ++t) any |= sbad[t];
+    mesh->dq = dq;
+    mesh->uniform = any ? 0u : 1u;
+    mesh->pad = 0u;
+  }
+}
+
+__global__ void table_kernel(TableArgs A, Tile *__restrict__ tiles, Item *__restrict__ items) {
+  const unsigned long long k = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x;
+  if (k < A.ntiles) {
+    uint32_t e = 0;
+    while (e + 1 < A.n_el && A.tile_base[e + 1] <= k) ++e;
+    const uint32_t t = (uint32_t)k - A.tile_base[e];
+    const uint32_t cnt = A.el_off[e + 1] - A.el_off[e];
+    tiles[k] = Tile{A.el_off[e] + t * kTT, min((uint32_t)kTT, cnt - t * kTT), e, 0u};
+  }
+  if (k >= A.nitems) return;
+  uint32_t p = 0, a = 0, b = 0;
+  // pair type p = (a, b) of item k: types in the order (0,0),(0,1),...,(1,1),...
+  for (;;) {
+    if (A.item_off[p + 1] > k) break;
+    ++p;
+    if (++b == A.n_el) b = ++a;
+  }
+  const unsigned long long r = k - A.item_off[p];
+  const uint64_t Ta = A.tile_base[a + 1] - A.tile_base[a];
+  uint64_t i, j;
+  if (a == b) {
+    // row i holds Ta - i items (J tiles i..Ta-1); rows before i hold
+    // c(i) = i Ta - i (i - 1) / 2
+    const double T2 = 2.0 * (double)Ta + 1.0;
+    double g = floor(0.5 * (T2 - sqrt(fmax(T2 * T2 - 8.0 * (double)r, 0.0))));
+    i = g < 0.0 ? 0 : (uint64_t)g;
+    if (i >= Ta) i = Ta - 1;
+    auto c = [Ta](uint64_t v) { return v * Ta - v * (v - 1) / 2; };
+    while (i > 0 && c(i) > r) --i;
+    while (i + 1 < Ta && c(i + 1) <= r) ++i;
+    j = i + (r - c(i));
+  } else {
+    const uint64_t Tb = A.tile_base[b + 1] - A.tile_base[b];
+    i = r / Tb;
+    j = r % Tb;
+  }
+  items[k] = Item{A.tile_base[a] + (uint32_t)i, A.tile_base[b] + (uint32_t)j, p, 0u};
+}
+
+__global__ void fill_kernel(double *__restrict__ p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
 static int check_args(const double *xyz, size_t n, const unsigned *el_count, unsigned n_el,
                       const double *q, unsigned nq, double *S) {
   if (!el_count || !q || !S || (!xyz && n)) return fail(XS_ERR_ARG, "debye: null pointer");
@@ -263,7 +402,82 @@ static int check_args(const double *xyz, size_t n, const unsigned *el_count, uns
   return XS_OK;
 }
 
+// Layout of the scratch for one call: MeshInfo, tiles, items.
+static size_t scratch_need(size_t ntiles, unsigned long long nitems, size_t *tiles_at,
+                           size_t *items_at) {
+  *tiles_at = 256;
+  *items_at = align_up(*tiles_at + ntiles * sizeof(Tile), 256);
+  return *items_at + (size_t)nitems * sizeof(Item);
+}
+
+static int run_noalloc(xs_ctx *ctx, const double *xyz, size_t n, const unsigned *el_count,
+                       unsigned n_el, const double *q, unsigned nq, double *S) {
+  const unsigned ntypes = ntypes_of(n_el);
+  TableArgs A{};
+  A.n_el = n_el;
+  uint64_t tiles = 0, off = 0;
+  for (unsigned e = 0; e < n_el; ++e) {
+    A.el_off[e] = (uint32_t)off;
+    A.tile_base[e] = (uint32_t)tiles;
+    off += el_count[e];
+    tiles += (el_count[e] + kTT - 1) / kTT;
+  }
+  A.el_off[n_el] = (uint32_t)off;
+  A.tile_base[n_el] = (uint32_t)tiles;
+  A.ntiles = (uint32_t)tiles;
+  unsigned long long items = 0;
+  unsigned p = 0;
+  for (unsigned a = 0; a < n_el; ++a)
+    for (unsigned b = a; b < n_el; ++b, ++p) {
+      const uint64_t ta = A.tile_base[a + 1] - A.tile_base[a];
+      const uint64_t tb = A.tile_base[b + 1] - A.tile_base[b];
+      A.item_off[p] = items;
+      items += a == b ? ta * (ta + 1) / 2 : ta * tb;
+    }
+  A.item_off[ntypes] = items;
+  A.nitems = items;
+  size_t tiles_at, items_at;
+  const size_t need = scratch_need((size_t)tiles, items, &tiles_at, &items_at);
+  if (need > kScratchBytes || items >= (1ull << 31))
+    return fail(XS_ERR_RANGE, "xs_debye_pairsum_noalloc: " + std::to_string(tiles) + " tiles and " +
+                                  std::to_string(items) + " tile pairs need " + std::to_string(need) +
+                                  " bytes of tables, beyond the " + std::to_string(kScratchBytes) +
+                                  "-byte scratch (about 65,000 atoms); use xs_debye_pairsum");
+  void *base = nullptr;
+  XS_TRY(scratch_ptr(ctx->device, &base));
+  char *sp = static_cast<char *>(base);
+  auto *mesh = reinterpret_cast<MeshInfo *>(sp);
+  auto *td = reinterpret_cast<Tile *>(sp + tiles_at);
+  auto *idv = reinterpret_cast<Item *>(sp + items_at);
+  hipStream_t st = ctx->stream;
+  const size_t ssize = (size_t)ntypes * nq;
+  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)std::min<size_t>((ssize + 255) / 256, 4096)),
+                     dim3(256), 0, st, S, ssize);
+  hipLaunchKernelGGL(mesh_info_kernel, dim3(1), dim3(256), 0, st, q, nq, mesh);
+  const unsigned long long nt = std::max<unsigned long long>(items, tiles);
+  if (nt) {
+    hipLaunchKernelGGL(table_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, A, td,
+                       idv);
+  }
+  if (items) {
+    dim3 grid((unsigned)items, (nq + kQBlock - 1) / kQBlock);
+    hipLaunchKernelGGL(debye_kernel<true>, grid, dim3(kBlock), 0, st, xyz, xyz + n, xyz + 2 * n, td,
+                       idv, q, nq, 0.0, 1u, S, mesh);
+    hipLaunchKernelGGL(debye_kernel<false>, grid, dim3(kBlock), 0, st, xyz, xyz + n, xyz + 2 * n, td,
+                       idv, q, nq, 0.0, 1u, S, mesh);
+  }
+  XS_HIP_CHECK(hipGetLastError());
+  return XS_OK;
+}
+
 }  // namespace debye
+
+// Called by xs_init with the context's device current.
+int debye_scratch_init(int device) {
+  void *p = nullptr;
+  return debye::scratch_ptr(device, &p);
+}
+
 }  // namespace xs
 
 using namespace xs;
@@ -292,4 +506,16 @@ extern "C" int xs_partial_debye_pairsum(xs_ctx *ctx, const double *xyz, size_t n
   }
   return debye::run(ctx, xyz, n_atoms, el_count, n_el, blk_count, nblocks, q, nq, S,
                     flags & XS_DEVICE_PTRS);
+}
+
+extern "C" int xs_debye_pairsum_noalloc(xs_ctx *ctx, const double *xyz, size_t n_atoms,
+                                        const unsigned *el_count, unsigned n_el, const double *q,
+                                        unsigned nq, double *S, unsigned flags) {
+  XS_TRY(activate(ctx));
+  XS_TRY(debye::check_args(xyz, n_atoms, el_count, n_el, q, nq, S));
+  if (!(flags & XS_DEVICE_PTRS))
+    return fail(XS_ERR_ARG, "xs_debye_pairsum_noalloc: needs device pointers (XS_DEVICE_PTRS)");
+  if (nq > 65535u * (unsigned)debye::kQBlock)
+    return fail(XS_ERR_RANGE, "xs_debye_pairsum_noalloc: nq too large");
+  return debye::run_noalloc(ctx, xyz, n_atoms, el_count, n_el, q, nq, S);
 }

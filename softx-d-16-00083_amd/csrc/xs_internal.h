@@ -159,6 +159,10 @@ int batch_hist_launch(xs_batch *b, unsigned long long *hist_d, bool prof);
 // Free a built ensemble's device memory and unregister it from its context.
 void destroy_atoms(xs_atoms *a);
 
+// Device address of the allocation-free Debye entry point's scratch array,
+// taken once per device (the device must be current).
+int debye_scratch_init(int device);
+
 // Device allocation owned by one call (freed on scope exit).
 struct DevMem {
   void *p = nullptr;

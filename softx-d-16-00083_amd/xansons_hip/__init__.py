@@ -67,6 +67,7 @@ def lib():
         L.xs_pdf.argtypes = [vp, vp, u, vp, u, d, i, d, vp, vp, u]
         L.xs_debye_pairsum.argtypes = [vp, vp, sz, vp, u, vp, u, vp, u]
         L.xs_partial_debye_pairsum.argtypes = [vp, vp, sz, vp, u, vp, u, vp, u, vp, u]
+        L.xs_debye_pairsum_noalloc.argtypes = [vp, vp, sz, vp, u, vp, u, vp, u]
         L.xs_amp2d.argtypes = [vp, vp, sz, vp, u, vp, sz, vp, u, vp, vp, u]
         L.xs_batch_plan.argtypes = [vp, u, vp, vp, vp, d, vp, u, C.POINTER(vp)]
         L.xs_batch_info.argtypes = [vp, vp, vp, vp, vp, vp]
@@ -231,6 +232,13 @@ class Context:
         ec = _u32(el_count)
         _check(lib().xs_debye_pairsum(self.h, C.c_void_p(xyz_ptr), int(n), _p(ec), len(ec),
                                       C.c_void_p(q_ptr), int(nq), C.c_void_p(S_ptr), XS_DEVICE_PTRS))
+
+    def debye_pairsum_noalloc_dev(self, xyz_ptr, n, el_count, q_ptr, nq, S_ptr):
+        """xs_debye_pairsum_noalloc: device pointers, no allocation, no wait."""
+        ec = _u32(el_count)
+        _check(lib().xs_debye_pairsum_noalloc(self.h, C.c_void_p(xyz_ptr), int(n), _p(ec), len(ec),
+                                              C.c_void_p(q_ptr), int(nq), C.c_void_p(S_ptr),
+                                              XS_DEVICE_PTRS))
 
     def partial_debye_pairsum(self, xyz, el_count, blk_count, nblocks, q):
         ec, bc = _u32(el_count), _u32(blk_count)
