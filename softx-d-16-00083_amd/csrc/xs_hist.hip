@@ -40,9 +40,11 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include <hipcub/hipcub.hpp>
@@ -1602,7 +1604,10 @@ static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const
   if (ntot_all >= (size_t(1) << 32))
     return fail(XS_ERR_RANGE, "xs_batch_plan: an ensemble must hold fewer than 2^32 atoms");
   const size_t nc = std::max<size_t>(ntot_all, 1);
-  std::vector<T> sorted(3 * nc);
+  // every element is written by the workers below: no value-initialisation
+  // (a serial pass over 24 bytes per atom before the parallel part)
+  std::unique_ptr<T[]> sorted_buf(new T[3 * nc]);
+  T *const sorted = sorted_buf.get();
 
   // 2. per structure, on worker threads (largest first): bounding box, bin
   //    count, k-d tiles; sorted coordinates straight into their place
@@ -1637,12 +1642,18 @@ static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const
         continue;
       }
       P.nh = nh;
-      tmp.resize(3 * n);
-      for (size_t i = 0; i < 3 * n; ++i) tmp[i] = (T)X[i];
+      const T *src;
+      if constexpr (std::is_same<T, double>::value) {
+        src = X;
+      } else {
+        tmp.resize(3 * n);
+        for (size_t i = 0; i < 3 * n; ++i) tmp[i] = (T)X[i];
+        src = tmp.data();
+      }
       if (sizeof(T) < sizeof(double)) {
         // coordinates beyond the float range become infinite in FP32 binning
         double fb[6];
-        host_bbox<T>(tmp.data(), n, fb);
+        host_bbox<T>(src, n, fb);
         P.rc = check_span(fb, bin);
         if (P.rc != XS_OK) {
           P.err = xs_last_error();
@@ -1650,9 +1661,9 @@ static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const
         }
       }
       const size_t a0 = abase[s];
-      sort_structure_host<T>(tmp.data(), n, el_count + ecf[s], n_el[s], bin, nh, sorted.data() + a0,
-                             sorted.data() + nc + a0, sorted.data() + 2 * nc + a0, (uint32_t)a0, P.tiles,
-                             P.boxes, P.ebase, P.en);
+      sort_structure_host<T>(src, n, el_count + ecf[s], n_el[s], bin, nh, sorted + a0,
+                             sorted + nc + a0, sorted + 2 * nc + a0, (uint32_t)a0, P.tiles, P.boxes,
+                             P.ebase, P.en);
     }
   };
   {
@@ -1730,7 +1741,7 @@ static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const
   }
   T *c = static_cast<T *>(b->coords);
   hipError_t e = hipStreamSynchronize(ctx->stream);
-  if (e == hipSuccess) e = hipMemcpy(c, sorted.data(), 3 * nc * sizeof(T), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c, sorted, 3 * nc * sizeof(T), hipMemcpyHostToDevice);
   if (e == hipSuccess && !tiles.empty())
     e = hipMemcpy(b->tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice);
   if (e == hipSuccess && !boxes.empty())

@@ -128,77 +128,15 @@ __global__ void slice_counts(const unsigned long long *__restrict__ hist, unsign
   }
 }
 
-template <int NA, int NB, int NT>
-__global__ __launch_bounds__(256, 2) void pairsum_mfma_kernel(const v4i *__restrict__ As,
-                                                           const v4i *__restrict__ Bs, unsigned MT,
-                                                           unsigned NTn, unsigned KT, unsigned nq,
-                                                           unsigned ncols, double *__restrict__ S) {
-  constexpr int DMIN = -(NA - 1);
-  constexpr int DMAX = NB - 1;
-  constexpr int NG = DMAX - DMIN + 1;
-  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const unsigned ngrp = NTn / NT;
-  unsigned tile = blockIdx.x;
-  if ((gridDim.x & 7u) == 0) tile = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
-  const unsigned mg = tile / ngrp, ng = tile % ngrp;
-  const unsigned mt = mg * 4 + wave;
-  const size_t sA = (size_t)MT * KT * 64, sB = (size_t)NTn * KT * 64;
-  const v4i *ap = As + (size_t)mt * KT * 64 + lane;
-  const v4i *bp = Bs + (size_t)ng * NT * KT * 64 + lane;
-
-  // The int32 accumulators go to S (FP64) every kFlush k tiles: the wave owns
-  // its output elements, so the first flush stores and later ones add.  No
-  // FP64 output registers and no operand double buffer, so that two waves fit
-  // on a SIMD and hide each other's operand loads.
-  for (unsigned kb = 0; kb < KT; kb += kFlush) {
-    const unsigned ke = min(KT, kb + kFlush);
-    v16i acc[NG][NT];
-#pragma unroll
-    for (int g = 0; g < NG; ++g)
-#pragma unroll
-      for (int t = 0; t < NT; ++t) acc[g][t] = v16i{0};
-    for (unsigned kt = kb; kt < ke; ++kt) {
-      v4i a[NA], b[NB][NT];
-#pragma unroll
-      for (int i = 0; i < NA; ++i) a[i] = ap[i * sA + (size_t)kt * 64];
-#pragma unroll
-      for (int j = 0; j < NB; ++j)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) b[j][t] = bp[j * sB + ((size_t)t * KT + kt) * 64];
-#pragma unroll
-      for (int i = 0; i < NA; ++i)
-#pragma unroll
-        for (int j = 0; j < NB; ++j)
-#pragma unroll
-          for (int t = 0; t < NT; ++t)
-            acc[j - i - DMIN][t] =
-                __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j][t], acc[j - i - DMIN][t], 0, 0, 0);
-    }
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const unsigned col = (ng * NT + t) * 32 + (lane & 31);
-      if (col >= ncols) continue;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const unsigned row = mt * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-        if (row >= nq) continue;
-        double v = 0.0;
-#pragma unroll
-        for (int g = 0; g < NG; ++g) v = fma((double)acc[g][t][e], ldexp(1.0, 7 * (g + DMIN) - 6), v);
-        double *o = S + (size_t)col * nq + row;
-        *o = kb == 0 ? v : *o + v;
-      }
-    }
-  }
-}
-
-// The same product with the operand tiles of the next k tile fetched into
-// LDS while the current one is multiplied.  Measured on the plain kernel
-// above: it fetches 1.4 GB from HBM per launch of the 768-histogram
-// workload (1.2 TB/s), hits L2 for 87% of its requests at a quarter of L2
-// bandwidth, and keeps the MFMA pipe busy 35% of the time -- it waits on
-// load latency, with two waves per SIMD and no registers to spare for a
-// prefetch.  Here every wave owns two LDS stages of NA + NB NT tiles (1 KiB
+// The product, with the operand tiles of the next k tile fetched into LDS
+// while the current one is multiplied.  The same kernel with its operands
+// loaded straight into registers fetched 1.4 GB from HBM per launch of the
+// 768-histogram workload (1.2 TB/s), hit L2 for 87% of its requests at a
+// quarter of L2 bandwidth, and kept the MFMA pipe busy 35% of the time: it
+// waited on load latency, with two waves per SIMD and no registers to spare
+// for a prefetch.  The int32 accumulators go to S (FP64) every kFlush k
+// tiles; the wave owns its output elements, so the first flush stores and
+// later ones add.  Here every wave owns two LDS stages of NA + NB NT tiles (1 KiB
 // each, the pre-tiled layout, so a global_load_lds_dwordx4 of a wave writes
 // exactly one tile: lane l's 16 bytes at 16 l); the loads of k tile kt + 1
 // are issued into the other stage before the products of kt, which read
