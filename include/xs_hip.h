@@ -235,7 +235,12 @@ int xs_pairlist_pairsum(xs_ctx *ctx, const double *r, const unsigned *type, size
  * float and binned as calcHistKernel (CalcFunctionsCUDA.cu:832-833), with
  * the quotient form chosen as in xs_hist (XS_F32_DOUBLE_QUOTIENT applies).
  * nhist[s] = 0 (or nhist == NULL) takes GetHistSize's bin count
- * (CalcFunctions.cpp:224-243).  Each structure needs at least one atom. */
+ * (CalcFunctions.cpp:224-243).  Each structure needs at least one atom.
+ * With XS_DEVICE_PTRS the xyz[s] are device pointers (the array xyz itself
+ * is host memory) and the structures are tiled on the GPU (radix sorts over
+ * all atoms at once; fewer than 2^31 atoms in all); otherwise they are
+ * tiled on the host's cores and uploaded.  Either way the call returns
+ * when the plan is ready, and the coordinates are not needed afterwards. */
 int xs_batch_plan(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el,
                   const unsigned *el_count, const double *const *xyz,
                   double bin, const unsigned *nhist, unsigned flags,

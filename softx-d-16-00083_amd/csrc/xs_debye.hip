@@ -13,12 +13,13 @@
 // the series in xs_sincos.h, advanced one q by the stored rotation and then
 // by the three-term recurrence s_{k+1} = 2 cos(dq r) s_k - s_{k-1} (one FMA;
 // a rounding error made at step j grows by |sin((k-j) dq r)/sin(dq r)| <=
-// k-j, so over 32 q it stays near 1e-14), so a pair-q term costs about 3 FP64
+// k-j, so over 32 q it stays near 1e-14), so a pair-q term costs 2 FP64
 // instructions plus 1/32 of the seed.  FP64 because the intensity at low q
 // is a strongly cancelling difference (see xs_iq.hip).  The 1e-8 of the
-// denominator is applied to first order through a second accumulator of
-// sin(q r)/r^2.  The half-waves' sums are added by a cross-lane shuffle, the
-// four waves' in LDS, and the result to S with FP64 atomics.
+// denominator is applied to first order through the pair's weight
+// 1/r - (1e-8/q_c)/r^2, one weight for every 4 neighbouring q of a lane.
+// The half-waves' sums are added by a cross-lane shuffle, the four waves'
+// in LDS, and the result to S with FP64 atomics.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -33,6 +34,7 @@ namespace debye {
 
 constexpr int kTT = 32;       // atoms per tile
 constexpr int kQPT = 32;      // q per lane
+constexpr int kSub = 4;       // q per denominator weight (uniform mesh)
 constexpr int kWaves = 4;
 constexpr int kBlock = 64 * kWaves;
 constexpr int kQBlock = 32 * kQPT;  // q per work group (a half-wave covers them)
@@ -101,56 +103,95 @@ __global__ __launch_bounds__(kBlock) void debye_kernel(const double *__restrict_
 
   // lane: 32 consecutive q of the block (lane & 31) for every other pair of
   // its wave's quarter (lane >> 5), so that the seed of the recurrence is
-  // spread over 32 q
+  // spread over 32 q.
+  // The denominator: sin(q r)/(q r + e) = (sin(q r)/q) (1/r - (e/q)/r^2)
+  // to first order (e = 1e-8; the second order is below 1e-16 relative for
+  // q r > 0.1).  Each term is weighted by w = 1/r - (e/q)/r^2, the pair's
+  // weight for its q.  On a uniform mesh one weight serves kSub neighbouring
+  // q, taken at their centre q_c: (e/q)/r^2 is then off by a factor within
+  // 1 +- 1.5 dq/q, so the 1e-8 correction (of order 1e-6 of I(q) where the
+  // low-q intensity cancels, less elsewhere) moves by a few percent of itself
+  // at the lowest q of a mesh and by less above.
+  // One FMA per q for the weights of kSub q, two per term (sum, recurrence).
   const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5;
   const unsigned iq0 = blockIdx.y * kQBlock + (lane & 31) * kQPT;
-  double acc[kQPT], corr[kQPT];
+  double acc[kQPT];
 #pragma unroll
-  for (int m = 0; m < kQPT; ++m) acc[m] = corr[m] = 0.0;
+  for (int m = 0; m < kQPT; ++m) acc[m] = 0.0;
   const double q0 = iq0 < nq ? q[iq0] : 0.0;
+  double ec[kQPT / kSub];  // e / q_c per group of kSub q (uniform mesh)
+  double qv[UNIFORM ? 1 : kQPT], eq[UNIFORM ? 1 : kQPT];  // q and e / q (any mesh)
+  if (UNIFORM) {
+#pragma unroll
+    for (int g = 0; g < kQPT / kSub; ++g) {
+      const double qc = fma((double)(g * kSub) + 0.5 * (kSub - 1), dq, q0);
+      ec[g] = qc > 0.0 ? 1e-8 / qc : 0.0;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kQPT; ++k) {
+      qv[k] = iq0 + k < nq ? q[iq0 + k] : 0.0;
+      eq[k] = qv[k] > 0.0 ? 1e-8 / qv[k] : 0.0;
+    }
+  }
   const unsigned pstart = wave * (kTT * kTT / kWaves), pend = pstart + kTT * kTT / kWaves;
-  for (unsigned m = pstart + half; m < pend; m += 2) {
-    const double4 v = pr[m];
-    if (v.y == 0.0) continue;
-    const double ir = v.y, ir2 = v.y * v.y;
-    if (UNIFORM) {
+  if (UNIFORM) {
+    // two pairs per step, so that their recurrences (each a chain of
+    // dependent FMAs) interleave; an empty pair (v.y = 0) has weight 0
+    for (unsigned m = pstart + half; m < pend; m += 4) {
+      const double4 va = pr[m], vb = pr[m + 2];
+      const double ira = va.y, ira2 = va.y * va.y, irb = vb.y, irb2 = vb.y * vb.y;
+      double wa[kQPT / kSub], wb[kQPT / kSub];
+#pragma unroll
+      for (int g = 0; g < kQPT / kSub; ++g) {
+        wa[g] = fma(-ec[g], ira2, ira);
+        wb[g] = fma(-ec[g], irb2, irb);
+      }
       // s_0 = sin(q0 r), s_1 by one rotation, then the three-term
       // recurrence s_{k+1} = 2 cos(dq r) s_k - s_{k-1} (one FMA a q)
-      double s0, c0;
-      sincos_2pi(q0 * v.x, &s0, &c0);
-      double sp = s0, s = fma(s0, v.w, c0 * v.z);
-      const double c2 = 2.0 * v.w;
-      acc[0] = fma(sp, ir, acc[0]);
-      corr[0] = fma(sp, ir2, corr[0]);
+      double sa0, ca0, sb0, cb0;
+      sincos_2pi(q0 * va.x, &sa0, &ca0);
+      sincos_2pi(q0 * vb.x, &sb0, &cb0);
+      double spa = sa0, sa = fma(sa0, va.w, ca0 * va.z);
+      double spb = sb0, sb = fma(sb0, vb.w, cb0 * vb.z);
+      const double c2a = 2.0 * va.w, c2b = 2.0 * vb.w;
+      acc[0] = fma(spa, wa[0], acc[0]);
+      acc[0] = fma(spb, wb[0], acc[0]);
 #pragma unroll
       for (int k = 1; k < kQPT; ++k) {
-        acc[k] = fma(s, ir, acc[k]);
-        corr[k] = fma(s, ir2, corr[k]);
+        acc[k] = fma(sa, wa[k / kSub], acc[k]);
+        acc[k] = fma(sb, wb[k / kSub], acc[k]);
         if (k + 1 < kQPT) {
-          const double sn = fma(c2, s, -sp);
-          sp = s;
-          s = sn;
+          const double sna = fma(c2a, sa, -spa), snb = fma(c2b, sb, -spb);
+          spa = sa;
+          sa = sna;
+          spb = sb;
+          sb = snb;
         }
       }
-    } else {
+    }
+  } else {
+    for (unsigned m = pstart + half; m < pend; m += 2) {
+      const double4 v = pr[m];
+      if (v.y == 0.0) continue;
+      const double ir = v.y, ir2 = v.y * v.y;
+      // any mesh: the weight of each q itself
 #pragma unroll
       for (int k = 0; k < kQPT; ++k) {
-        const double qk = iq0 + k < nq ? q[iq0 + k] : 0.0;
         double s, c;
-        sincos_2pi(qk * v.x, &s, &c);
-        acc[k] = fma(s, ir, acc[k]);
-        corr[k] = fma(s, ir2, corr[k]);
+        sincos_2pi(qv[k] * v.x, &s, &c);
+        acc[k] = fma(s, fma(-eq[k], ir2, ir), acc[k]);
       }
     }
   }
   // sum over the half-waves (lane l + 32 onto lane l, a cross-lane shuffle)
   // and the waves: waves 1..3 park their 32 x 32 sums of
-  // sin/(q r) - 1e-8 sin/(q r)^2 in LDS (3 x 32 x 32 doubles, 24 KiB)
+  // sin(q r)/(q r + 1e-8) in LDS (3 x 32 x 32 doubles, 24 KiB)
   double res[kQPT];
 #pragma unroll
   for (int k = 0; k < kQPT; ++k) {
     const double qk = iq0 + k < nq ? q[iq0 + k] : 0.0;
-    res[k] = qk == 0.0 ? 0.0 : (acc[k] - (1e-8 / qk) * corr[k]) / qk;
+    res[k] = qk == 0.0 ? 0.0 : acc[k] / qk;
     res[k] += __shfl_down(res[k], 32, 64);
   }
   __syncthreads();

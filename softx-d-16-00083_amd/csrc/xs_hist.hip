@@ -37,6 +37,7 @@
 //   xs_batch_run (xs_batch.hip): zero fills, histograms, pair sums and PDFs
 //     are launches only, optionally captured once into a hipGraph.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -1564,6 +1565,59 @@ static void free_plan(xs_batch *b) {
   delete b;
 }
 
+// The tail of a plan, host-built or device-built: work items (the
+// 8192-bin window class, then the 32768 class, each longest first), the
+// device tables and overflow counters, and, for a host-built plan, the
+// upload of its sorted coordinates (host_sorted, 3 b->natoms values; a
+// device-built plan has filled b->coords already).  Frees b on failure.
+template <class T>
+static int plan_finish(xs_ctx *ctx, xs_batch *b, const std::vector<Tile> &tiles,
+                       const std::vector<double> &boxes, const std::vector<TypeDesc> &types,
+                       unsigned long long hoff, xs_batch **out, const T *host_sorted) {
+  b->hist_len = hoff;
+  b->ntypes_total = types.size();
+  std::vector<Item> all, small, large;
+  add_items(types, 0, (uint32_t)types.size(), all, sizeof(T) == 8);
+  for (const Item &it : all) (types[it.type].nhist <= 8192 ? small : large).push_back(it);
+  order_items(small, types, tiles, sizeof(T) == 8);
+  order_items(large, types, tiles, sizeof(T) == 8);
+  b->items8192 = small.size();
+  b->items32768 = large.size();
+  small.insert(small.end(), large.begin(), large.end());
+  const size_t nc = b->natoms;
+  const bool ok =
+      (b->coords || hipMalloc(&b->coords, 3 * nc * sizeof(T)) == hipSuccess) &&
+      hipMalloc(&b->tiles, std::max<size_t>(tiles.size(), 1) * sizeof(Tile)) == hipSuccess &&
+      hipMalloc(&b->boxes, std::max<size_t>(boxes.size(), 6) * 8) == hipSuccess &&
+      hipMalloc(&b->types, std::max<size_t>(types.size(), 1) * sizeof(TypeDesc)) == hipSuccess &&
+      hipMalloc(&b->items, std::max<size_t>(small.size(), 1) * sizeof(Item)) == hipSuccess &&
+      hipMalloc((void **)&b->ovf, std::max<size_t>(b->nstruct, 1) * 8) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    free_plan(b);
+    return fail(XS_ERR_NOMEM, "xs_batch_plan: device allocation failed");
+  }
+  T *c = static_cast<T *>(b->coords);
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess && host_sorted)
+    e = hipMemcpy(c, host_sorted, 3 * nc * sizeof(T), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !tiles.empty())
+    e = hipMemcpy(b->tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !boxes.empty())
+    e = hipMemcpy(b->boxes, boxes.data(), boxes.size() * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !types.empty())
+    e = hipMemcpy(b->types, types.data(), types.size() * sizeof(TypeDesc), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !small.empty())
+    e = hipMemcpy(b->items, small.data(), small.size() * sizeof(Item), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    free_plan(b);
+    return fail(XS_ERR_HIP, std::string("xs_batch_plan: upload failed: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  ctx->plans.insert(b);
+  return XS_OK;
+}
+
 // Per-structure part of a batch plan, built on a worker thread.
 struct StructPart {
   int rc = XS_OK;
@@ -1715,48 +1769,423 @@ static int plan_build(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el, const
     for (unsigned e = 0; e < ne; ++e) b->el_count.push_back(el_count[ecf[s] + e]);
     hoff += (unsigned long long)ntypes_of(ne) * nh;
   }
-  b->hist_len = hoff;
-  b->ntypes_total = types.size();
-  // work items: the 8192-bin window class, then the 32768 class, each longest first
-  std::vector<Item> all, small, large;
-  add_items(types, 0, (uint32_t)types.size(), all, sizeof(T) == 8);
-  for (const Item &it : all) (types[it.type].nhist <= 8192 ? small : large).push_back(it);
-  order_items(small, types, tiles, sizeof(T) == 8);
-  order_items(large, types, tiles, sizeof(T) == 8);
-  b->items8192 = small.size();
-  b->items32768 = large.size();
-  small.insert(small.end(), large.begin(), large.end());
   b->natoms = nc;
-  const bool ok =
-      hipMalloc(&b->coords, 3 * nc * sizeof(T)) == hipSuccess &&
-      hipMalloc(&b->tiles, std::max<size_t>(tiles.size(), 1) * sizeof(Tile)) == hipSuccess &&
-      hipMalloc(&b->boxes, std::max<size_t>(boxes.size(), 6) * 8) == hipSuccess &&
-      hipMalloc(&b->types, std::max<size_t>(types.size(), 1) * sizeof(TypeDesc)) == hipSuccess &&
-      hipMalloc(&b->items, std::max<size_t>(small.size(), 1) * sizeof(Item)) == hipSuccess &&
-      hipMalloc((void **)&b->ovf, std::max<size_t>(nstruct, 1) * 8) == hipSuccess;
-  if (!ok) {
+  return plan_finish<T>(ctx, b, tiles, boxes, types, hoff, out, sorted);
+}
+
+// ---- batch plans from coordinates already on the device ----
+//
+// The same plan as plan_build, tiled on the GPU: every structure's
+// coordinates are gathered into one SoA array, and three stable radix sorts
+// over all atoms of all structures at once order them by (element segment,
+// x), then (slab, y), then (column, z), with 64-bit keys whose high word is
+// the segment / slab / column number and whose low word the order-keeping
+// bits of the coordinate as float.  Slabs and columns hold equal counts, so
+// the tiles follow from the counts alone (equal_count_tiles, on the host).
+// A structure whose equal-count tiles come out wider than its window allows
+// (an element that forms a hollow shell) is re-tiled on the host with the
+// k-d split, from its own coordinates, as plan_build does.
+
+struct SegDev {
+  uint32_t start, n;  // atoms [start, start + n) of the concatenated arrays
+  uint32_t gx, slab0;  // its slabs: global numbers slab0 .. slab0 + gx - 1
+};
+struct SlabDev {
+  uint32_t start, n;  // positions of the slab after the x sort
+  uint32_t gy, col0;  // its columns: global numbers col0 .. col0 + gy - 1
+};
+
+// Bits of (float)v whose unsigned order is the order of v.
+__device__ __forceinline__ uint32_t ord_bits(double v) {
+  const uint32_t u = __float_as_uint((float)v);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// raw[a], raw[N + a], raw[2N + a] = atom a of the concatenation of the
+// structures' own SoA arrays (structure s holds atoms abase[s] .. abase[s+1]).
+__global__ void batch_gather(const double *const *__restrict__ src,
+                             const unsigned long long *__restrict__ abase, unsigned nstruct,
+                             size_t ntot, double *__restrict__ raw) {
+  for (size_t a = blockIdx.x * (size_t)blockDim.x + threadIdx.x; a < ntot;
+       a += (size_t)gridDim.x * blockDim.x) {
+    unsigned lo = 0, hi = nstruct - 1;
+    while (lo < hi) {
+      const unsigned mid = (lo + hi + 1) / 2;
+      if (abase[mid] <= a) lo = mid;
+      else hi = mid - 1;
+    }
+    const size_t n = abase[lo + 1] - abase[lo], i = a - abase[lo];
+    const double *p = src[lo];
+    raw[a] = p[i];
+    raw[ntot + a] = p[n + i];
+    raw[2 * ntot + a] = p[2 * n + i];
+  }
+}
+
+// Bounding box of every segment (one work group each), NaN if one is NaN.
+__global__ void seg_boxes(const double *__restrict__ x, const double *__restrict__ y,
+                          const double *__restrict__ z, const SegDev *__restrict__ segs,
+                          double *__restrict__ box) {
+  const SegDev sg = segs[blockIdx.x];
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  int nan = 0;
+  for (uint32_t i = sg.start + threadIdx.x; i < sg.start + sg.n; i += blockDim.x) {
+    const double v[3] = {x[i], y[i], z[i]};
+    for (int a = 0; a < 3; ++a) {
+      nan |= v[a] != v[a];
+      lo[a] = fmin(lo[a], v[a]);
+      hi[a] = fmax(hi[a], v[a]);
+    }
+  }
+  __shared__ double sh[6][256];
+  for (int a = 0; a < 3; ++a) {
+    sh[a][threadIdx.x] = lo[a];
+    sh[3 + a][threadIdx.x] = hi[a];
+  }
+  const int any_nan = __syncthreads_or(nan);
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int a = 0; a < 3; ++a) {
+        sh[a][threadIdx.x] = fmin(sh[a][threadIdx.x], sh[a][threadIdx.x + w]);
+        sh[3 + a][threadIdx.x] = fmax(sh[3 + a][threadIdx.x], sh[3 + a][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) box[(size_t)blockIdx.x * 6 + threadIdx.x] = any_nan ? NAN : sh[threadIdx.x][0];
+}
+
+// Keys of the first sort: (segment, x), values the atom numbers.
+__global__ void seg_keys(const double *__restrict__ x, const SegDev *__restrict__ segs,
+                         unsigned long long *__restrict__ keys, uint32_t *__restrict__ vals) {
+  const SegDev sg = segs[blockIdx.x];
+  for (uint32_t i = sg.start + threadIdx.x; i < sg.start + sg.n; i += blockDim.x) {
+    keys[i] = ((unsigned long long)blockIdx.x << 32) | ord_bits(x[i]);
+    vals[i] = i;
+  }
+}
+
+// After the (segment, x) sort: the slab of each atom.
+__global__ void slab_labels(const unsigned long long *__restrict__ keys,
+                            const uint32_t *__restrict__ vals, const SegDev *__restrict__ segs,
+                            size_t ntot, uint32_t *__restrict__ lab) {
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < ntot;
+       p += (size_t)gridDim.x * blockDim.x) {
+    const SegDev sg = segs[keys[p] >> 32];
+    lab[vals[p]] = sg.slab0 + equal_part(p - sg.start, sg.n, sg.gx);
+  }
+}
+
+// After the (slab, y) sort: the column of each atom.
+__global__ void col_labels(const unsigned long long *__restrict__ keys,
+                           const uint32_t *__restrict__ vals, const SlabDev *__restrict__ slabs,
+                           size_t ntot, uint32_t *__restrict__ lab) {
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < ntot;
+       p += (size_t)gridDim.x * blockDim.x) {
+    const SlabDev sb = slabs[keys[p] >> 32];
+    lab[vals[p]] = sb.col0 + equal_part(p - sb.start, sb.n, sb.gy);
+  }
+}
+
+// Keys (label, coordinate) of the second and third sorts.
+__global__ void label_keys(const double *__restrict__ c, const uint32_t *__restrict__ lab,
+                           size_t ntot, unsigned long long *__restrict__ keys,
+                           uint32_t *__restrict__ vals) {
+  for (size_t a = blockIdx.x * (size_t)blockDim.x + threadIdx.x; a < ntot;
+       a += (size_t)gridDim.x * blockDim.x) {
+    keys[a] = ((unsigned long long)lab[a] << 32) | ord_bits(c[a]);
+    vals[a] = (uint32_t)a;
+  }
+}
+
+// The plan's coordinates in the final order (as T).
+template <class T>
+__global__ void gather_sorted(const double *__restrict__ raw, const uint32_t *__restrict__ ord,
+                              size_t ntot, size_t nc, T *__restrict__ out) {
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < ntot;
+       p += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t a = ord[p];
+    out[p] = (T)raw[a];
+    out[nc + p] = (T)raw[ntot + a];
+    out[2 * nc + p] = (T)raw[2 * ntot + a];
+  }
+}
+
+static unsigned grid_for(size_t n) {
+  return (unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 16384));
+}
+
+// One stable sort of (key, value) pairs over all atoms on the low `bits`
+// key bits (scratch from tmp, sized by the caller).
+static int sort_u64(xs_ctx *ctx, void *tmp, size_t tmp_bytes, const unsigned long long *kin,
+                    unsigned long long *kout, const uint32_t *vin, uint32_t *vout, size_t n,
+                    int bits) {
+  size_t b = tmp_bytes;
+  XS_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, b, kin, kout, vin, vout, (int)n, 0, bits,
+                                                  ctx->stream));
+  return XS_OK;
+}
+
+template <class T>
+static int plan_build_device(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el,
+                             const unsigned *el_count, const double *const *xyz, double bin,
+                             const unsigned *nhist_in, bool f64, bool quot, xs_batch **out) {
+  // 1. sizes (host)
+  std::vector<size_t> nat(nstruct), ecf(nstruct);
+  std::vector<unsigned long long> abase(nstruct + 1, 0ull);
+  size_t ec = 0, nseg = 0;
+  for (unsigned s = 0; s < nstruct; ++s) {
+    const unsigned ne = n_el[s];
+    if (ne == 0 || ne > (unsigned)kMaxEl)
+      return fail(XS_ERR_RANGE, "xs_batch_plan: n_el must be in [1, 16]");
+    size_t n = 0;
+    for (unsigned e = 0; e < ne; ++e) n += el_count[ec + e];
+    if (n == 0 || n >= (size_t(1) << 31))
+      return fail(XS_ERR_RANGE, "xs_batch_plan: a structure must hold 1 to 2^31-1 atoms");
+    nat[s] = n;
+    ecf[s] = ec;
+    abase[s + 1] = abase[s] + n;
+    ec += ne;
+    nseg += ne;
+  }
+  const size_t ntot = abase[nstruct];
+  if (ntot >= (size_t(1) << 31))
+    return fail(XS_ERR_RANGE, "xs_batch_plan: device tiling takes fewer than 2^31 atoms in all");
+  std::vector<SegDev> segs(nseg);
+  for (unsigned s = 0, g = 0; s < nstruct; ++s) {
+    uint32_t o = (uint32_t)abase[s];
+    for (unsigned e = 0; e < n_el[s]; ++e, ++g) {
+      segs[g] = SegDev{o, el_count[ecf[s] + e], 1u, 0u};
+      o += el_count[ecf[s] + e];
+    }
+  }
+
+  // 2. gather and per-segment boxes (one read-back)
+  hipStream_t st = ctx->stream;
+  DevMem raw, ptrs, segd, boxd;
+  XS_TRY(raw.alloc(3 * ntot * sizeof(double), "batch coordinates"));
+  XS_TRY(ptrs.alloc(nstruct * sizeof(void *) + (nstruct + 1) * 8, "batch pointers"));
+  XS_TRY(segd.alloc(nseg * sizeof(SegDev), "batch segments"));
+  XS_TRY(boxd.alloc(nseg * 6 * sizeof(double), "batch boxes"));
+  XS_HIP_CHECK(hipStreamSynchronize(st));
+  XS_HIP_CHECK(hipMemcpy(ptrs.p, xyz, nstruct * sizeof(void *), hipMemcpyHostToDevice));
+  auto *abase_d = reinterpret_cast<unsigned long long *>(static_cast<char *>(ptrs.p) +
+                                                         nstruct * sizeof(void *));
+  XS_HIP_CHECK(hipMemcpy(abase_d, abase.data(), (nstruct + 1) * 8, hipMemcpyHostToDevice));
+  XS_HIP_CHECK(hipMemcpy(segd.p, segs.data(), nseg * sizeof(SegDev), hipMemcpyHostToDevice));
+  double *rx = raw.at<double>(), *ry = rx + ntot, *rz = rx + 2 * ntot;
+  hipLaunchKernelGGL(batch_gather, dim3(grid_for(ntot)), dim3(256), 0, st,
+                     static_cast<const double *const *>(ptrs.p), abase_d, nstruct, ntot, rx);
+  hipLaunchKernelGGL(seg_boxes, dim3((unsigned)nseg), dim3(256), 0, st, rx, ry, rz,
+                     segd.at<SegDev>(), boxd.at<double>());
+  XS_HIP_CHECK(hipGetLastError());
+  std::vector<double> sbox(nseg * 6);
+  XS_HIP_CHECK(hipMemcpyAsync(sbox.data(), boxd.p, nseg * 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+  XS_HIP_CHECK(hipStreamSynchronize(st));
+
+  // 3. bin counts, window caps, slabs and columns (host)
+  std::vector<uint32_t> nh_of(nstruct);
+  std::vector<double> cap_of(nstruct);
+  std::vector<SlabDev> slabs;
+  std::vector<uint32_t> seg_gy(nseg), seg_nz(nseg);
+  uint32_t ncols = 0;
+  for (unsigned s = 0, g = 0; s < nstruct; ++s) {
+    double bb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    bool nan = false;
+    for (unsigned e = 0; e < n_el[s]; ++e) {
+      const double *b = &sbox[(g + e) * 6];
+      for (int a = 0; a < 3; ++a) {
+        nan |= !(b[a] == b[a]) || !(b[3 + a] == b[3 + a]);
+        bb[a] = std::min(bb[a], b[a]);
+        bb[3 + a] = std::max(bb[3 + a], b[3 + a]);
+      }
+    }
+    if (nan)
+      for (int a = 0; a < 6; ++a) bb[a] = NAN;
+    XS_TRY(check_span(bb, bin));
+    uint32_t nh = nhist_in ? nhist_in[s] : 0u;
+    if (!nh) {
+      // GetHistSize (CalcFunctions.cpp:224-243) over the union of the element boxes
+      const double dx = bb[3] - bb[0], dy = bb[4] - bb[1], dz = bb[5] - bb[2];
+      nh = (uint32_t)(std::sqrt(dx * dx + dy * dy + dz * dz) / bin) + 1;
+    }
+    if (!f64) {
+      if (nh > (1u << 24)) return fail(XS_ERR_RANGE, "xs_batch_plan: nhist > 2^24 needs XS_F64_BINNING");
+      double fb[6];
+      for (int a = 0; a < 6; ++a) fb[a] = (double)(float)bb[a];
+      XS_TRY(check_span(fb, bin));  // coordinates beyond the float range
+    }
+    nh_of[s] = nh;
+    cap_of[s] = cell_cap(bin, nh, wcap_for(nh));
+    for (unsigned e = 0; e < n_el[s]; ++e, ++g) {
+      uint32_t gx, gy, nz;
+      choose_equal_count(segs[g].n, &sbox[g * 6], cap_of[s], sizeof(T) == 8, gx, gy, nz);
+      segs[g].gx = gx;
+      segs[g].slab0 = (uint32_t)slabs.size();
+      seg_gy[g] = gy;
+      seg_nz[g] = nz;
+      for (uint64_t sl = 0; sl < gx; ++sl) {
+        const uint32_t a0 = (uint32_t)(sl * segs[g].n / gx), a1 = (uint32_t)((sl + 1) * segs[g].n / gx);
+        slabs.push_back(SlabDev{segs[g].start + a0, a1 - a0, gy, ncols});
+        ncols += gy;
+      }
+    }
+  }
+
+  // 4. the three sorts (device)
+  DevMem slabd, keys, vals, lab, tmp;
+  size_t tb = 0;
+  XS_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (const unsigned long long *)nullptr,
+                                                  (unsigned long long *)nullptr,
+                                                  (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                                  (int)ntot, 0, 64, st));
+  XS_TRY(slabd.alloc(std::max<size_t>(slabs.size(), 1) * sizeof(SlabDev), "batch slabs"));
+  XS_TRY(keys.alloc(2 * ntot * 8, "batch sort keys"));
+  XS_TRY(vals.alloc(2 * ntot * 4, "batch sort values"));
+  XS_TRY(lab.alloc(ntot * 4, "batch labels"));
+  XS_TRY(tmp.alloc(tb, "batch sort scratch"));
+  XS_HIP_CHECK(hipMemcpy(segd.p, segs.data(), nseg * sizeof(SegDev), hipMemcpyHostToDevice));
+  XS_HIP_CHECK(hipMemcpy(slabd.p, slabs.data(), slabs.size() * sizeof(SlabDev), hipMemcpyHostToDevice));
+  unsigned long long *k0 = keys.at<unsigned long long>(), *k1 = k0 + ntot;
+  uint32_t *v0 = vals.at<uint32_t>(), *v1 = v0 + ntot, *lb = lab.at<uint32_t>();
+  hipLaunchKernelGGL(seg_keys, dim3((unsigned)nseg), dim3(256), 0, st, rx, segd.at<SegDev>(), k0, v0);
+  XS_TRY(sort_u64(ctx, tmp.p, tb, k0, k1, v0, v1, ntot, 32 + bits_for(nseg)));
+  hipLaunchKernelGGL(slab_labels, dim3(grid_for(ntot)), dim3(256), 0, st, k1, v1, segd.at<SegDev>(),
+                     ntot, lb);
+  hipLaunchKernelGGL(label_keys, dim3(grid_for(ntot)), dim3(256), 0, st, ry, lb, ntot, k0, v0);
+  XS_TRY(sort_u64(ctx, tmp.p, tb, k0, k1, v0, v1, ntot, 32 + bits_for(slabs.size())));
+  hipLaunchKernelGGL(col_labels, dim3(grid_for(ntot)), dim3(256), 0, st, k1, v1, slabd.at<SlabDev>(),
+                     ntot, lb);
+  hipLaunchKernelGGL(label_keys, dim3(grid_for(ntot)), dim3(256), 0, st, rz, lb, ntot, k0, v0);
+  XS_TRY(sort_u64(ctx, tmp.p, tb, k0, k1, v0, v1, ntot, 32 + bits_for(std::max(ncols, 1u))));
+  XS_HIP_CHECK(hipGetLastError());
+
+  // 5. the plan: sorted coordinates, equal-count tiles and their boxes
+  auto *b = new xs_batch();
+  b->ctx = ctx;
+  b->f64 = f64;
+  b->quot_f32 = quot;
+  b->bin = bin;
+  b->nstruct = nstruct;
+  const size_t nc = ntot;
+  b->natoms = nc;
+  std::vector<Tile> tiles;
+  std::vector<uint32_t> tile0(nstruct + 1), ebase_all(nseg), en_all(nseg);
+  for (unsigned s = 0, g = 0; s < nstruct; ++s) {
+    tile0[s] = (uint32_t)tiles.size();
+    for (unsigned e = 0; e < n_el[s]; ++e, ++g) {
+      ebase_all[g] = (uint32_t)tiles.size();
+      equal_count_tiles(segs[g].start, segs[g].n, segs[g].gx, seg_gy[g], seg_nz[g], sizeof(T) == 8,
+                        tiles);
+      en_all[g] = (uint32_t)tiles.size() - ebase_all[g];
+    }
+  }
+  tile0[nstruct] = (uint32_t)tiles.size();
+  DevMem tiled, boxes_d;
+  if (hipMalloc(&b->coords, 3 * nc * sizeof(T)) != hipSuccess) {
     (void)hipGetLastError();
     free_plan(b);
     return fail(XS_ERR_NOMEM, "xs_batch_plan: device allocation failed");
   }
   T *c = static_cast<T *>(b->coords);
-  hipError_t e = hipStreamSynchronize(ctx->stream);
-  if (e == hipSuccess) e = hipMemcpy(c, sorted, 3 * nc * sizeof(T), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !tiles.empty())
-    e = hipMemcpy(b->tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !boxes.empty())
-    e = hipMemcpy(b->boxes, boxes.data(), boxes.size() * 8, hipMemcpyHostToDevice);
-  if (e == hipSuccess && !types.empty())
-    e = hipMemcpy(b->types, types.data(), types.size() * sizeof(TypeDesc), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !small.empty())
-    e = hipMemcpy(b->items, small.data(), small.size() * sizeof(Item), hipMemcpyHostToDevice);
+  hipLaunchKernelGGL((gather_sorted<T>), dim3(grid_for(ntot)), dim3(256), 0, st, rx, v1, ntot, nc, c);
+  int rc = tiled.alloc(std::max<size_t>(tiles.size(), 1) * sizeof(Tile), "batch tiles");
+  if (rc == XS_OK) rc = boxes_d.alloc(std::max<size_t>(tiles.size(), 1) * 48, "batch tile boxes");
+  if (rc != XS_OK) {
+    free_plan(b);
+    return rc;
+  }
+  std::vector<double> boxes(tiles.size() * 6);
+  hipError_t e = hipMemcpy(tiled.p, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL((hist_tile_boxes<T>), dim3((unsigned)tiles.size()), dim3(256), 0, st, c, c + nc,
+                       c + 2 * nc, tiled.at<Tile>(), boxes_d.at<double>());
+    e = hipMemcpyAsync(boxes.data(), boxes_d.p, boxes.size() * 8, hipMemcpyDeviceToHost, st);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) {
     free_plan(b);
-    return fail(XS_ERR_HIP, std::string("xs_batch_plan: upload failed: ") + hipGetErrorString(e));
+    return fail(XS_ERR_HIP, std::string("xs_batch_plan: device tiling failed: ") + hipGetErrorString(e));
   }
-  *out = b;
-  ctx->plans.insert(b);
-  return XS_OK;
+
+  // 6. structures whose tiles exceed their window cap: k-d tiles on the host
+  std::vector<std::vector<Tile>> kd_t(nstruct);
+  std::vector<std::vector<double>> kd_b(nstruct);
+  std::vector<std::array<uint32_t, kMaxEl>> kd_eb(nstruct), kd_en(nstruct);
+  std::vector<char> redo(nstruct, 0);
+  for (unsigned s = 0; s < nstruct; ++s) {
+    if (!std::isfinite(cap_of[s])) continue;
+    for (uint32_t t = tile0[s]; t < tile0[s + 1] && !redo[s]; ++t)
+      for (int a = 0; a < 3; ++a)
+        if (boxes[6 * t + 3 + a] - boxes[6 * t + a] > cap_of[s]) redo[s] = 1;
+  }
+  for (unsigned s = 0; s < nstruct; ++s) {
+    if (!redo[s]) continue;
+    const size_t n = nat[s];
+    std::vector<double> xs(3 * n);
+    std::vector<T> src(3 * n), sx(n), sy(n), sz(n);
+    e = hipMemcpy(xs.data(), xyz[s], 3 * n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      free_plan(b);
+      return fail(XS_ERR_HIP, std::string("xs_batch_plan: read-back failed: ") + hipGetErrorString(e));
+    }
+    for (size_t i = 0; i < 3 * n; ++i) src[i] = (T)xs[i];
+    sort_structure_host<T>(src.data(), n, el_count + ecf[s], n_el[s], bin, nh_of[s], sx.data(),
+                           sy.data(), sz.data(), (uint32_t)abase[s], kd_t[s], kd_b[s],
+                           kd_eb[s].data(), kd_en[s].data());
+    e = hipMemcpy(c + abase[s], sx.data(), n * sizeof(T), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c + nc + abase[s], sy.data(), n * sizeof(T), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(c + 2 * nc + abase[s], sz.data(), n * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      free_plan(b);
+      return fail(XS_ERR_HIP, std::string("xs_batch_plan: upload failed: ") + hipGetErrorString(e));
+    }
+  }
+
+  // 7. tables in structure order, as plan_build
+  std::vector<Tile> ftiles;
+  std::vector<double> fboxes;
+  std::vector<TypeDesc> types;
+  unsigned long long hoff = 0;
+  for (unsigned s = 0, g = 0; s < nstruct; ++s) {
+    const unsigned ne = n_el[s];
+    const uint32_t nh = nh_of[s];
+    const uint32_t tbase = (uint32_t)ftiles.size();
+    uint32_t ebase[kMaxEl], en[kMaxEl];
+    if (redo[s]) {
+      ftiles.insert(ftiles.end(), kd_t[s].begin(), kd_t[s].end());
+      fboxes.insert(fboxes.end(), kd_b[s].begin(), kd_b[s].end());
+      for (unsigned el = 0; el < ne; ++el) {
+        ebase[el] = tbase + kd_eb[s][el];
+        en[el] = kd_en[s][el];
+      }
+    } else {
+      ftiles.insert(ftiles.end(), tiles.begin() + tile0[s], tiles.begin() + tile0[s + 1]);
+      fboxes.insert(fboxes.end(), boxes.begin() + 6 * (size_t)tile0[s], boxes.begin() + 6 * (size_t)tile0[s + 1]);
+      for (unsigned el = 0; el < ne; ++el) {
+        ebase[el] = tbase + (ebase_all[g + el] - tile0[s]);
+        en[el] = en_all[g + el];
+      }
+    }
+    g += ne;
+    const size_t t0 = types.size();
+    add_types(ne, ebase, en, nh, bin, hoff, s, types);
+    b->type_first.push_back(t0);
+    for (size_t t = t0; t < types.size(); ++t) {
+      b->type_hist_off.push_back(types[t].hist_off);
+      b->type_nhist.push_back(nh);
+    }
+    b->pdf_off.push_back(b->pdf_len);
+    b->pdf_len += (unsigned long long)(1 + ntypes_of(ne)) * nh;
+    b->n_el.push_back(ne);
+    b->el_first.push_back(ecf[s]);
+    b->nhist.push_back(nh);
+    b->ntot.push_back((unsigned)nat[s]);
+    b->hist_off.push_back(hoff);
+    for (unsigned el = 0; el < ne; ++el) b->el_count.push_back(el_count[ecf[s] + el]);
+    hoff += (unsigned long long)ntypes_of(ne) * nh;
+  }
+  return plan_finish<T>(ctx, b, ftiles, fboxes, types, hoff, out, nullptr);
 }
 
 // Histogram launches of a plan (zero fills + kernels): no host work, capturable.
@@ -1855,12 +2284,19 @@ extern "C" int xs_batch_plan(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el
   if (!(bin > 0.0) || !std::isfinite(bin)) return fail(XS_ERR_ARG, "xs_batch_plan: bin must be > 0");
   for (unsigned s = 0; s < nstruct; ++s)
     if (!xyz[s]) return fail(XS_ERR_ARG, "xs_batch_plan: null structure");
+  const bool device = flags & XS_DEVICE_PTRS;
   if (flags & XS_F64_BINNING)
-    return hist::plan_build<double>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, true, false, plan);
+    return device ? hist::plan_build_device<double>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, true,
+                                                    false, plan)
+                  : hist::plan_build<double>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, true, false,
+                                             plan);
   bool quot = false;
   XS_TRY(hist::sqrt_f32_checked(ctx, bin));
   if (!(flags & XS_F32_DOUBLE_QUOTIENT)) XS_TRY(hist::quot_f32_exact(ctx, bin, &quot));
-  return hist::plan_build<float>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, false, quot, plan);
+  return device ? hist::plan_build_device<float>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, false,
+                                                 quot, plan)
+                : hist::plan_build<float>(ctx, nstruct, n_el, el_count, xyz, bin, nhist, false, quot,
+                                          plan);
 }
 
 extern "C" int xs_batch_free(xs_batch *plan) {

@@ -318,16 +318,23 @@ class Batch:
     element-major.  ``f64`` bins in double as the reference CPU path."""
 
     def __init__(self, ctx: Context, structures, bin_: float, nhist=None, f64: bool = True,
-                 double_quotient: bool = False):
+                 double_quotient: bool = False, device: bool = False):
+        """device=True: ``structures`` holds (device pointer of the (3, N)
+        float64 SoA, el_count) pairs and the plan is tiled on the GPU."""
         self.ctx = ctx
         self.bin = float(bin_)
         n_el = _u32([len(ec) for _, ec in structures])
         ecs = _u32(np.concatenate([np.asarray(ec, dtype=np.uint32) for _, ec in structures]))
-        self._xyz = [np.ascontiguousarray(x, dtype=np.float64) for x, _ in structures]
-        ptrs = (C.c_void_p * len(structures))(*[x.ctypes.data for x in self._xyz])
+        if device:
+            self._xyz = None
+            ptrs = (C.c_void_p * len(structures))(*[int(x) for x, _ in structures])
+        else:
+            self._xyz = [np.ascontiguousarray(x, dtype=np.float64) for x, _ in structures]
+            ptrs = (C.c_void_p * len(structures))(*[x.ctypes.data for x in self._xyz])
         nh = None if nhist is None else _u32(nhist)
         self.h = C.c_void_p()
-        flags = (XS_F64_BINNING if f64 else 0) | (XS_F32_DOUBLE_QUOTIENT if double_quotient else 0)
+        flags = (XS_F64_BINNING if f64 else 0) | (XS_F32_DOUBLE_QUOTIENT if double_quotient else 0) | \
+            (XS_DEVICE_PTRS if device else 0)
         _check(lib().xs_batch_plan(ctx.h, len(structures), _p(n_el), _p(ecs), ptrs, self.bin,
                                    _p(nh), flags, C.byref(self.h)))
         self._xyz = None

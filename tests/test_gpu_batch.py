@@ -97,6 +97,74 @@ def test_batch_f32_matches_oracle(bctx, ensemble, double_quotient):
     b.close()
 
 
+def device_plan(bctx, ens, **kw):
+    """A plan tiled on the GPU from device copies of the coordinates."""
+    import torch
+    import xansons_hip as xh
+    dev = torch.device("cuda", 0)
+    xd = [torch.from_numpy(np.ascontiguousarray(e.xyz64())).to(dev) for e in ens]
+    torch.cuda.synchronize()
+    b = xh.Batch(bctx, [(x.data_ptr(), e.el_count) for x, e in zip(xd, ens)], BIN, device=True, **kw)
+    del xd  # the plan keeps no reference to the input coordinates
+    return b
+
+
+@pytest.mark.parametrize("f64", [True, False])
+def test_device_plan_equals_host_plan(bctx, ensemble, f64):
+    """Tiling on the GPU (radix sorts over all structures) gives the same
+    bin counts, PDFs and pair sums as host tiling: the counts do not depend
+    on the tiling, and both equal the oracle."""
+    import xansons_hip as xh
+    q = st.q_mesh(0.5, 20.0, 300)
+    sl, rho = sl_for(ensemble), rho_for(ensemble)
+    bh = xh.Batch(bctx, [(e.xyz64(), e.el_count) for e in ensemble], BIN, f64=f64)
+    bd = device_plan(bctx, ensemble, f64=f64)
+    assert list(bd.nhist) == list(bh.nhist)
+    assert (bd.hist_off == bh.hist_off).all() and (bd.pdf_off == bh.pdf_off).all()
+    hh, Sh, ph = bh.run(q=q, pdf_type="rPDF", rho0=rho, sl=sl)
+    hd, Sd, pd = bd.run(q=q, pdf_type="rPDF", rho0=rho, sl=sl)
+    print(f"device plan f64={f64}: mismatched bins {int(np.count_nonzero(hh != hd))}", flush=True)
+    assert np.array_equal(hh, hd)
+    assert np.array_equal(ph, pd)
+    assert np.allclose(Sd, Sh, rtol=1e-12, atol=1e-12 * np.abs(Sh).max())
+    for s, (e, (h, _, _)) in enumerate(zip(ensemble, split(bd, ensemble, hd, None, None, 0))):
+        ho, _ = (O.hist_f64(e.xyz64(), e.el_count, BIN, int(bd.nhist[s])) if f64 else
+                 O.hist_f32(e.xyz, e.el_count, BIN, int(bd.nhist[s])))
+        assert np.array_equal(h, ho), s
+    bh.close()
+    bd.close()
+
+
+def test_device_plan_wide_runs_fall_back_to_kd_tiles(bctx):
+    """Structures whose equal-count tiles would be wider than the window
+    allows: two clusters 160 A apart along z with unequal counts (one
+    column; the run that holds the last atoms of one cluster and the first
+    of the other spans the gap), and a hollow shell.  Such a structure is
+    re-tiled with the host k-d split.  Counts equal the oracle, beside a
+    solid sphere tiled on the GPU."""
+    rng = np.random.default_rng(7)
+    a = st.fcc_sphere(radius=20.0, seed=3).xyz64()
+    b = st.fcc_sphere(radius=16.0, seed=4).xyz64()
+    two = np.concatenate([a + np.array([[0.0], [0.0], [-80.0]]), b + np.array([[0.0], [0.0], [80.0]])],
+                         axis=1)
+    u = rng.normal(size=(3, 30000))
+    shell = u / np.linalg.norm(u, axis=0) * 60.0 + rng.normal(0, 0.3, (3, 30000))
+    ens = [st.Ensemble(np.ascontiguousarray(two), [two.shape[1]], ["A"]),
+           st.Ensemble(np.ascontiguousarray(shell), [30000], ["A"]),
+           st.Ensemble(np.ascontiguousarray(a), [a.shape[1]], ["B"]),
+           st.Ensemble(np.ascontiguousarray(np.concatenate([a, shell], axis=1)),
+                       [a.shape[1], 30000], ["B", "A"])]
+    bd = device_plan(bctx, ens, f64=True)
+    assert int(bd.nhist[0]) > 8192 and int(bd.nhist[1]) > 8192
+    hd, _, _ = bd.run()
+    for s, (e, (h, _, _)) in enumerate(zip(ens, split(bd, ens, hd, None, None, 0))):
+        ho, _ = O.hist_f64(e.xyz64(), e.el_count, BIN, int(bd.nhist[s]))
+        print(f"wide-run case {s}: N={e.n} nhist={int(bd.nhist[s])} "
+              f"mismatches={int(np.count_nonzero(h != ho))}", flush=True)
+        assert np.array_equal(h, ho), s
+    bd.close()
+
+
 def test_batch_given_nhist_and_overflow(bctx, ensemble):
     import xansons_hip as xh
     sub = ensemble[:4]
@@ -148,7 +216,8 @@ def test_batch_graph_replays_match_eager(bctx, ensemble):
 
 def test_batch_full_ensemble_workload(bctx):
     """The ensemble workload at its full size: 1,000 particles of 500-200,000
-    atoms (1-4 elements), bin 0.01 A.  The batched histograms equal the
+    atoms (1-4 elements), bin 0.01 A, planned on the GPU.  The batched
+    histograms equal the
     per-structure xs_hist calls bit for bit for every particle (both FP64
     binning; the per-structure path is pinned to the oracle by the other
     tests), the 40 smallest particles and the smallest of over 20,000
@@ -156,7 +225,7 @@ def test_batch_full_ensemble_workload(bctx):
     import torch
     import xansons_hip as xh
     parts = st.particle_ensemble(1000)
-    b = xh.Batch(bctx, [(p.xyz64(), p.el_count) for p in parts], BIN, f64=True)
+    b = device_plan(bctx, parts, f64=True)
     dev = torch.device("cuda", 0)
     hist = torch.empty(b.hist_len, dtype=torch.int64, device=dev)
     hist2 = torch.empty_like(hist)
