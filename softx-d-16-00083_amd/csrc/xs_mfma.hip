@@ -128,22 +128,27 @@ __global__ void slice_counts(const unsigned long long *__restrict__ hist, unsign
   }
 }
 
-// The product, with the operand tiles of the next k tile fetched into LDS
-// while the current one is multiplied.  The same kernel with its operands
-// loaded straight into registers fetched 1.4 GB from HBM per launch of the
-// 768-histogram workload (1.2 TB/s), hit L2 for 87% of its requests at a
-// quarter of L2 bandwidth, and kept the MFMA pipe busy 35% of the time: it
-// waited on load latency, with two waves per SIMD and no registers to spare
-// for a prefetch.  The int32 accumulators go to S (FP64) every kFlush k
-// tiles; the wave owns its output elements, so the first flush stores and
-// later ones add.  Here every wave owns two LDS stages of NA + NB NT tiles (1 KiB
-// each, the pre-tiled layout, so a global_load_lds_dwordx4 of a wave writes
-// exactly one tile: lane l's 16 bytes at 16 l); the loads of k tile kt + 1
-// are issued into the other stage before the products of kt, which read
-// their operands from LDS (ds_read_b128 at 16 l, conflict-free).  No
-// registers beyond the plain kernel's, no barrier (each wave has its own
-// stages): the wave waits only for its own loads (vmcnt) and, before it
-// refills a stage, for its own reads of it (lgkmcnt).
+// The product.  A work group is 4 waves stacked along q (4 x 32 q) over NT
+// column tiles.  Per k tile it needs 4 NA A tiles (each wave its own) and
+// NB NT B tiles (the same for all 4 waves), 1 KiB each in the pre-tiled
+// layout, so a global_load_lds_dwordx4 of one wave writes exactly one tile
+// (lane l's 16 bytes at 16 l).  The tiles go through three LDS stages: those
+// of k tile kt + 2 are fetched while the products of kt run, and the
+// products read their operands from LDS (ds_read_b128 at 16 l, no bank
+// conflicts).  Each B tile is fetched once per work group, tile b by wave
+// b mod 4, which costs one barrier per k tile: before it, each wave waits
+// for its own loads of stage kt (vmcnt) and for its own reads of stage
+// kt - 1 (lgkmcnt); after it, stage kt is complete in LDS and stage kt - 1,
+// which is refilled next with k tile kt + 2, is read by no wave.
+// The int32 accumulators go to S (FP64) every kFlush k tiles; a wave owns
+// its output elements, so the first flush stores and later ones add.
+//
+// Measured on the 768-histogram workload (768 x 16,384 bins x 8,192 q):
+// operands loaded straight into registers kept the MFMA pipe busy 35% of
+// the time, waiting on load latency (two waves per SIMD at 250 VGPRs, no
+// registers for a prefetch); per-wave LDS stages one k tile ahead, with
+// every wave fetching its own copy of the B tiles (9 KiB per wave per k
+// tile), 50%.
 template <int NA, int NB, int NT>
 __global__ __launch_bounds__(256, 2) void pairsum_mfma_lds_kernel(const v4i *__restrict__ As,
                                                                 const v4i *__restrict__ Bs,
@@ -153,10 +158,16 @@ __global__ __launch_bounds__(256, 2) void pairsum_mfma_lds_kernel(const v4i *__r
   constexpr int DMIN = -(NA - 1);
   constexpr int DMAX = NB - 1;
   constexpr int NG = DMAX - DMIN + 1;
-  constexpr int TPS = NA + NB * NT;  // tiles per stage
-  // two stages per wave, four waves
-  __shared__ v4i stage[4][2][TPS][64];
-  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int NBT = NB * NT;         // B tiles per k tile, shared by the waves
+  constexpr int TPS = 4 * NA + NBT;    // tiles per stage
+  constexpr int NS = 3;                // stages
+  // loads per stage that every wave issues at least (waves b < NBT mod 4
+  // issue one more; waiting for this count still covers their older stage)
+  constexpr int VM = NA + NBT / 4;
+  static_assert(VM <= 15, "vmcnt field");
+  __shared__ v4i stage[NS][TPS][64];
+  const unsigned lane = threadIdx.x & 63;
+  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const unsigned ngrp = NTn / NT;
   unsigned tile = blockIdx.x;
   if ((gridDim.x & 7u) == 0) tile = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
@@ -167,18 +178,30 @@ __global__ __launch_bounds__(256, 2) void pairsum_mfma_lds_kernel(const v4i *__r
   const v4i *bp = Bs + (size_t)ng * NT * KT * 64 + lane;
   typedef __attribute__((address_space(3))) void lds_t;
 
-  // issue the TPS tile loads of k tile kt into stage st (one vmcnt each)
+  // this wave's loads of k tile kt into stage st: its NA A tiles, and the
+  // B tiles b = wave, wave + 4, ... (b = j NT + t: slice j, column tile t)
   auto fetch = [&](unsigned kt, int st) {
 #pragma unroll
     for (int i = 0; i < NA; ++i)
       __builtin_amdgcn_global_load_lds((const void *)(ap + i * sA + (size_t)kt * 64),
-                                       (lds_t *)&stage[wave][st][i][0], 16, 0, 0);
+                                       (lds_t *)&stage[st][wave * NA + i][0], 16, 0, 0);
 #pragma unroll
-    for (int j = 0; j < NB; ++j)
-#pragma unroll
-      for (int t = 0; t < NT; ++t)
+    for (int b0 = 0; b0 < NBT; b0 += 4) {
+      const int b = b0 + (int)wave;
+      if (b < NBT) {
+        const int j = b / NT, t = b % NT;
         __builtin_amdgcn_global_load_lds((const void *)(bp + j * sB + ((size_t)t * KT + kt) * 64),
-                                         (lds_t *)&stage[wave][st][NA + j * NT + t][0], 16, 0, 0);
+                                         (lds_t *)&stage[st][4 * NA + b][0], 16, 0, 0);
+      }
+    }
+  };
+  // all waves: own LDS reads done, then the barrier; the empty asm keeps the
+  // compiler from moving LDS accesses across it
+  auto sync_lds = [&]() {
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
   };
 
   for (unsigned kb = 0; kb < KT; kb += kFlush) {
@@ -188,28 +211,25 @@ __global__ __launch_bounds__(256, 2) void pairsum_mfma_lds_kernel(const v4i *__r
     for (int g = 0; g < NG; ++g)
 #pragma unroll
       for (int t = 0; t < NT; ++t) acc[g][t] = v16i{0};
-    // no read of either stage may be pending when the first fetch of the
-    // block overwrites it
-    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    // no wave still reads a stage of the previous block
+    sync_lds();
     fetch(kb, 0);
+    if (kb + 1 < ke) fetch(kb + 1, 1);
+    int st = 0;
     for (unsigned kt = kb; kt < ke; ++kt) {
-      const int st = (kt - kb) & 1;
-      if (kt + 1 < ke) {
-        // the other stage was last read for k tile kt - 1, whose products
-        // have consumed their operands; the wait makes that explicit
-        __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
-        fetch(kt + 1, st ^ 1);
-        __builtin_amdgcn_s_waitcnt(0x0f70 | TPS);  // vmcnt(TPS): stage st has landed
-      } else {
-        __builtin_amdgcn_s_waitcnt(0x0f70);        // vmcnt(0)
-      }
+      if (kt + 1 < ke)
+        __builtin_amdgcn_s_waitcnt(0x0f70 | VM);  // vmcnt(VM): own loads of stage kt landed
+      else
+        __builtin_amdgcn_s_waitcnt(0x0f70);       // vmcnt(0)
+      sync_lds();
+      if (kt + 2 < ke) fetch(kt + 2, st == 0 ? 2 : st - 1);
       v4i a[NA], b[NB][NT];
 #pragma unroll
-      for (int i = 0; i < NA; ++i) a[i] = stage[wave][st][i][lane];
+      for (int i = 0; i < NA; ++i) a[i] = stage[st][wave * NA + i][lane];
 #pragma unroll
       for (int j = 0; j < NB; ++j)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) b[j][t] = stage[wave][st][NA + j * NT + t][lane];
+        for (int t = 0; t < NT; ++t) b[j][t] = stage[st][4 * NA + j * NT + t][lane];
 #pragma unroll
       for (int i = 0; i < NA; ++i)
 #pragma unroll
@@ -218,6 +238,7 @@ __global__ __launch_bounds__(256, 2) void pairsum_mfma_lds_kernel(const v4i *__r
           for (int t = 0; t < NT; ++t)
             acc[j - i - DMIN][t] =
                 __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j][t], acc[j - i - DMIN][t], 0, 0, 0);
+      st = st == 2 ? 0 : st + 1;
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
