@@ -10,7 +10,8 @@
 // into LDS (r/2pi, 1/r and, on a uniform q mesh, sin/cos of the phase step
 // dq r), then each lane owns 32 consecutive q and each half-wave every other
 // pair of its wave's quarter: sin(q r) is seeded once per pair and lane from
-// the series in xs_sincos.h, advanced one q by the stored rotation and then
+// a 64-point table of the turn and a short series (sincos_2pi_tab), advanced
+// one q by the stored rotation and then
 // by the three-term recurrence s_{k+1} = 2 cos(dq r) s_k - s_{k-1} (one FMA;
 // a rounding error made at step j grows by |sin((k-j) dq r)/sin(dq r)| <=
 // k-j, so over 32 q it stays near 1e-14), so a pair-q term costs 2 FP64
@@ -19,7 +20,7 @@
 // denominator is applied to first order through the pair's weight
 // 1/r - (1e-8/q_c)/r^2, one weight for every 4 neighbouring q of a lane.
 // The half-waves' sums are added by a cross-lane shuffle, the four waves'
-// in LDS, and the result to S with FP64 atomics.
+// in LDS, and the result to S with FP64 atomics; a last pass divides S by q.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -58,6 +59,44 @@ struct MeshInfo {
   uint32_t uniform, pad;
 };
 
+// sin and cos of 2 pi r for any double r, from a table of the 64 points
+// k/64 of a turn (tab[k] = (sin, cos) of 2 pi k/64, within an ulp) and a
+// short series: t = r - rint(r) and u = t - n/64 with n = rint(64 t) are
+// exact (u is a multiple of ulp(t), |u| <= |t|), |u| <= 1/128, so
+// x = 2 pi u lies within 0.0491 rad, where sin through x^9 and cos through
+// x^8 leave truncation errors below 1e-19 relative; then
+// sin(a + x) = sin a cos x + cos a sin x and likewise for cos.  24 VALU
+// instructions and one LDS read against 37 for the quarter-turn series of
+// sincos_2pi, with the same few-ulp result.
+__device__ __forceinline__ void sincos_2pi_tab(double r, const double2 *tab, double *s, double *c) {
+  const double t = r - rint(r);
+  const double n = rint(64.0 * t);
+  const double x = kTwoPi * fma(n, -1.0 / 64.0, t);
+  const double x2 = x * x;
+  double ps = fma_k(x2, inv_factorial(9), -inv_factorial(7));
+  ps = fma_k(ps, x2, inv_factorial(5));
+  ps = fma_k(ps, x2, -inv_factorial(3));
+  const double sx = fma(x * x2, ps, x);
+  double pc = fma_k(x2, inv_factorial(8), -inv_factorial(6));
+  pc = fma_k(pc, x2, inv_factorial(4));
+  pc = fma_k(pc, x2, -inv_factorial(2));
+  const double cx = fma(x2, pc, 1.0);
+  const double2 a = tab[(int)n & 63];
+  *s = fma(a.x, cx, a.y * sx);
+  *c = fma(a.y, cx, -(a.x * sx));
+}
+
+// S[p][k] /= q[k] (0 where q[k] = 0): debye_kernel adds up
+// q sin(q r)/(q r + 1e-8) and leaves the division by q to this pass.
+__global__ void divide_by_q_kernel(double *__restrict__ S, const double *__restrict__ q, unsigned nq,
+                                   size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double qk = q[i % nq];
+    S[i] = qk == 0.0 ? 0.0 : S[i] / qk;
+  }
+}
+
 // mesh == nullptr: the host decided UNIFORM and passed dq.  Otherwise both
 // variants are launched and the one that does not match mesh->uniform
 // returns at once (the flag is the same for the whole grid).
@@ -76,6 +115,13 @@ __global__ __launch_bounds__(kBlock) void debye_kernel(const double *__restrict_
     dq = mesh->dq;
   }
   __shared__ double4 pr[kTT * kTT];
+  __shared__ double2 tab[64];
+  __shared__ double ecs[(kQPT / kSub) * 32];  // e / q_c per (group, lane & 31)
+  if (threadIdx.x < 64) {
+    double s, c;
+    sincos_2pi((double)threadIdx.x / 64.0, &s, &c);
+    tab[threadIdx.x] = make_double2(s, c);
+  }
   const Item it = items[blockIdx.x];
   const Tile A = tiles[it.ti], B = tiles[it.tj];
   const bool diag = it.ti == it.tj;
@@ -119,14 +165,16 @@ __global__ __launch_bounds__(kBlock) void debye_kernel(const double *__restrict_
 #pragma unroll
   for (int m = 0; m < kQPT; ++m) acc[m] = 0.0;
   const double q0 = iq0 < nq ? q[iq0] : 0.0;
-  double ec[kQPT / kSub];  // e / q_c per group of kSub q (uniform mesh)
   double qv[UNIFORM ? 1 : kQPT], eq[UNIFORM ? 1 : kQPT];  // q and e / q (any mesh)
   if (UNIFORM) {
-#pragma unroll
-    for (int g = 0; g < kQPT / kSub; ++g) {
-      const double qc = fma((double)(g * kSub) + 0.5 * (kSub - 1), dq, q0);
-      ec[g] = qc > 0.0 ? 1e-8 / qc : 0.0;
+    // ecs[g][l]: e / q_c of group g of lane l's q
+    for (unsigned m = threadIdx.x; m < (kQPT / kSub) * 32; m += kBlock) {
+      const unsigned g = m / 32, l = m % 32;
+      const unsigned i0 = blockIdx.y * kQBlock + l * kQPT;
+      const double qc = fma((double)(g * kSub) + 0.5 * (kSub - 1), dq, i0 < nq ? q[i0] : 0.0);
+      ecs[m] = qc > 0.0 ? 1e-8 / qc : 0.0;
     }
+    __syncthreads();
   } else {
 #pragma unroll
     for (int k = 0; k < kQPT; ++k) {
@@ -141,26 +189,31 @@ __global__ __launch_bounds__(kBlock) void debye_kernel(const double *__restrict_
     for (unsigned m = pstart + half; m < pend; m += 4) {
       const double4 va = pr[m], vb = pr[m + 2];
       const double ira = va.y, ira2 = va.y * va.y, irb = vb.y, irb2 = vb.y * vb.y;
-      double wa[kQPT / kSub], wb[kQPT / kSub];
+      // e / q_c of the lane's groups from LDS; each group's two weights are
+      // formed just before its kSub q, so that only two are live (126 VGPRs,
+      // 4 waves per SIMD, against 164 with all 16 weights in registers)
+      double ev[kQPT / kSub];
 #pragma unroll
-      for (int g = 0; g < kQPT / kSub; ++g) {
-        wa[g] = fma(-ec[g], ira2, ira);
-        wb[g] = fma(-ec[g], irb2, irb);
-      }
+      for (int g = 0; g < kQPT / kSub; ++g) ev[g] = ecs[g * 32 + (lane & 31)];
       // s_0 = sin(q0 r), s_1 by one rotation, then the three-term
       // recurrence s_{k+1} = 2 cos(dq r) s_k - s_{k-1} (one FMA a q)
       double sa0, ca0, sb0, cb0;
-      sincos_2pi(q0 * va.x, &sa0, &ca0);
-      sincos_2pi(q0 * vb.x, &sb0, &cb0);
+      sincos_2pi_tab(q0 * va.x, tab, &sa0, &ca0);
+      sincos_2pi_tab(q0 * vb.x, tab, &sb0, &cb0);
       double spa = sa0, sa = fma(sa0, va.w, ca0 * va.z);
       double spb = sb0, sb = fma(sb0, vb.w, cb0 * vb.z);
       const double c2a = 2.0 * va.w, c2b = 2.0 * vb.w;
-      acc[0] = fma(spa, wa[0], acc[0]);
-      acc[0] = fma(spb, wb[0], acc[0]);
+      double wa = fma(-ev[0], ira2, ira), wb = fma(-ev[0], irb2, irb);
+      acc[0] = fma(spa, wa, acc[0]);
+      acc[0] = fma(spb, wb, acc[0]);
 #pragma unroll
       for (int k = 1; k < kQPT; ++k) {
-        acc[k] = fma(sa, wa[k / kSub], acc[k]);
-        acc[k] = fma(sb, wb[k / kSub], acc[k]);
+        if (k % kSub == 0) {
+          wa = fma(-ev[k / kSub], ira2, ira);
+          wb = fma(-ev[k / kSub], irb2, irb);
+        }
+        acc[k] = fma(sa, wa, acc[k]);
+        acc[k] = fma(sb, wb, acc[k]);
         if (k + 1 < kQPT) {
           const double sna = fma(c2a, sa, -spa), snb = fma(c2b, sb, -spb);
           spa = sa;
@@ -179,21 +232,18 @@ __global__ __launch_bounds__(kBlock) void debye_kernel(const double *__restrict_
 #pragma unroll
       for (int k = 0; k < kQPT; ++k) {
         double s, c;
-        sincos_2pi(qv[k] * v.x, &s, &c);
+        sincos_2pi_tab(qv[k] * v.x, tab, &s, &c);
         acc[k] = fma(s, fma(-eq[k], ir2, ir), acc[k]);
       }
     }
   }
   // sum over the half-waves (lane l + 32 onto lane l, a cross-lane shuffle)
-  // and the waves: waves 1..3 park their 32 x 32 sums of
-  // sin(q r)/(q r + 1e-8) in LDS (3 x 32 x 32 doubles, 24 KiB)
+  // and the waves: waves 1..3 park their 32 x 32 sums of q sin(q r)/(q r +
+  // 1e-8) in LDS (3 x 32 x 32 doubles, 24 KiB); the division by q is left to
+  // divide_by_q_kernel, once per output element
   double res[kQPT];
 #pragma unroll
-  for (int k = 0; k < kQPT; ++k) {
-    const double qk = iq0 + k < nq ? q[iq0 + k] : 0.0;
-    res[k] = qk == 0.0 ? 0.0 : acc[k] / qk;
-    res[k] += __shfl_down(res[k], 32, 64);
-  }
+  for (int k = 0; k < kQPT; ++k) res[k] = acc[k] + __shfl_down(acc[k], 32, 64);
   __syncthreads();
   double *park = reinterpret_cast<double *>(pr);
   if (wave > 0 && half == 0) {
@@ -299,6 +349,8 @@ static int run(xs_ctx *ctx, const double *xyz_in, size_t n, const unsigned *el_c
     else
       hipLaunchKernelGGL(debye_kernel<false>, grid, dim3(kBlock), 0, ctx->stream, xyz, xyz + n,
                          xyz + 2 * n, td, idv, qd, nq, dq, nparts, Sd, nullptr);
+    hipLaunchKernelGGL(divide_by_q_kernel, dim3((unsigned)std::min<size_t>((ssize + 255) / 256, 4096)),
+                       dim3(256), 0, ctx->stream, Sd, qd, nq, ssize);
     XS_HIP_CHECK(hipGetLastError());
     XS_TRY(prof_end(ctx, "debye", ev));
   }
@@ -506,6 +558,8 @@ static int run_noalloc(xs_ctx *ctx, const double *xyz, size_t n, const unsigned 
                        idv, q, nq, 0.0, 1u, S, mesh);
     hipLaunchKernelGGL(debye_kernel<false>, grid, dim3(kBlock), 0, st, xyz, xyz + n, xyz + 2 * n, td,
                        idv, q, nq, 0.0, 1u, S, mesh);
+    hipLaunchKernelGGL(divide_by_q_kernel, dim3((unsigned)std::min<size_t>((ssize + 255) / 256, 4096)),
+                       dim3(256), 0, st, S, q, nq, ssize);
   }
   XS_HIP_CHECK(hipGetLastError());
   return XS_OK;
