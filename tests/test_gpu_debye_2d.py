@@ -71,6 +71,27 @@ def test_debye_nonuniform_q(ctx):
     assert pw <= TOL and l2 <= TOL
 
 
+@pytest.mark.parametrize("q", [np.linspace(0.0, 12.0, 2500), np.array([3.0]),
+                               np.linspace(0.5, 9.5, 33)], ids=["from0_2500", "one_q", "nq33"])
+def test_debye_edge_meshes(ctx, q):
+    """Uniform meshes that start at q = 0 (S(0) = 0 exactly, the first group's
+    weight taken at q_c = 1.5 dq), span several 1,024-q blocks with a ragged
+    last one, hold a single q (dq = 0), or end inside a lane's 32 q."""
+    ens = st.random_cluster([70, 50], box=12.0, min_dist=1.0, seed=21)
+    Sg = ctx.debye_pairsum(ens.xyz64(), ens.el_count, q)
+    So = O.debye_pairsum(ens.xyz64(), ens.el_count, q)
+    if q[0] == 0.0:
+        assert np.all(Sg.reshape(-1, q.size)[:, 0] == 0.0)
+    I = lambda S: O.intensity_1d(S, ens.el_count, q, sl=np.array([2.0, 3.0]))
+    Ig, Io = I(Sg), I(So)
+    if q.size > 1:
+        pw, l2 = rel_err(Ig, Io)
+        print(f"debye mesh {q[0]}-{q[-1]} x {q.size}: {pw:.3e} {l2:.3e}", flush=True)
+        assert pw <= TOL and l2 <= TOL
+    else:
+        assert abs(Ig[0] - Io[0]) <= TOL * abs(Io[0])
+
+
 def test_partial_debye_two_blocks(ctx):
     g = golden("partial_2blocks")
     x64 = g["xyz"].astype(np.float64)
