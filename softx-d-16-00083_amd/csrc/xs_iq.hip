@@ -7,10 +7,10 @@
 //   skipped uniformly.  The terms are FP64: at low q the intensity is a
 //   difference of the pair sum and the self term that cancels by up to four
 //   orders of magnitude, which FP32 terms (6e-8 relative each) cannot resolve
-//   to 1e-5.  FP64 vector instructions issue at the non-packed FP32 rate on
-//   gfx950, and the sine costs four FMAs per bin: sin/cos((k+1/2) q bin) is
-//   advanced from bin to bin by the rotation with angle q bin, seeded once per
-//   chunk from the series in xs_sincos.h (drift after 2048 steps ~1e-13).
+//   to 1e-5.  A bin costs four FP64 instructions per q: sin((k+1/2) q bin)
+//   is advanced from bin to bin by Reinsch's form of the three-term
+//   recurrence (an add and an FMA), seeded once per chunk from the series in
+//   xs_sincos.h, and the term's weight and sum take one FMA each.
 //   Chunk sums are scaled by 1/(q bin) and added into the result with FP64
 //   atomics.  The reference's denominator is q r + 1e-8 (CalcFunctions.cpp:861);
 //   1/(x + e) = (1/x)(1 - e/x) + O((e/x)^2) is applied per term, because at the
@@ -40,28 +40,53 @@ __global__ void fill_f64(double *__restrict__ p, size_t n) {
 }
 
 // Sum over one staged chunk of bins [k0, k0 + kn) for one q (qv != 0).
+// bins[b] = (h y, h y^2) with y = 1/(k0 + b + 1/2), so that the term
+// h y (1 - eps_q y) sin((k + 1/2) q bin) is one FMA for its weight and one
+// to add it.  The sine s_k = sin((k + 1/2) t), t = q bin, is advanced by
+// Reinsch's form of the three-term recurrence (Stoer and Bulirsch,
+// Introduction to Numerical Analysis, sec. 2.3.3): with the difference
+// d_k = s_{k+1} - s_k and lam = -4 sin^2(t/2) = 2 cos t - 2,
+//   s_{k+1} = s_k + d_k,   d_{k+1} = d_k + lam s_{k+1}
+// (one add, one FMA), seeded with d_{k0} = 2 sin(t/2) cos((k0 + 1) t).  The
+// plain recurrence s_{k+1} = 2 cos t s_k - s_{k-1} rounds 2 cos t, whose
+// error shifts the frequency by ~eps cot t: over a 2048-bin chunk its sines
+// drifted by up to 2e-11 at t = 1e-4..1e-3 (an FP64 simulation against
+// extended precision), against 4e-16..2e-15 for this form and 2e-14 for a
+// rotation of (sin, cos) by t (four FMAs).  Two accumulators (even and odd
+// bins) halve the dependent chain of the sums.
 __device__ __forceinline__ double pairsum_chunk(const double2 *bins, unsigned k0, unsigned kn,
                                                 double qv, double bin) {
   const double turns = qv * bin / (2.0 * kPi);  // phase step per bin, in turns
-  double s, c, ds, dc;
+  double s, c, h, g;
   sincos_2pi(((double)k0 + 0.5) * turns, &s, &c);
-  sincos_2pi(turns, &ds, &dc);
+  sincos_2pi(0.5 * turns, &h, &g);
+  const double lam = -4.0 * h * h;
+  double d = 2.0 * h * fma(c, g, -(s * h));  // 2 sin(t/2) cos((k0 + 1) t)
   const double eps_q = 1e-8 / (qv * bin);
-  double acc = 0.0;
-  for (unsigned b = 0; b < kn; ++b) {
-    const double2 e = bins[b];
-    if (e.x != 0.0) acc = fma(e.x * e.y * fma(-eps_q, e.y, 1.0), s, acc);
-    const double sn = fma(s, dc, c * ds);
-    c = fma(c, dc, -(s * ds));
-    s = sn;
+  double acc0 = 0.0, acc1 = 0.0;
+  unsigned b = 0;
+  for (; b + 2 <= kn; b += 2) {
+    const double2 e0 = bins[b], e1 = bins[b + 1];
+    acc0 = fma(fma(-eps_q, e0.y, e0.x), s, acc0);
+    s += d;
+    d = fma(lam, s, d);
+    acc1 = fma(fma(-eps_q, e1.y, e1.x), s, acc1);
+    s += d;
+    d = fma(lam, s, d);
   }
-  return acc;
+  if (b < kn) {
+    const double2 e0 = bins[b];
+    acc0 = fma(fma(-eps_q, e0.y, e0.x), s, acc0);
+  }
+  return acc0 + acc1;
 }
 
 __device__ __forceinline__ void stage_chunk(double2 *bins, const unsigned long long *__restrict__ h,
                                             unsigned k0, unsigned kn) {
-  for (unsigned b = threadIdx.x; b < kn; b += kQBlock)
-    bins[b] = make_double2((double)h[b], 1.0 / ((double)(k0 + b) + 0.5));
+  for (unsigned b = threadIdx.x; b < kn; b += kQBlock) {
+    const double y = 1.0 / ((double)(k0 + b) + 0.5), hy = (double)h[b] * y;
+    bins[b] = make_double2(hy, hy * y);
+  }
 }
 
 __global__ __launch_bounds__(kQBlock) void pairsum_kernel(const unsigned long long *__restrict__ hist,
