@@ -260,6 +260,54 @@ def ensemble_leg(ctx, count, steps, warmup, use_graph=True):
     split_ms = {k: round(bctx.profile_read(k)[0], 3) for k in ("hist", "batch_pairsum", "batch_pdf")}
     bctx.profile(False)
 
+    # the same plan with the pair sums on the int8 matrix cores
+    # (XS_PAIRSUM_MFMA): its first run (column groups planned and uploaded),
+    # then replays, then the kernel split of one eager run
+    Sm = torch.empty_like(S)
+    torch.cuda.synchronize()
+
+    def batched_mfma(graph):
+        b.run_dev(hist.data_ptr(), qd.data_ptr(), nq, Sm.data_ptr(), "rPDF", rho, sl,
+                  pdf.data_ptr(), graph=graph, mfma=True)
+
+    t0 = time.perf_counter()
+    batched_mfma(False)
+    bctx.synchronize()
+    t_first_m = time.perf_counter() - t0
+    for _ in range(warmup):
+        batched_mfma(use_graph)
+    bctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batched_mfma(use_graph)
+    bctx.synchronize()
+    t_bm = (time.perf_counter() - t0) / steps
+    bctx.profile(True)
+    batched_mfma(False)
+    bctx.synchronize()
+    pairsum_mfma_ms = round(bctx.profile_read("batch_pairsum")[0], 3)
+    bctx.profile(False)
+    torch.cuda.synchronize()
+    sm_rel = float(((Sm - S).abs().max() / S.abs().max()).item())
+    # I(q) of every particle from both pair sums (the reference's combination:
+    # sum_e N_e sl_e^2 + 2 sum_{a<=b} sl_a sl_b S_ab, over N), pointwise
+    # where I >= 1e-3 max I
+    S_h, Sm_h = S.cpu().numpy(), Sm.cpu().numpy()
+    im_rel, e0 = 0.0, 0
+    for s_i, p in enumerate(parts):
+        sle, ec = sl[e0:e0 + p.n_el], p.el_count
+        e0 += p.n_el
+        t0_ = int(b.type_first[s_i])
+        w = [sle[a_] * sle[b_] for a_ in range(p.n_el) for b_ in range(a_, p.n_el)]
+        self_ = sum(ec[a_] * sle[a_] ** 2 for a_ in range(p.n_el))
+
+        def inten(SS):
+            rows = SS[t0_ * nq:(t0_ + len(w)) * nq].reshape(len(w), nq)
+            return (self_ + 2.0 * (np.asarray(w)[:, None] * rows).sum(axis=0)) / p.n
+        Ie, Ig = inten(S_h), inten(Sm_h)
+        m = np.abs(Ie) >= 1e-3 * np.abs(Ie).max()
+        im_rel = max(im_rel, float(np.max(np.abs(Ig[m] - Ie[m]) / np.abs(Ie[m]))))
+
     # structure by structure, same kernels, outputs in the batch layout
     hist2 = torch.empty_like(hist)
     S2 = torch.empty_like(S)
@@ -344,6 +392,19 @@ def ensemble_leg(ctx, count, steps, warmup, use_graph=True):
         "plain_f32_note": "structure by structure with FP32 coordinates and the reference GPU binning "
                           "(IEEE float sqrt and division); the batched form bins in FP64 (the reference "
                           "CPU expression)",
+        "mfma_pairsums": {
+            "patterns_per_s_batched": round(count / (t_plan + t_first_m), 2),
+            "speedup": round(t_p / (t_plan + t_first_m), 3),
+            "patterns_per_s_batched_replay": round(count / t_bm, 2),
+            "speedup_replay": round(t_p / t_bm, 3),
+            "ms_first_run": round(t_first_m * 1e3, 2),
+            "ms_batched": round(t_bm * 1e3, 2),
+            "pairsum_ms": pairsum_mfma_ms,
+            "S_max_rel_diff_vs_fp64_pairsums": sm_rel,
+            "I_max_rel_diff_vs_fp64_pairsums": im_rel,
+            "note": "the same plan and runs with XS_PAIRSUM_MFMA: pair sums of all pair types as "
+                    "int8 matrix-core products (types grouped by count slices and bin count); "
+                    "histograms and PDFs unchanged"},
         "ms_batched": round(t_b * 1e3, 2),
         "ms_plain": round(t_p * 1e3, 2),
         "gpairs_per_s_batched_replay": round(pairs / t_b / 1e9, 2),

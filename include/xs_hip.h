@@ -262,6 +262,12 @@ int xs_batch_info(const xs_batch *plan, unsigned *nhist,
  * call is synchronous.  XS_BATCH_GRAPH: the launches are captured into a
  * hipGraph on the first call and replayed by later calls with the same
  * pointers, nq, pdf_type and flags (needs a non-default stream).
+ * XS_PAIRSUM_MFMA: the pair sums as int8 matrix-core products (as
+ * xs_hist_pairsum with that flag), with the pair types grouped by bin count
+ * and count slices taken from each type's pair count (no read-back, so the
+ * run stays capturable); XS_ERR_RANGE if a pair type has 2^35 pairs or more.
+ * Its slices take about 5*nq*max(nhist) + (count slices)*sum(nhist) bytes of
+ * the plan's run buffer.
  * Overlapping runs of one plan race; use one plan per stream. */
 int xs_batch_run(xs_batch *plan, uint64_t *hist, const double *q, unsigned nq,
                  double *S, int pdf_type, const double *rho0, const double *sl,

@@ -31,12 +31,13 @@ void release_batch_run(BatchRunState &r) {
 namespace {
 
 struct RunLayout {
-  size_t chunks, pdfdesc, q, hist, S, pdf, total;
+  size_t chunks, pdfdesc, q, hist, S, pdf, mm_cols, mm_tiles, mm_a, mm_b, total;
 };
 
-// Byte offsets inside the run buffer; host-mode staging only when needed.
+// Byte offsets inside the run buffer; host-mode staging only when needed;
+// the matrix-core tables and slices when mm (a plan with mm.nq != 0) is given.
 RunLayout layout(const xs_batch *b, size_t nchunks, unsigned nq, bool host, bool want_S,
-                 bool want_pdf) {
+                 bool want_pdf, const MfmaBatchPlan *mm) {
   RunLayout L{};
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -51,6 +52,10 @@ RunLayout layout(const xs_batch *b, size_t nchunks, unsigned nq, bool host, bool
   L.hist = take(host ? b->hist_len * 8 : 0);
   L.S = take(host && want_S ? b->ntypes_total * nq * 8 : 0);
   L.pdf = take(host && want_pdf ? b->pdf_len * 8 : 0);
+  L.mm_cols = take(mm ? mm->cols.size() * sizeof(MfmaCol) : 0);
+  L.mm_tiles = take(mm ? mm->tiles.size() * sizeof(MfmaTile) : 0);
+  L.mm_a = take(mm ? mm->a_bytes : 0);
+  L.mm_b = take(mm ? mm->b_bytes : 0);
   L.total = align_up(off, 256);
   return L;
 }
@@ -95,11 +100,29 @@ struct RunPtrs {
   double *pdf;
 };
 
+// An upper bound of every count of each pair type: its number of pairs.
+std::vector<unsigned long long> pair_counts(const xs_batch *b) {
+  std::vector<unsigned long long> c(b->ntypes_total);
+  for (unsigned s = 0; s < b->nstruct; ++s) {
+    const unsigned *ec = &b->el_count[b->el_first[s]];
+    size_t t = b->type_first[s];
+    for (unsigned i = 0; i < b->n_el[s]; ++i)
+      for (unsigned j = i; j < b->n_el[s]; ++j, ++t)
+        c[t] = i == j ? (unsigned long long)ec[i] * (ec[i] - (ec[i] > 0)) / 2
+                      : (unsigned long long)ec[i] * ec[j];
+  }
+  return c;
+}
+
 int launch_all(xs_batch *b, const RunPtrs &P, const RunLayout &L, size_t nchunks, unsigned nq,
-               int pdf_type, unsigned pdf_blocks, bool cpu_radius, bool prof) {
+               int pdf_type, unsigned pdf_blocks, bool cpu_radius, bool mfma, bool prof) {
   char *buf = static_cast<char *>(b->run.buf);
   XS_TRY(batch_hist_launch(b, P.hist, prof));
-  if (P.S)
+  if (P.S && mfma)
+    XS_TRY(pairsum_mfma_batch(b->ctx, b->run.mm, P.hist, reinterpret_cast<const MfmaCol *>(buf + L.mm_cols),
+                              reinterpret_cast<const MfmaTile *>(buf + L.mm_tiles), buf + L.mm_a,
+                              buf + L.mm_b, b->bin, P.q, nq, P.S, prof));
+  else if (P.S)
     XS_TRY(batch_pairsum_launch(b->ctx, P.hist, reinterpret_cast<const PairsumChunk *>(buf + L.chunks),
                                 nchunks, b->ntypes_total, b->bin, P.q, nq, P.S, prof));
   if (P.pdf)
@@ -136,11 +159,20 @@ extern "C" int xs_batch_run(xs_batch *plan, uint64_t *hist, const double *q, uns
     return fail(XS_ERR_ARG, "xs_batch_run: graph capture needs a non-default stream");
 
   BatchRunState &R = plan->run;
+  const bool mfma = want_S && (flags & XS_PAIRSUM_MFMA);
+  if (mfma && R.mm.nq != nq) {
+    // column order, groups and slice counts for this nq (the count bounds
+    // are the pair counts, so no read-back: the run stays capturable)
+    if (!mfma_batch_plan(plan->type_hist_off, plan->type_nhist, pair_counts(plan), nq, &R.mm))
+      return fail(XS_ERR_RANGE, "xs_batch_run: a pair type has 2^35 pairs or more, beyond the "
+                                "matrix-core path; run without XS_PAIRSUM_MFMA");
+    R.mm_uploaded = false;
+  }
   const std::vector<PairsumChunk> chunks = make_chunks(plan);
   unsigned pdf_blocks = 0;
   const std::vector<PdfDesc> desc = make_pdf_desc(plan, want_pdf ? rho0 : nullptr, want_pdf ? sl : nullptr,
                                                   &pdf_blocks);
-  const RunLayout L = layout(plan, chunks.size(), nq, !device, want_S, want_pdf);
+  const RunLayout L = layout(plan, chunks.size(), nq, !device, want_S, want_pdf, mfma ? &R.mm : nullptr);
 
   // (re)allocate the run buffer and upload the descriptors when they change
   bool fresh = false;
@@ -178,6 +210,15 @@ extern "C" int xs_batch_run(xs_batch *plan, uint64_t *hist, const double *q, uns
     R.key_rho0 = key_rho0;
     R.key_sl = key_sl;
   }
+  if (mfma && (fresh || !R.mm_uploaded || R.mm_at != L.mm_cols)) {
+    XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    XS_HIP_CHECK(hipMemcpy(buf + L.mm_cols, R.mm.cols.data(), R.mm.cols.size() * sizeof(MfmaCol),
+                           hipMemcpyHostToDevice));
+    XS_HIP_CHECK(hipMemcpy(buf + L.mm_tiles, R.mm.tiles.data(), R.mm.tiles.size() * sizeof(MfmaTile),
+                           hipMemcpyHostToDevice));
+    R.mm_uploaded = true;
+    R.mm_at = L.mm_cols;
+  }
 
   RunPtrs P{};
   if (device) {
@@ -196,9 +237,9 @@ extern "C" int xs_batch_run(xs_batch *plan, uint64_t *hist, const double *q, uns
   const bool cpu_radius = flags & XS_PDF_CPU_RADIUS;
 
   if (!graph) {
-    XS_TRY(launch_all(plan, P, L, chunks.size(), nq, pdf_type, pdf_blocks, cpu_radius, true));
+    XS_TRY(launch_all(plan, P, L, chunks.size(), nq, pdf_type, pdf_blocks, cpu_radius, mfma, true));
   } else {
-    const unsigned kflags = flags & (XS_DEVICE_PTRS | XS_PDF_CPU_RADIUS);
+    const unsigned kflags = flags & (XS_DEVICE_PTRS | XS_PDF_CPU_RADIUS | (mfma ? XS_PAIRSUM_MFMA : 0u));
     const bool same = R.exec && R.key_hist == P.hist && R.key_S == P.S && R.key_pdf == P.pdf &&
                       R.key_q == P.q && R.key_nq == (want_S ? nq : 0u) &&
                       R.key_pdf_type == (want_pdf ? pdf_type : -1) && R.key_flags == kflags;
@@ -209,7 +250,8 @@ extern "C" int xs_batch_run(xs_batch *plan, uint64_t *hist, const double *q, uns
       R.exec = nullptr;
       R.graph = nullptr;
       XS_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      const int rc = launch_all(plan, P, L, chunks.size(), nq, pdf_type, pdf_blocks, cpu_radius, false);
+      const int rc =
+          launch_all(plan, P, L, chunks.size(), nq, pdf_type, pdf_blocks, cpu_radius, mfma, false);
       hipGraph_t g = nullptr;
       const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
       if (rc != XS_OK || ec != hipSuccess) {

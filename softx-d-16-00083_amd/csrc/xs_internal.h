@@ -10,6 +10,8 @@
 #include <string>
 #include <vector>
 
+#include "xs_mfma_plan.h"
+
 #include "xs_hip.h"
 
 namespace xs {
@@ -53,6 +55,9 @@ struct BatchRunState {
   std::vector<double> key_rho0, key_sl;
   void *buf = nullptr;   // pair-sum and PDF descriptors, host-mode staging
   size_t buf_cap = 0;
+  MfmaBatchPlan mm;      // XS_PAIRSUM_MFMA: columns, groups, buffer sizes (mm.nq = 0: none)
+  bool mm_uploaded = false;
+  size_t mm_at = 0;      // offset in buf of the uploaded matrix-core tables
 };
 
 }  // namespace xs
@@ -236,6 +241,13 @@ int batch_pairsum_launch(xs_ctx *ctx, const unsigned long long *hist, const Pair
 // H3 as an int8 matrix-core product (device pointers, S[ncols * nq] overwritten).
 int pairsum_mfma(xs_ctx *ctx, const unsigned long long *hist, unsigned ncols, unsigned nhist,
                  double bin, const double *q, unsigned nq, double *S);
+
+// Launches only (no allocation, copy or wait; graph-capturable): slices of
+// the sinc matrix into a_buf, of the counts into b_buf, then the products;
+// S[type * nq + iq] is overwritten for every type.
+int pairsum_mfma_batch(xs_ctx *ctx, const MfmaBatchPlan &plan, const unsigned long long *hist,
+                       const MfmaCol *cols_d, const MfmaTile *tiles_d, void *a_buf, void *b_buf,
+                       double bin, const double *q, unsigned nq, double *S, bool prof);
 
 // PDFs of many structures; blk0 = first 256-bin block of the structure.
 struct PdfDesc {

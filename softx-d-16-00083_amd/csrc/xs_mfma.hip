@@ -40,8 +40,8 @@ namespace mm {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-constexpr int kNA = 5;             // sinc slices
-constexpr int kMaxNB = 5;          // count slices (counts < 2^35)
+constexpr int kNA = kMfmaSlicesA;        // sinc slices
+constexpr int kMaxNB = kMfmaMaxSlicesB;  // count slices (counts < 2^35)
 constexpr unsigned kFlush = 1024;  // k tiles (of 32 bins) per int32 accumulation
 constexpr double kPi = 3.14159265358979323846;
 
@@ -128,6 +128,42 @@ __global__ void slice_counts(const unsigned long long *__restrict__ hist, unsign
   }
 }
 
+// Counts of an ensemble's pair types: one thread per (group, column tile t
+// of the group, k tile below kt_max, lane); threads past their group's k
+// range return.  Group g's slices go to b_off in the layout [nb][NT][kt][64];
+// columns past ncols and bins past a column's nhist are zero.
+__global__ void slice_counts_batch(const unsigned long long *__restrict__ hist,
+                                   const MfmaCol *__restrict__ cols, unsigned ncols,
+                                   const MfmaTile *__restrict__ tiles, unsigned tile0,
+                                   unsigned ntiles, unsigned kt_max, int nb, int NT,
+                                   v4i *__restrict__ Bs) {
+  const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t per_t = (size_t)kt_max * 64, per_group = per_t * NT;
+  if (tid >= per_group * ntiles) return;
+  const unsigned g = (unsigned)(tid / per_group);
+  const size_t r = tid % per_group;
+  const unsigned t = (unsigned)(r / per_t);
+  const unsigned kt = (unsigned)((r % per_t) >> 6);
+  const unsigned lane = tid & 63;
+  const MfmaTile d = tiles[tile0 + g];
+  if (kt >= d.kt) return;
+  const unsigned cg = t * 32 + (lane & 31);  // column within the group
+  const unsigned k0 = kt * 32 + 16 * (lane >> 5);
+  unsigned long long cnt[16];
+  MfmaCol c{0ull, 0u, 0u};
+  if (cg < d.ncol && d.col0 + cg < ncols) c = cols[d.col0 + cg];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) cnt[j] = k0 + j < c.nhist ? hist[c.hist_off + k0 + j] : 0ull;
+  for (int s = 0; s < nb; ++s) {
+    int8_t b[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) b[j] = (int8_t)((cnt[j] >> (7 * s)) & 127ull);
+    v4i w;
+    __builtin_memcpy(&w, b, 16);
+    Bs[d.b_off + ((size_t)(s * NT + t) * d.kt + kt) * 64 + lane] = w;
+  }
+}
+
 // The product.  A work group is 4 waves stacked along q (4 x 32 q) over NT
 // column tiles.  Per k tile it needs 4 NA A tiles (each wave its own) and
 // NB NT B tiles (the same for all 4 waves), 1 KiB each in the pre-tiled
@@ -149,12 +185,21 @@ __global__ void slice_counts(const unsigned long long *__restrict__ hist, unsign
 // registers for a prefetch); per-wave LDS stages one k tile ahead, with
 // every wave fetching its own copy of the B tiles (9 KiB per wave per k
 // tile), 50%.
-template <int NA, int NB, int NT>
+//
+// BATCH: the column tiles come in groups of NT with their own k range and
+// slices (tiles[tile0 + group]: the count slices at b_off, laid out
+// [NB][NT][kt][64]), A is laid out with KT = the largest k range, and the
+// columns go to the rows of S that cols[] names.  Otherwise all columns
+// share KT and B is laid out [NB][NTn][KT][64].
+template <int NA, int NB, int NT, bool BATCH>
 __global__ __launch_bounds__(256, 2) void pairsum_mfma_lds_kernel(const v4i *__restrict__ As,
                                                                 const v4i *__restrict__ Bs,
                                                                 unsigned MT, unsigned NTn, unsigned KT,
                                                                 unsigned nq, unsigned ncols,
-                                                                double *__restrict__ S) {
+                                                                double *__restrict__ S,
+                                                                const MfmaTile *__restrict__ tiles,
+                                                                const MfmaCol *__restrict__ cols,
+                                                                unsigned tile0) {
   constexpr int DMIN = -(NA - 1);
   constexpr int DMAX = NB - 1;
   constexpr int NG = DMAX - DMIN + 1;
@@ -173,9 +218,22 @@ __global__ __launch_bounds__(256, 2) void pairsum_mfma_lds_kernel(const v4i *__r
   if ((gridDim.x & 7u) == 0) tile = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
   const unsigned mg = tile / ngrp, ng = tile % ngrp;
   const unsigned mt = mg * 4 + wave;
-  const size_t sA = (size_t)MT * KT * 64, sB = (size_t)NTn * KT * 64;
+  const size_t sA = (size_t)MT * KT * 64;
   const v4i *ap = As + (size_t)mt * KT * 64 + lane;
+  // this group's k tiles, B slice stride, B column-tile stride, B base
+  unsigned KTg = KT;
+  size_t sB = (size_t)NTn * KT * 64, tB = (size_t)KT * 64;
   const v4i *bp = Bs + (size_t)ng * NT * KT * 64 + lane;
+  unsigned col0 = ng * NT * 32, colend = ncols;
+  if constexpr (BATCH) {
+    const MfmaTile d = tiles[tile0 + ng];
+    KTg = d.kt;
+    sB = (size_t)NT * d.kt * 64;
+    tB = (size_t)d.kt * 64;
+    bp = Bs + d.b_off + lane;
+    col0 = d.col0;
+    colend = min(ncols, d.col0 + d.ncol);
+  }
   typedef __attribute__((address_space(3))) void lds_t;
 
   // this wave's loads of k tile kt into stage st: its NA A tiles, and the
@@ -190,7 +248,7 @@ __global__ __launch_bounds__(256, 2) void pairsum_mfma_lds_kernel(const v4i *__r
       const int b = b0 + (int)wave;
       if (b < NBT) {
         const int j = b / NT, t = b % NT;
-        __builtin_amdgcn_global_load_lds((const void *)(bp + j * sB + ((size_t)t * KT + kt) * 64),
+        __builtin_amdgcn_global_load_lds((const void *)(bp + j * sB + t * tB + (size_t)kt * 64),
                                          (lds_t *)&stage[st][4 * NA + b][0], 16, 0, 0);
       }
     }
@@ -204,8 +262,8 @@ __global__ __launch_bounds__(256, 2) void pairsum_mfma_lds_kernel(const v4i *__r
     __asm__ volatile("" ::: "memory");
   };
 
-  for (unsigned kb = 0; kb < KT; kb += kFlush) {
-    const unsigned ke = min(KT, kb + kFlush);
+  for (unsigned kb = 0; kb < KTg; kb += kFlush) {
+    const unsigned ke = min(KTg, kb + kFlush);
     v16i acc[NG][NT];
 #pragma unroll
     for (int g = 0; g < NG; ++g)
@@ -242,8 +300,10 @@ __global__ __launch_bounds__(256, 2) void pairsum_mfma_lds_kernel(const v4i *__r
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      const unsigned col = (ng * NT + t) * 32 + (lane & 31);
-      if (col >= ncols) continue;
+      const unsigned col = col0 + t * 32 + (lane & 31);
+      if (col >= colend) continue;
+      size_t srow = col;
+      if constexpr (BATCH) srow = cols[col].type;
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
         const unsigned row = mt * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
@@ -251,41 +311,41 @@ __global__ __launch_bounds__(256, 2) void pairsum_mfma_lds_kernel(const v4i *__r
         double v = 0.0;
 #pragma unroll
         for (int g = 0; g < NG; ++g) v = fma((double)acc[g][t][e], ldexp(1.0, 7 * (g + DMIN) - 6), v);
-        double *o = S + (size_t)col * nq + row;
+        double *o = S + srow * nq + row;
         *o = kb == 0 ? v : *o + v;
       }
     }
   }
 }
 
-template <int NT>
+template <int NT, bool BATCH>
 static void launch_gemm(int nb, dim3 grid, hipStream_t st, const v4i *As, const v4i *Bs, unsigned MT,
-                        unsigned NTn, unsigned KT, unsigned nq, unsigned ncols, double *S) {
+                        unsigned NTn, unsigned KT, unsigned nq, unsigned ncols, double *S,
+                        const MfmaTile *tiles = nullptr, const MfmaCol *cols = nullptr,
+                        unsigned tile0 = 0) {
+#define XS_MM_LAUNCH(NBV, NTV)                                                                       \
+  hipLaunchKernelGGL((pairsum_mfma_lds_kernel<kNA, NBV, NTV, BATCH>), grid, dim3(256), 0, st, As, Bs, \
+                     MT, NTn, KT, nq, ncols, S, tiles, cols, tile0)
   switch (nb) {
     case 1:
-      hipLaunchKernelGGL((pairsum_mfma_lds_kernel<kNA, 1, NT>), grid, dim3(256), 0, st, As, Bs, MT, NTn, KT,
-                         nq, ncols, S);
+      XS_MM_LAUNCH(1, NT);
       break;
     case 2:
-      hipLaunchKernelGGL((pairsum_mfma_lds_kernel<kNA, 2, NT>), grid, dim3(256), 0, st, As, Bs, MT, NTn, KT,
-                         nq, ncols, S);
+      XS_MM_LAUNCH(2, NT);
       break;
     case 3:
-      if constexpr (NT == 1)  // three count slices with two column tiles would spill
-        hipLaunchKernelGGL((pairsum_mfma_lds_kernel<kNA, 3, 1>), grid, dim3(256), 0, st, As, Bs, MT, NTn,
-                           KT, nq, ncols, S);
+      if constexpr (NT == 1) XS_MM_LAUNCH(3, 1);  // three count slices with two column tiles would spill
       break;
     default:
       if constexpr (NT == 1) {
         if (nb == 4)
-          hipLaunchKernelGGL((pairsum_mfma_lds_kernel<kNA, 4, 1>), grid, dim3(256), 0, st, As, Bs, MT, NTn,
-                             KT, nq, ncols, S);
+          XS_MM_LAUNCH(4, 1);
         else
-          hipLaunchKernelGGL((pairsum_mfma_lds_kernel<kNA, 5, 1>), grid, dim3(256), 0, st, As, Bs, MT, NTn,
-                             KT, nq, ncols, S);
+          XS_MM_LAUNCH(5, 1);
       }
       break;
   }
+#undef XS_MM_LAUNCH
 }
 
 }  // namespace mm
@@ -332,11 +392,43 @@ int pairsum_mfma(xs_ctx *ctx, const unsigned long long *hist, unsigned ncols, un
   XS_TRY(prof_begin(ctx, "pairsum_mfma", &ev));
   const dim3 grid((MT / 4) * (NTn / NT));
   if (NT == 2)
-    launch_gemm<2>(nb, grid, ctx->stream, As, Bs, MT, NTn, KT, nq, ncols, S);
+    launch_gemm<2, false>(nb, grid, ctx->stream, As, Bs, MT, NTn, KT, nq, ncols, S);
   else
-    launch_gemm<1>(nb, grid, ctx->stream, As, Bs, MT, NTn, KT, nq, ncols, S);
+    launch_gemm<1, false>(nb, grid, ctx->stream, As, Bs, MT, NTn, KT, nq, ncols, S);
   XS_HIP_CHECK(hipGetLastError());
   XS_TRY(prof_end(ctx, "pairsum_mfma", ev));
+  return XS_OK;
+}
+
+int pairsum_mfma_batch(xs_ctx *ctx, const MfmaBatchPlan &plan, const unsigned long long *hist,
+                       const MfmaCol *cols_d, const MfmaTile *tiles_d, void *a_buf, void *b_buf,
+                       double bin, const double *q, unsigned nq, double *S, bool prof) {
+  using namespace mm;
+  if (nq != plan.nq) return fail(XS_ERR_ARG, "pairsum_mfma_batch: plan made for another nq");
+  const unsigned MT = (nq + 127) / 128 * 4;
+  const unsigned KT = plan.kt_max;
+  const unsigned ncols = (unsigned)plan.cols.size();
+  v4i *As = static_cast<v4i *>(a_buf), *Bs = static_cast<v4i *>(b_buf);
+  hipEvent_t ev{};
+  if (prof) XS_TRY(prof_begin(ctx, "batch_pairsum", &ev));
+  const size_t ta = (size_t)MT * KT * 64;
+  hipLaunchKernelGGL(slice_sinc, dim3((unsigned)((ta + 255) / 256)), dim3(256), 0, ctx->stream, q, nq,
+                     KT * 32, bin, MT, KT, As);
+  for (const MfmaLaunch &L : plan.launches) {
+    const size_t tb = (size_t)L.ntiles * L.nt * KT * 64;
+    hipLaunchKernelGGL(slice_counts_batch, dim3((unsigned)((tb + 255) / 256)), dim3(256), 0,
+                       ctx->stream, hist, cols_d, ncols, tiles_d, L.tile0, L.ntiles, KT, (int)L.nb,
+                       (int)L.nt, Bs);
+    const dim3 grid((MT / 4) * L.ntiles);
+    if (L.nt == 2)
+      launch_gemm<2, true>((int)L.nb, grid, ctx->stream, As, Bs, MT, L.ntiles * 2, KT, nq, ncols, S,
+                           tiles_d, cols_d, L.tile0);
+    else
+      launch_gemm<1, true>((int)L.nb, grid, ctx->stream, As, Bs, MT, L.ntiles, KT, nq, ncols, S,
+                           tiles_d, cols_d, L.tile0);
+  }
+  XS_HIP_CHECK(hipGetLastError());
+  if (prof) XS_TRY(prof_end(ctx, "batch_pairsum", ev));
   return XS_OK;
 }
 

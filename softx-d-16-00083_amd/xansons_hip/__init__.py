@@ -363,8 +363,10 @@ class Batch:
         except Exception:
             pass
 
-    def run(self, q=None, pdf_type=None, rho0=None, sl=None, cpu_radius=True, graph=False):
-        """Host arrays in and out: (hist, S or None, pdf or None) as flat arrays."""
+    def run(self, q=None, pdf_type=None, rho0=None, sl=None, cpu_radius=True, graph=False,
+            mfma=False):
+        """Host arrays in and out: (hist, S or None, pdf or None) as flat arrays.
+        mfma=True: the pair sums on the int8 matrix cores."""
         hist = np.zeros(self.hist_len, dtype=np.uint64)
         S = pdf = None
         qa = None
@@ -378,18 +380,19 @@ class Batch:
             pdf = np.zeros(self.pdf_len)
             sla = np.ascontiguousarray(sl, dtype=np.float64)
             rho = None if rho0 is None else np.ascontiguousarray(rho0, dtype=np.float64)
-        flags = (XS_PDF_CPU_RADIUS if cpu_radius else 0) | (XS_BATCH_GRAPH if graph else 0)
+        flags = (XS_PDF_CPU_RADIUS if cpu_radius else 0) | (XS_BATCH_GRAPH if graph else 0) | \
+            (XS_PAIRSUM_MFMA if mfma else 0)
         _check(lib().xs_batch_run(self.h, _p(hist), _p(qa), 0 if qa is None else qa.size, _p(S),
                                   int(t), _p(rho), _p(sla), _p(pdf), flags))
         return hist, S, pdf
 
     def run_dev(self, hist_ptr, q_ptr=None, nq=0, S_ptr=None, pdf_type=None, rho0=None, sl=None,
-                pdf_ptr=None, cpu_radius=True, graph=False):
+                pdf_ptr=None, cpu_radius=True, graph=False, mfma=False):
         t = -1 if pdf_type is None else PDF_TYPES.get(pdf_type, pdf_type)
         sla = None if sl is None else np.ascontiguousarray(sl, dtype=np.float64)
         rho = None if rho0 is None else np.ascontiguousarray(rho0, dtype=np.float64)
         flags = XS_DEVICE_PTRS | (XS_PDF_CPU_RADIUS if cpu_radius else 0) | \
-            (XS_BATCH_GRAPH if graph else 0)
+            (XS_BATCH_GRAPH if graph else 0) | (XS_PAIRSUM_MFMA if mfma else 0)
         vp = lambda v: C.c_void_p(v) if v else None  # noqa: E731
         _check(lib().xs_batch_run(self.h, vp(hist_ptr), vp(q_ptr), int(nq), vp(S_ptr), int(t),
                                   _p(rho), _p(sla), vp(pdf_ptr), flags))

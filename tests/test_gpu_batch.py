@@ -260,3 +260,61 @@ def test_batch_full_ensemble_workload(bctx):
             o, _ = O.hist_f64(p.xyz64(), ec, BIN, nh)
             assert np.array_equal(h, o), s
     b.close()
+
+
+def test_batch_mfma_pairsums(bctx, ensemble):
+    """XS_PAIRSUM_MFMA in a batch run: the pair types grouped by count slices
+    (1-4 here, from pair counts of 1 to 1.8e7) and bin count, on the int8
+    matrix cores.  S within 1e-9 of max|S| per structure and I(q) within 1e-5
+    of the FP64 oracle; histograms and PDFs as without the flag; runs with
+    and without the flag alternate on one plan; graph replays on the device
+    reproduce the host-mode run bit for bit."""
+    import torch
+    import xansons_hip as xh
+    q = st.q_mesh(0.5, 20.0, 600)
+    sl, rho = sl_for(ensemble), rho_for(ensemble)
+    b = xh.Batch(bctx, [(e.xyz64(), e.el_count) for e in ensemble], BIN, f64=True)
+    h_v, S_v, p_v = b.run(q=q, pdf_type="rPDF", rho0=rho, sl=sl)
+    h_m, S_m, p_m = b.run(q=q, pdf_type="rPDF", rho0=rho, sl=sl, mfma=True)
+    assert np.array_equal(h_m, h_v) and np.array_equal(p_m, p_v)
+    so = 0
+    for s, (e, (h, Ss, _)) in enumerate(zip(ensemble, split(b, ensemble, h_m, S_m, p_m, q.size))):
+        sle = sl[so:so + e.n_el]
+        so += e.n_el
+        So = O.hist_pairsum(h, BIN, q)
+        srel = float(np.max(np.abs(Ss - So)) / max(np.abs(So).max(), 1e-300))
+        Io = O.intensity_1d(So, e.el_count, q, sl=sle)
+        pw, l2 = rel_err(O.intensity_1d(Ss, e.el_count, q, sl=sle), Io)
+        print(f"struct {s}: N={e.n} S {srel:.2e} I {pw:.2e} {l2:.2e}", flush=True)
+        assert srel <= 1e-9 and pw <= TOL and l2 <= TOL, s
+    # without the flag again (its chunk sums meet in FP64 atomics, so to
+    # rounding), then with it: the same results
+    _, S_v2, _ = b.run(q=q)
+    assert np.allclose(S_v2, S_v, rtol=1e-12, atol=1e-12 * np.abs(S_v).max())
+    dev = torch.device("cuda", 0)
+    qd = torch.from_numpy(q).to(dev)
+    hist = torch.empty(b.hist_len, dtype=torch.int64, device=dev)
+    S = torch.empty(b.ntypes_total * q.size, dtype=torch.float64, device=dev)
+    for rep in range(3):
+        S.fill_(float("nan"))
+        torch.cuda.synchronize()
+        b.run_dev(hist.data_ptr(), qd.data_ptr(), q.size, S.data_ptr(), graph=True, mfma=True)
+        bctx.synchronize()
+        assert np.array_equal(S.cpu().numpy(), S_m), rep
+        print(f"mfma replay {rep}: ok", flush=True)
+    b.close()
+
+
+def test_batch_mfma_refuses_2p35_pairs(bctx):
+    """A pair type of 2^35 pairs or more needs more count slices than the
+    kernels take: the run is refused before any launch (checked on the
+    plan's pair counts, the structure is never binned)."""
+    import xansons_hip as xh
+    # 262,145 atoms of one element: 3.4e10 pairs >= 2^35
+    n = 262145
+    x = np.zeros((3, n))
+    x[0] = np.arange(n) * 1e-3
+    b = xh.Batch(bctx, [(x, [n])], 1.0, f64=True)
+    with pytest.raises(RuntimeError, match="2\\^35"):
+        b.run(q=st.q_mesh(0.5, 5.0, 8), mfma=True)
+    b.close()

@@ -1,4 +1,5 @@
-"""Host planning of the histogram kernels (no GPU): the work items keep every
+"""Host planning of the histogram kernels and of the batched matrix-core pair
+sums (no GPU).  Histograms: the work items keep every
 item below 2^32 pairs, the uint32 LDS slots' limit, and visit every tile
 pair exactly once.  The checks are C++ (tests/cpp/hist_plan_check.cpp)
 compiled with g++ against the library's own planning header."""
@@ -23,3 +24,18 @@ def test_item_tables_bound_and_coverage(tmp_path):
     assert r.stdout.strip().endswith("ok")
     # the longest paired runs are capped at 2047 tiles: 2 * 1024 * 2047 * 1024 pairs
     assert "max_pairs=4292870144" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ not available")
+def test_mfma_batch_plan_tables(tmp_path):
+    """The batched matrix-core pair sums' column groups (csrc/xs_mfma_plan.h):
+    every pair type is one column in exactly one group of one slice count
+    that covers its pair count, every group's k range covers its columns'
+    bins, the count slices do not overlap, and bounds of 2^35 are refused."""
+    exe = tmp_path / "mfma_plan_check"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-I", CSRC,
+                    os.path.join(HERE, "cpp", "mfma_plan_check.cpp"), "-o", str(exe)],
+                   check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("ok")
