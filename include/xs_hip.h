@@ -266,6 +266,8 @@ int xs_batch_info(const xs_batch *plan, unsigned *nhist,
  * xs_hist_pairsum with that flag), with the pair types grouped by bin count
  * and count slices taken from each type's pair count (no read-back, so the
  * run stays capturable); XS_ERR_RANGE if a pair type has 2^35 pairs or more.
+ * Its error is absolute, at most 2^-35 x (sum of the type's counts) per S
+ * entry, so relative to I(q) it grows with the structure's atom count.
  * Its slices take about 5*nq*max(nhist) + (count slices)*sum(nhist) bytes of
  * the plan's run buffer.
  * Overlapping runs of one plan race; use one plan per stream. */
