@@ -28,6 +28,13 @@
 // Work groups: 4 waves stacked along q (128 q) x NT tiles of 32 columns;
 // tiles are ordered so that the work groups that share a q block run on the
 // same XCD (consecutive work-group ids go to different XCDs).
+//
+// Ensembles (pairsum_mfma_batch): the columns are the pair types of many
+// structures, each with its own bin count.  They come sorted and grouped by
+// xs_mfma_plan.h; each group of NT column tiles has its own k range and
+// count slices, one launch covers the groups of one slice count, and the
+// columns' results go to their pair types' rows of S.  The slice count of a
+// type comes from its pair count, so nothing is read back.
 #include <algorithm>
 #include <cmath>
 
