@@ -739,12 +739,24 @@ __device__ __forceinline__ void pair_group4x2(const IAtom &ia, const IAtom &ib, 
                                               unsigned long long *__restrict__ hp,
                                               unsigned long long *__restrict__ ovf) {
 #pragma clang fp contract(off)
+  // the next group's J records are read before this group's increments, so
+  // that their LDS reads do not queue behind the atomics
+  double qx[4], qy[4], qz[4], qn[4];
+  if (ng) {
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      qx[p] = tx[p];
+      qy[p] = ty[p];
+      qz[p] = tz[p];
+      qn[p] = tn[p];
+    }
+  }
   for (uint32_t j0 = 0; j0 < ng; j0 += 4) {
     uint32_t k[8];
     bool bad[8], any = false;
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
-      const double rx = tx[j0 + p], ry = ty[j0 + p], rz = tz[j0 + p], rn = tn[j0 + p];
+      const double rx = qx[p], ry = qy[p], rz = qz[p], rn = qn[p];
       double s = ia.n + rn;
       s = __builtin_fma(ia.az, rz, s);
       s = __builtin_fma(ia.ay, ry, s);
@@ -769,6 +781,15 @@ __device__ __forceinline__ void pair_group4x2(const IAtom &ia, const IAtom &ib, 
           s = s + dz * dz;
           k[p] = exact_bin<double>(s, bin, nhist);
         }
+      }
+    }
+    if (j0 + 4 < ng) {
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        qx[p] = tx[j0 + 4 + p];
+        qy[p] = ty[j0 + 4 + p];
+        qz[p] = tz[j0 + 4 + p];
+        qn[p] = tn[j0 + 4 + p];
       }
     }
     if constexpr (SAFE) {
