@@ -188,10 +188,11 @@ def ensemble_leg(ctx, count, steps, warmup, use_graph=True):
     """Batched (one plan, graph replays) vs structure-by-structure runs of the
     same kernels on `count` nanoparticles: histograms (FP64 binning), pair
     sums on 2,048 q and rPDF per particle.  Both write the same layout, which
-    is compared after the timed runs.  A fresh ensemble costs its plan (host
-    tiling and upload) plus one run; replays of the same coordinates cost the
-    run alone; both are reported.  use_graph=False replays eagerly (for
-    kernel traces)."""
+    is compared after the timed runs.  A fresh ensemble costs its plan (tiled
+    on the GPU from the device coordinates) plus one run; replays of the same
+    coordinates cost the run alone; both are reported, and both again with
+    the pair sums on the int8 matrix cores (XS_PAIRSUM_MFMA).
+    use_graph=False replays eagerly (for kernel traces)."""
     import torch
     import xansons_hip as xh
     from xansons_hip import structures as st2
