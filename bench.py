@@ -43,13 +43,13 @@ FP64_PEAK_TFLOPS = 78.6           # MI355X published FP64 vector peak (binning i
 VALU_ISSUE_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12   # non-packed lane-ops/s (39.3 T)
 OPS_PER_PAIR = 11                 # 3 sub, 3 mul, 2 add, sqrt, div, cvt
 # VALU issue cycles per 64 pairs of the histogram kernel, from rocprofv3 SQ
-# counters on this workload: 14.61 VALU wave-instructions per 64 pairs, 1.03
+# counters on this workload: 11.89 VALU wave-instructions per 64 pairs, 1.04
 # of them v_rsq_f64, which occupies the issue port four times as long as
-# v_fma_f64 (3 extra cycles); SQ_ACTIVE_INST_VALU reads the same 17.69.  A CU
+# v_fma_f64 (3 extra cycles); SQ_ACTIVE_INST_VALU reads 15.01.  A CU
 # issues one 64-lane VALU instruction per cycle (4 SIMDs x 16 lanes), so the
 # fraction of the CUs' issue cycles at the 2.4 GHz peak clock the kernel
 # spends issuing VALU work is SLOTS_PER_PAIR * pairs / 64 / (256 * 2.4e9 * t).
-SLOTS_PER_PAIR = 17.69
+SLOTS_PER_PAIR = 15.01
 CU_ISSUE_PER_S = 256 * 2.4e9
 # FP32 binning kernel (hist_kernel_f32<32768, true>) on the same workload:
 # SQ_ACTIVE_INST_VALU reads 23.86 wave-instructions per 64 pairs.  Its

@@ -50,6 +50,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include "xs_fixbin.h"
 #include "xs_hist_plan.h"
 #include "xs_internal.h"
 
@@ -77,6 +78,7 @@ struct Args {
   unsigned long long *hist;
   unsigned long long *overflow;
   uint32_t quot_f32;  // FP32 binning: the bin passed quot_f32_check_kernel
+  uint32_t narrow;    // FP64 binning: every pair type has at most kNarrowMax bins
 };
 
 // ---------------------------------------------------------------- prep kernels
@@ -484,10 +486,25 @@ __device__ __forceinline__ uint32_t fast_bin(double s, double inv_bin, double ba
   return (uint32_t)kd;
 }
 
+// The same fast path in fixed point (xs_fixbin.h) for grids of at most
+// kNarrowMax bins: T's high word kFixHi + bin, its low word L; the estimate
+// is floored directly when L > FixBand::lm1.
+__device__ __forceinline__ uint32_t fix_bin(double s, double inv_bin, double m, uint32_t &lo) {
+  return fix_from_rsq(s, __builtin_amdgcn_rsq(s), inv_bin, m, lo);
+}
+
 template <class T>
 __device__ __forceinline__ uint32_t exact_bin(T s, T bin, uint32_t nhist) {
   const T t = (T)__builtin_sqrt((double)s) / bin;
   return t < (T)nhist ? (uint32_t)t : nhist;
+}
+
+// An atom index the compiler cannot see through, so that coordinates read
+// again at a later point are not merged with an earlier read (which would
+// keep them in registers in between).
+__device__ __forceinline__ size_t opaque_index(size_t g) {
+  asm volatile("" : "+v"(g));
+  return g;
 }
 
 // FP64 fast path of one I atom: its coordinates relative to a centre c of
@@ -726,19 +743,31 @@ __global__ __launch_bounds__(kTile) void hist_kernel_f32(Args<float> A) {
 
 // FP64 groups of 4 J atoms against the thread's two I atoms a and b (8
 // independent chains; each J record read from LDS serves two pairs).  Counts
-// of an I atom that does not exist (va / vb false) are dropped.
-template <int WCAP, bool SAFE>
+// of an I atom that does not exist (va / vb false) are dropped.  The exact
+// evaluation reads the I coordinates again (xg, yg, zg at ga, gb).
+// NARROW: the fixed-point fast path (fix_bin; nhist <= kNarrowMax), k[] then
+// holding kFixHi + bin.
+template <int WCAP, bool SAFE, bool NARROW>
 __device__ __forceinline__ void pair_group4x2(const IAtom &ia, const IAtom &ib, bool va, bool vb,
-                                              double xa, double ya, double za, double xb,
-                                              double yb, double zb, const double *sx,
-                                              const double *sy, const double *sz, const double *tx,
+                                              const double *xg, const double *yg,
+                                              const double *zg, size_t ga, size_t gb,
+                                              const double *sx, const double *sy,
+                                              const double *sz, const double *tx,
                                               const double *ty, const double *tz, const double *tn,
                                               uint32_t ng, double bin, double inv_bin, double band,
-                                              uint32_t nhist, uint32_t B, uint32_t wlim, uint32_t D,
-                                              uint32_t off, uint32_t *win,
-                                              unsigned long long *__restrict__ hp,
+                                              const FixBand &fb, uint32_t nhist, uint32_t B,
+                                              uint32_t wlim, uint32_t D, uint32_t off,
+                                              uint32_t *win, unsigned long long *__restrict__ hp,
                                               unsigned long long *__restrict__ ovf) {
 #pragma clang fp contract(off)
+  constexpr uint32_t kb = NARROW ? kFixHi : 0u;  // offset of k[] from the bin
+  // (opaque, so that the slot address stays one shift-add of k[])
+  uint32_t offk = off - kb;
+  asm volatile("" : "+v"(offk));
+  // SAFE: every lane increments for every pair, by 1 or, for an I atom that
+  // does not exist (placed at the I box's centre, so its bins stay inside
+  // the J tile's range), by 0
+  const uint32_t inca = va, incb = vb;
   // the next group's J records are read before this group's increments, so
   // that their LDS reads do not queue behind the atomics
   double qx[4], qy[4], qz[4], qn[4];
@@ -750,10 +779,21 @@ __device__ __forceinline__ void pair_group4x2(const IAtom &ia, const IAtom &ib, 
       qz[p] = tz[p];
       qn[p] = tn[p];
     }
+    if constexpr (SAFE) {
+      // eight increments of 0 after the first group's reads, as after every
+      // later group's reads: the loop's waits for those reads then leave the
+      // increments in flight on every path into the loop
+      asm volatile("" ::: "memory");
+      uint32_t z;
+      asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+#pragma unroll
+      for (int p = 0; p < 8; ++p) atomicAdd(&win[(threadIdx.x + 64u * p) & (WCAP - 1)], z);
+    }
   }
   for (uint32_t j0 = 0; j0 < ng; j0 += 4) {
     uint32_t k[8];
     bool bad[8], any = false;
+    uint32_t lo[8];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
       const double rx = qx[p], ry = qy[p], rz = qz[p], rn = qn[p];
@@ -761,25 +801,40 @@ __device__ __forceinline__ void pair_group4x2(const IAtom &ia, const IAtom &ib, 
       s = __builtin_fma(ia.az, rz, s);
       s = __builtin_fma(ia.ay, ry, s);
       s = __builtin_fma(ia.ax, rx, s);
-      k[p] = fast_bin(s, inv_bin, band, bad[p]);
       double t = ib.n + rn;
       t = __builtin_fma(ib.az, rz, t);
       t = __builtin_fma(ib.ay, ry, t);
       t = __builtin_fma(ib.ax, rx, t);
-      k[4 + p] = fast_bin(t, inv_bin, band, bad[4 + p]);
-      any |= bad[p] | bad[4 + p];
+      if constexpr (NARROW) {
+        k[p] = fix_bin(s, inv_bin, fb.m, lo[p]);
+        k[4 + p] = fix_bin(t, inv_bin, fb.m, lo[4 + p]);
+      } else {
+        k[p] = fast_bin(s, inv_bin, band, bad[p]);
+        k[4 + p] = fast_bin(t, inv_bin, band, bad[4 + p]);
+        any |= bad[p] | bad[4 + p];
+      }
+    }
+    if constexpr (NARROW) {
+      // one test of the smallest low word; then all 8 pairs take the exact
+      // path (its result equals the fast one wherever that one holds)
+      const uint32_t m = min(min(min(lo[0], lo[1]), min(lo[2], lo[3])),
+                             min(min(lo[4], lo[5]), min(lo[6], lo[7])));
+      any = m <= fb.lm1;
+#pragma unroll
+      for (int p = 0; p < 8; ++p) bad[p] = true;
     }
     if (__builtin_expect(any, 0)) {
 #pragma unroll
       for (int p = 0; p < 8; ++p) {
         if (bad[p]) {
-          const double xi = p < 4 ? xa : xb, yi = p < 4 ? ya : yb, zi = p < 4 ? za : zb;
+          const size_t g = opaque_index(p < 4 ? ga : gb);
+          const double xi = xg[g], yi = yg[g], zi = zg[g];
           const double dx = xi - sx[j0 + (p & 3)];
           const double dy = yi - sy[j0 + (p & 3)];
           const double dz = zi - sz[j0 + (p & 3)];
           double s = dx * dx + dy * dy;
           s = s + dz * dz;
-          k[p] = exact_bin<double>(s, bin, nhist);
+          k[p] = exact_bin<double>(s, bin, nhist) + kb;
         }
       }
     }
@@ -793,22 +848,20 @@ __device__ __forceinline__ void pair_group4x2(const IAtom &ia, const IAtom &ib, 
       }
     }
     if constexpr (SAFE) {
-      if (va) {
+      // slot k - kb + off, a window index (32-bit wrap-around); after the
+      // next group's reads
+      asm volatile("" ::: "memory");
 #pragma unroll
-        for (int p = 0; p < 4; ++p) atomicAdd(&win[k[p] + off], 1u);
-      }
-      if (vb) {
-#pragma unroll
-        for (int p = 4; p < 8; ++p) atomicAdd(&win[k[p] + off], 1u);
-      }
+      for (int p = 0; p < 8; ++p) atomicAdd(&win[k[p] + offk], p < 4 ? inca : incb);
     } else {
 #pragma unroll
       for (int p = 0; p < 8; ++p) {
         if (!(p < 4 ? va : vb)) continue;
-        if (k[p] - B < wlim)
-          atomicAdd(&win[(k[p] + D) & (WCAP - 1)], 1u);
-        else if (k[p] < nhist)
-          atomicAdd(&hp[k[p]], 1ull);
+        const uint32_t kp = k[p] - kb;
+        if (kp - B < wlim)
+          atomicAdd(&win[(kp + D) & (WCAP - 1)], 1u);
+        else if (kp < nhist)
+          atomicAdd(&hp[kp], 1ull);
         else
           atomicAdd(ovf, 1ull);
       }
@@ -824,7 +877,7 @@ __device__ __forceinline__ void pair_group4x2(const IAtom &ia, const IAtom &ib, 
 // within the item's own tile pair the J tile 2 ti pairs with I tile 2 ti only
 // (j > i) and J tile 2 ti + 1 with I tile 2 ti fully and with 2 ti + 1 for
 // j > i; those two J tiles take the per-pair loop.
-template <int WCAP>
+template <int WCAP, bool NARROW>
 __global__ __launch_bounds__(kTile) void hist_kernel_f64(Args<double> A) {
   __shared__ uint32_t win[WCAP];
   __shared__ unsigned long long keys[kSortCap];
@@ -838,20 +891,14 @@ __global__ __launch_bounds__(kTile) void hist_kernel_f64(Args<double> A) {
   const bool hasB = tB < td.a_n;
   const Tile IA = A.tiles[td.a_base + tA];
   const Tile IB = hasB ? A.tiles[td.a_base + tB] : Tile{0u, 0u};
+  if (IA.cnt == 0) return;
   const bool va = tid < IA.cnt, vb = tid < IB.cnt;
-  double xa = 0, ya = 0, za = 0, xb = 0, yb = 0, zb = 0;
-  if (va) {
-    const size_t g = (size_t)IA.off + tid;
-    xa = A.x[g];
-    ya = A.y[g];
-    za = A.z[g];
-  }
-  if (vb) {
-    const size_t g = (size_t)IB.off + tid;
-    xb = A.x[g];
-    yb = A.y[g];
-    zb = A.z[g];
-  }
+  // an I atom that does not exist stands in as the first atom of tile tA:
+  // its counts are dropped, and its bins lie inside every J tile's range
+  const size_t ga = (size_t)IA.off + (va ? tid : 0u);
+  const size_t gb = vb ? (size_t)IB.off + tid : (size_t)IA.off;
+  const double xa = A.x[ga], ya = A.y[ga], za = A.z[ga];
+  const double xb = A.x[gb], yb = A.y[gb], zb = A.z[gb];
   const uint32_t nhist = td.nhist;
   const double bin = td.bin;
   // union box of the two I tiles
@@ -869,6 +916,7 @@ __global__ __launch_bounds__(kTile) void hist_kernel_f64(Args<double> A) {
   }
   double c[3];
   const double band = tile_band(bu, td.band, td.inv_bin, nhist, c);
+  const FixBand fb = fix_band(band);
   IAtom ia, ib;
   {
     double u = xa - c[0], v = ya - c[1], w = za - c[2];
@@ -961,35 +1009,40 @@ __global__ __launch_bounds__(kTile) void hist_kernel_f64(Args<double> A) {
         tn[tid] = fma(w, w, fma(v, v, u * u));
       }
       __syncthreads();
+      // the I coordinates are read again where the per-pair loop needs them,
+      // so that they take no registers in the grouped loop
       if (jA) {
+        const size_t g = opaque_index(ga);
         if (va)
-          pair_block<double, WCAP, true>(xa, ya, za, tid, sx, sy, sz, cn, s0, bin, td.inv_bin,
-                                         td.band, nhist, B, wlim, D, win, hp, ovf);
+          pair_block<double, WCAP, true>(A.x[g], A.y[g], A.z[g], tid, sx, sy, sz, cn, s0, bin,
+                                         td.inv_bin, td.band, nhist, B, wlim, D, win, hp, ovf);
       } else if (jB) {
+        const size_t g = opaque_index(ga), h = opaque_index(gb);
         if (va)
-          pair_block<double, WCAP, false>(xa, ya, za, tid, sx, sy, sz, cn, s0, bin, td.inv_bin,
-                                          td.band, nhist, B, wlim, D, win, hp, ovf);
+          pair_block<double, WCAP, false>(A.x[g], A.y[g], A.z[g], tid, sx, sy, sz, cn, s0, bin,
+                                          td.inv_bin, td.band, nhist, B, wlim, D, win, hp, ovf);
         if (vb)
-          pair_block<double, WCAP, true>(xb, yb, zb, tid, sx, sy, sz, cn, s0, bin, td.inv_bin,
-                                         td.band, nhist, B, wlim, D, win, hp, ovf);
+          pair_block<double, WCAP, true>(A.x[h], A.y[h], A.z[h], tid, sx, sy, sz, cn, s0, bin,
+                                         td.inv_bin, td.band, nhist, B, wlim, D, win, hp, ovf);
       } else if (va || vb) {
         const uint32_t ng = cn & ~3u;
         if (safe)
-          pair_group4x2<WCAP, true>(ia, ib, va, vb, xa, ya, za, xb, yb, zb, sx, sy, sz, tx, ty, tz,
-                                    tn, ng, bin, td.inv_bin, band, nhist, B, wlim, D, off, win, hp,
-                                    ovf);
+          pair_group4x2<WCAP, true, NARROW>(ia, ib, va, vb, A.x, A.y, A.z, ga, gb, sx, sy, sz, tx,
+                                            ty, tz, tn, ng, bin, td.inv_bin, band, fb, nhist, B,
+                                            wlim, D, off, win, hp, ovf);
         else
-          pair_group4x2<WCAP, false>(ia, ib, va, vb, xa, ya, za, xb, yb, zb, sx, sy, sz, tx, ty, tz,
-                                     tn, ng, bin, td.inv_bin, band, nhist, B, wlim, D, off, win, hp,
-                                     ovf);
+          pair_group4x2<WCAP, false, NARROW>(ia, ib, va, vb, A.x, A.y, A.z, ga, gb, sx, sy, sz, tx,
+                                             ty, tz, tn, ng, bin, td.inv_bin, band, fb, nhist, B,
+                                             wlim, D, off, win, hp, ovf);
+        const size_t g = opaque_index(ga), h = opaque_index(gb);
         if (va)
-          pair_block<double, WCAP, false>(xa, ya, za, tid, sx + ng, sy + ng, sz + ng, cn - ng,
-                                          s0 + ng, bin, td.inv_bin, td.band, nhist, B, wlim, D,
-                                          win, hp, ovf);
+          pair_block<double, WCAP, false>(A.x[g], A.y[g], A.z[g], tid, sx + ng, sy + ng, sz + ng,
+                                          cn - ng, s0 + ng, bin, td.inv_bin, td.band, nhist, B,
+                                          wlim, D, win, hp, ovf);
         if (vb)
-          pair_block<double, WCAP, false>(xb, yb, zb, tid, sx + ng, sy + ng, sz + ng, cn - ng,
-                                          s0 + ng, bin, td.inv_bin, td.band, nhist, B, wlim, D,
-                                          win, hp, ovf);
+          pair_block<double, WCAP, false>(A.x[h], A.y[h], A.z[h], tid, sx + ng, sy + ng, sz + ng,
+                                          cn - ng, s0 + ng, bin, td.inv_bin, td.band, nhist, B,
+                                          wlim, D, win, hp, ovf);
       }
       __syncthreads();
     }
@@ -1206,10 +1259,17 @@ static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wc
   if (prof) XS_TRY(prof_begin(ctx, "hist", &ev));
   const dim3 grid((unsigned)nitems), block(kTile);
   if constexpr (sizeof(T) == 8) {
-    if (wcap == 8192)
-      hipLaunchKernelGGL((hist_kernel_f64<8192>), grid, block, 0, ctx->stream, A);
-    else
-      hipLaunchKernelGGL((hist_kernel_f64<32768>), grid, block, 0, ctx->stream, A);
+    if (wcap == 8192) {
+      if (A.narrow)
+        hipLaunchKernelGGL((hist_kernel_f64<8192, true>), grid, block, 0, ctx->stream, A);
+      else
+        hipLaunchKernelGGL((hist_kernel_f64<8192, false>), grid, block, 0, ctx->stream, A);
+    } else {
+      if (A.narrow)
+        hipLaunchKernelGGL((hist_kernel_f64<32768, true>), grid, block, 0, ctx->stream, A);
+      else
+        hipLaunchKernelGGL((hist_kernel_f64<32768, false>), grid, block, 0, ctx->stream, A);
+    }
   } else {
     if (wcap == 8192) {
       if (A.quot_f32)
@@ -1506,6 +1566,7 @@ static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_co
   XS_TRY(prof_end(ctx, "hist_prep", ev));
   Args<T> A{sxp, syp, szp, tiles_d, boxes, types_d, items_d, hist_d, ovf_d};
   A.quot_f32 = quot;
+  A.narrow = nhist <= kNarrowMax;
   return launch_hist<T>(ctx, A, items.size(), wcap, true);
 }
 
@@ -1561,6 +1622,7 @@ static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_coun
   hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, ctx->stream, ovf_d, (size_t)1);
   Args<T> A{sorted, sorted + n, sorted + 2 * n, tiles_d, boxes_d, types_d, items_d, hist_d, ovf_d};
   A.quot_f32 = quot;
+  A.narrow = nhist <= kNarrowMax;
   XS_TRY(launch_hist<T>(ctx, A, items.size(), wcap_for(nhist), true));
   XS_HIP_CHECK(hipMemcpyAsync(hist_out, hist_d, hsize * 8, hipMemcpyDeviceToHost, ctx->stream));
   XS_HIP_CHECK(hipMemcpyAsync(ctx->pin, ovf_d, 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -2230,6 +2292,7 @@ static int plan_hist(xs_batch *b, unsigned long long *hist_d, bool prof) {
             hist_d,
             b->ovf};
   A.quot_f32 = b->quot_f32;
+  A.narrow = std::all_of(b->nhist.begin(), b->nhist.end(), [](unsigned h) { return h <= kNarrowMax; });
   XS_TRY(launch_hist<T>(ctx, A, b->items8192, 8192, prof));
   A.items = items + b->items8192;
   XS_TRY(launch_hist<T>(ctx, A, b->items32768, 32768, prof));

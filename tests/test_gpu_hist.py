@@ -267,6 +267,46 @@ def test_fp64_tile_centred_distances(ctx, case):
     assert np.array_equal(devh, want)
 
 
+@pytest.mark.parametrize("nhist", [1 << 19, (1 << 19) + 1, None])
+def test_fp64_grid_at_the_fixed_point_limit(ctx, nhist):
+    """Grids of at most 2^19 bins take the FP64 fast path in fixed point (the
+    estimate's floor and fraction read from the words of estimate + 2^20),
+    wider ones the fract form.  A random cluster whose distances span about
+    495,000 bins of 1.4e-4 A, binned on 2^19 and 2^19 + 1 bins, and on its
+    own 693,000 bins of 1e-4 A: host and device paths, bit-exact."""
+    rng = np.random.Generator(np.random.PCG64(1931))
+    x64 = np.ascontiguousarray(rng.uniform(-20.0, 20.0, (3, 2500)))
+    el = [1400, 1100]
+    bin_ = 1.4e-4 if nhist else 1.0e-4
+    nh = nhist or O.hist_size(x64, bin_)
+    assert nh >= O.hist_size(x64, bin_)
+    want, oo = O.hist_f64(x64, el, bin_, nh)
+    host, ov_h, devh, ov_d = _hist64_host_and_device(ctx, x64, el, bin_, nh)
+    print(f"nhist={nh} pairs={int(want.sum())} last bin={int(np.nonzero(want.sum(axis=0))[0].max())} "
+          f"host mismatched={int(np.sum(host != want))} device mismatched={int(np.sum(devh != want))}",
+          flush=True)
+    assert ov_h == oo and ov_d == oo
+    assert np.array_equal(host, want)
+    assert np.array_equal(devh, want)
+
+
+def test_fp64_coincident_atoms_of_two_elements(ctx):
+    """Two elements on the same sites (a mixed site written out as two atoms):
+    the cross-type pairs lie at distance 0, where the tile-centred squared
+    distance rounds to 0 or a tiny value of either sign and the fast path's
+    estimate is NaN, infinite or near 0.  They must land in bin 0 as in the
+    reference expression; FP64 and FP32 binning, bit-exact."""
+    ens = st.fcc_sphere(radius=14.0, seed=9)
+    x64 = ens.xyz64()
+    n = x64.shape[1]
+    both = st.Ensemble(np.ascontiguousarray(np.concatenate([ens.xyz, ens.xyz], axis=1)), [n, n],
+                       ["A", "B"])
+    r32, r64 = both_modes(ctx, both, 0.01)
+    assert r64[2][1][0] >= n
+    assert_same(r64, "coincident f64")
+    assert_same(r32, "coincident f32")
+
+
 @pytest.mark.parametrize("f64", [False, True])
 @pytest.mark.parametrize("axis", [0, 1, 2])
 def test_nan_coordinate_rejected(ctx, f64, axis):
