@@ -77,6 +77,7 @@ struct Args {
   const Item *items;
   unsigned long long *hist;
   unsigned long long *overflow;
+  const double *rec;  // FP64 binning: J records, 4 per atom (hist_tile_records)
   uint32_t quot_f32;  // FP32 binning: the bin passed quot_f32_check_kernel
   uint32_t narrow;    // FP64 binning: every pair type has at most kNarrowMax bins
 };
@@ -507,16 +508,19 @@ __device__ __forceinline__ size_t opaque_index(size_t g) {
   return g;
 }
 
-// FP64 fast path of one I atom: its coordinates relative to a centre c of
-// its I tiles' box times -2, and its squared distance from c.  With the J
-// atoms' coordinates relative to c (t) and their squared norms (tn) staged in
-// LDS, |ri - rj|^2 = n + tn + (-2 ri').rj' costs one add and three FMAs.
+// FP64 fast path of one I atom: its coordinates relative to the centre c of
+// a J tile's box times -2, and its squared distance from c.  With the J
+// atoms' coordinates relative to c (rj') and their squared norms (nj) in the
+// J records, |ri - rj|^2 = n + nj + (-2 ri').rj' costs one add and three
+// FMAs.
 struct IAtom {
   double ax, ay, az, n;
 };
 
-// Centre c of an I box (bi, 6 doubles) and the half-width of the FP64 fast
-// path's band for I atoms in that box taken relative to c.
+// Centre c of a J tile's box (bi, 6 doubles) and the half-width of the FP64
+// fast path's band for pairs formed relative to c.  The bound below is
+// symmetric in the two atoms, so it holds with the J box as the one of
+// radius rho.
 // Relative to IEEE sqrt(s)/bin the estimate errs by (a) below nhist 2^-45.3
 // bins from v_rsq_f64, the Newton step and the roundings of the reference
 // expression, and (b) below u (64 R^2 + 36 R + 10 T) bins from forming s as
@@ -741,23 +745,28 @@ __global__ __launch_bounds__(kTile) void hist_kernel_f32(Args<float> A) {
   flush_window<WCAP>(win, B, min(B + (uint32_t)WCAP, nhist), D, hp);
 }
 
+// Loads through the constant address space: with a uniform address they
+// become scalar loads into SGPRs (the J records below), read-only data.
+typedef const double __attribute__((address_space(4))) cdouble;
+
 // FP64 groups of 4 J atoms against the thread's two I atoms a and b (8
-// independent chains; each J record read from LDS serves two pairs).  Counts
-// of an I atom that does not exist (va / vb false) are dropped.  The exact
-// evaluation reads the I coordinates again (xg, yg, zg at ga, gb).
+// independent chains; each J record serves two pairs).  The J records (x',
+// y', z', |r'|^2, relative to the centre of the J tile's box; rec) are read
+// with scalar loads: the address is the same for every lane, so they take
+// SGPRs and no LDS.  Counts of an I atom that does not exist (va / vb false)
+// are dropped.  The exact evaluation reads the coordinates again (xg, yg, zg
+// at ga, gb for the I atoms, jg + j for the J atoms).
 // NARROW: the fixed-point fast path (fix_bin; nhist <= kNarrowMax), k[] then
 // holding kFixHi + bin.
 template <int WCAP, bool SAFE, bool NARROW>
 __device__ __forceinline__ void pair_group4x2(const IAtom &ia, const IAtom &ib, bool va, bool vb,
                                               const double *xg, const double *yg,
-                                              const double *zg, size_t ga, size_t gb,
-                                              const double *sx, const double *sy,
-                                              const double *sz, const double *tx,
-                                              const double *ty, const double *tz, const double *tn,
-                                              uint32_t ng, double bin, double inv_bin, double band,
-                                              const FixBand &fb, uint32_t nhist, uint32_t B,
-                                              uint32_t wlim, uint32_t D, uint32_t off,
-                                              uint32_t *win, unsigned long long *__restrict__ hp,
+                                              const double *zg, size_t ga, size_t gb, size_t jg,
+                                              const double *rec, uint32_t ng, double bin,
+                                              double inv_bin, double band, const FixBand &fb,
+                                              uint32_t nhist, uint32_t B, uint32_t wlim,
+                                              uint32_t D, uint32_t off, uint32_t *win,
+                                              unsigned long long *__restrict__ hp,
                                               unsigned long long *__restrict__ ovf) {
 #pragma clang fp contract(off)
   constexpr uint32_t kb = NARROW ? kFixHi : 0u;  // offset of k[] from the bin
@@ -765,38 +774,17 @@ __device__ __forceinline__ void pair_group4x2(const IAtom &ia, const IAtom &ib, 
   uint32_t offk = off - kb;
   asm volatile("" : "+v"(offk));
   // SAFE: every lane increments for every pair, by 1 or, for an I atom that
-  // does not exist (placed at the I box's centre, so its bins stay inside
-  // the J tile's range), by 0
+  // does not exist (a stand-in whose bins lie in the J tile's range), by 0
   const uint32_t inca = va, incb = vb;
-  // the next group's J records are read before this group's increments, so
-  // that their LDS reads do not queue behind the atomics
-  double qx[4], qy[4], qz[4], qn[4];
-  if (ng) {
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      qx[p] = tx[p];
-      qy[p] = ty[p];
-      qz[p] = tz[p];
-      qn[p] = tn[p];
-    }
-    if constexpr (SAFE) {
-      // eight increments of 0 after the first group's reads, as after every
-      // later group's reads: the loop's waits for those reads then leave the
-      // increments in flight on every path into the loop
-      asm volatile("" ::: "memory");
-      uint32_t z;
-      asm volatile("v_mov_b32 %0, 0" : "=v"(z));
-#pragma unroll
-      for (int p = 0; p < 8; ++p) atomicAdd(&win[(threadIdx.x + 64u * p) & (WCAP - 1)], z);
-    }
-  }
+  cdouble *jr = (cdouble *)rec;
   for (uint32_t j0 = 0; j0 < ng; j0 += 4) {
     uint32_t k[8];
     bool bad[8], any = false;
     uint32_t lo[8];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
-      const double rx = qx[p], ry = qy[p], rz = qz[p], rn = qn[p];
+      const double rx = jr[4 * (j0 + p)], ry = jr[4 * (j0 + p) + 1], rz = jr[4 * (j0 + p) + 2],
+                   rn = jr[4 * (j0 + p) + 3];
       double s = ia.n + rn;
       s = __builtin_fma(ia.az, rz, s);
       s = __builtin_fma(ia.ay, ry, s);
@@ -827,30 +815,18 @@ __device__ __forceinline__ void pair_group4x2(const IAtom &ia, const IAtom &ib, 
 #pragma unroll
       for (int p = 0; p < 8; ++p) {
         if (bad[p]) {
-          const size_t g = opaque_index(p < 4 ? ga : gb);
-          const double xi = xg[g], yi = yg[g], zi = zg[g];
-          const double dx = xi - sx[j0 + (p & 3)];
-          const double dy = yi - sy[j0 + (p & 3)];
-          const double dz = zi - sz[j0 + (p & 3)];
+          const size_t g = opaque_index(p < 4 ? ga : gb), h = jg + j0 + (p & 3);
+          const double dx = xg[g] - xg[h];
+          const double dy = yg[g] - yg[h];
+          const double dz = zg[g] - zg[h];
           double s = dx * dx + dy * dy;
           s = s + dz * dz;
           k[p] = exact_bin<double>(s, bin, nhist) + kb;
         }
       }
     }
-    if (j0 + 4 < ng) {
-#pragma unroll
-      for (int p = 0; p < 4; ++p) {
-        qx[p] = tx[j0 + 4 + p];
-        qy[p] = ty[j0 + 4 + p];
-        qz[p] = tz[j0 + 4 + p];
-        qn[p] = tn[j0 + 4 + p];
-      }
-    }
     if constexpr (SAFE) {
-      // slot k - kb + off, a window index (32-bit wrap-around); after the
-      // next group's reads
-      asm volatile("" ::: "memory");
+      // slot k - kb + off, a window index (32-bit wrap-around)
 #pragma unroll
       for (int p = 0; p < 8; ++p) atomicAdd(&win[k[p] + offk], p < 4 ? inca : incb);
     } else {
@@ -869,20 +845,48 @@ __device__ __forceinline__ void pair_group4x2(const IAtom &ia, const IAtom &ib, 
   }
 }
 
+// I atom relative to a centre c, for the fast path.
+__device__ __forceinline__ IAtom centred(double x, double y, double z, const double c[3]) {
+  const double u = x - c[0], v = y - c[1], w = z - c[2];
+  return IAtom{-2.0 * u, -2.0 * v, -2.0 * w, fma(w, w, fma(v, v, u * u))};
+}
+
+// Records of the FP64 kernel's J atoms: each atom relative to the centre c
+// of its tile's box (the centre tile_band takes), (x - c, y - c, z - c,
+// |r - c|^2), four doubles per atom in the sorted order.
+__global__ void hist_tile_records(const double *__restrict__ x, const double *__restrict__ y,
+                                  const double *__restrict__ z, const Tile *__restrict__ tiles,
+                                  const double *__restrict__ boxes, double *__restrict__ rec) {
+  const Tile t = tiles[blockIdx.x];
+  const double *b = boxes + 6 * (size_t)blockIdx.x;
+  double c[3];
+  for (int a = 0; a < 3; ++a) c[a] = 0.5 * (b[a] + b[3 + a]);
+  for (uint32_t i = threadIdx.x; i < t.cnt; i += blockDim.x) {
+    const size_t g = (size_t)t.off + i;
+    const double u = x[g] - c[0], v = y[g] - c[1], w = z[g] - c[2];
+    double *r = rec + 4 * g;
+    r[0] = u;
+    r[1] = v;
+    r[2] = w;
+    r[3] = fma(w, w, fma(v, v, u * u));
+  }
+}
+
 // FP64 binning: one work item = (pair type, I tile pair {2 ti, 2 ti + 1},
 // run of J tiles), two I atoms per thread (atom tid of each tile).  The J
 // tiles are sorted and the window managed for the union box of the two I
-// tiles; J records are taken relative to that box's centre.  Like-element
-// types visit tile pairs by a half rotation over I tile pairs (add_items), so
-// within the item's own tile pair the J tile 2 ti pairs with I tile 2 ti only
-// (j > i) and J tile 2 ti + 1 with I tile 2 ti fully and with 2 ti + 1 for
-// j > i; those two J tiles take the per-pair loop.
+// tiles.  Distances are formed relative to the centre of each J tile's box:
+// the J records hold the J atoms relative to it (A.rec), and the I atoms are
+// taken relative to it once per J tile.  Like-element types visit tile pairs
+// by a half rotation over I tile pairs (add_items), so within the item's own
+// tile pair the J tile 2 ti pairs with I tile 2 ti only (j > i) and J tile
+// 2 ti + 1 with I tile 2 ti fully and with 2 ti + 1 for j > i; those two J
+// tiles take the per-pair loop.  No J atoms are staged in LDS, so the work
+// group synchronises only where the window moves.
 template <int WCAP, bool NARROW>
 __global__ __launch_bounds__(kTile) void hist_kernel_f64(Args<double> A) {
   __shared__ uint32_t win[WCAP];
   __shared__ unsigned long long keys[kSortCap];
-  __shared__ double sx[kStage], sy[kStage], sz[kStage];
-  __shared__ double tx[kStage], ty[kStage], tz[kStage], tn[kStage];
 
   const Item it = A.items[blockIdx.x];
   const TypeDesc td = A.types[it.type];
@@ -913,18 +917,6 @@ __global__ __launch_bounds__(kTile) void hist_kernel_f64(Args<double> A) {
         bu[3 + a] = fmax(bu[3 + a], b1[3 + a]);
       }
     }
-  }
-  double c[3];
-  const double band = tile_band(bu, td.band, td.inv_bin, nhist, c);
-  const FixBand fb = fix_band(band);
-  IAtom ia, ib;
-  {
-    double u = xa - c[0], v = ya - c[1], w = za - c[2];
-    ia = IAtom{-2.0 * u, -2.0 * v, -2.0 * w, fma(w, w, fma(v, v, u * u))};
-    u = xb - c[0];
-    v = yb - c[1];
-    w = zb - c[2];
-    ib = IAtom{-2.0 * u, -2.0 * v, -2.0 * w, fma(w, w, fma(v, v, u * u))};
   }
   const uint32_t n = it.jcount;
   uint32_t npow = 1;
@@ -968,14 +960,16 @@ __global__ __launch_bounds__(kTile) void hist_kernel_f64(Args<double> A) {
     const uint32_t m = (uint32_t)(keys[s] & 0xffffffffu);
     const uint32_t tj = (it.jstart + m) % td.b_n;
     const uint32_t tjg = td.b_base + tj;
+    const double *bj = A.boxes + 6 * (size_t)tjg;
     uint32_t kl, ku;
     bool fits;
-    box_bins(bu, A.boxes + 6 * (size_t)tjg, bin, nhist, kl, ku, fits);
+    box_bins(bu, bj, bin, nhist, kl, ku, fits);
     uint32_t wlim = 0;
     const bool span_ok = ku - kl + 1 <= (uint32_t)WCAP;
     if (span_ok) {
       const uint32_t nb = ku + 1 > (uint32_t)WCAP ? ku + 1 - WCAP : 0u;
       if (nb > B) {
+        __syncthreads();
         flush_window<WCAP>(win, B, min(min(nb, B + (uint32_t)WCAP), nhist), D, hp);
         B = nb;
         __syncthreads();
@@ -992,61 +986,51 @@ __global__ __launch_bounds__(kTile) void hist_kernel_f64(Args<double> A) {
       off = D - ((kl + D) & ~(uint32_t)(WCAP - 1));
     }
     const Tile J = A.tiles[tjg];
-    // like-element types: J tiles of the item's own tile pair
+    const double *jx = A.x + J.off, *jy = A.y + J.off, *jz = A.z + J.off;
+    // like-element types: J tiles of the item's own tile pair.  The I
+    // coordinates are read again where the per-pair loop needs them, so that
+    // they take no registers in the grouped loop.
     const bool jA = td.same && tj == tA, jB = td.same && hasB && tj == tB;
-    for (uint32_t s0 = 0; s0 < J.cnt; s0 += kStage) {
-      const uint32_t cn = min((uint32_t)kStage, J.cnt - s0);
-      if (tid < cn) {
-        const size_t g = (size_t)J.off + s0 + tid;
-        const double x = A.x[g], y = A.y[g], z = A.z[g];
-        sx[tid] = x;
-        sy[tid] = y;
-        sz[tid] = z;
-        const double u = x - c[0], v = y - c[1], w = z - c[2];
-        tx[tid] = u;
-        ty[tid] = v;
-        tz[tid] = w;
-        tn[tid] = fma(w, w, fma(v, v, u * u));
-      }
-      __syncthreads();
-      // the I coordinates are read again where the per-pair loop needs them,
-      // so that they take no registers in the grouped loop
-      if (jA) {
-        const size_t g = opaque_index(ga);
-        if (va)
-          pair_block<double, WCAP, true>(A.x[g], A.y[g], A.z[g], tid, sx, sy, sz, cn, s0, bin,
-                                         td.inv_bin, td.band, nhist, B, wlim, D, win, hp, ovf);
-      } else if (jB) {
-        const size_t g = opaque_index(ga), h = opaque_index(gb);
-        if (va)
-          pair_block<double, WCAP, false>(A.x[g], A.y[g], A.z[g], tid, sx, sy, sz, cn, s0, bin,
-                                          td.inv_bin, td.band, nhist, B, wlim, D, win, hp, ovf);
-        if (vb)
-          pair_block<double, WCAP, true>(A.x[h], A.y[h], A.z[h], tid, sx, sy, sz, cn, s0, bin,
-                                         td.inv_bin, td.band, nhist, B, wlim, D, win, hp, ovf);
-      } else if (va || vb) {
-        const uint32_t ng = cn & ~3u;
-        if (safe)
-          pair_group4x2<WCAP, true, NARROW>(ia, ib, va, vb, A.x, A.y, A.z, ga, gb, sx, sy, sz, tx,
-                                            ty, tz, tn, ng, bin, td.inv_bin, band, fb, nhist, B,
-                                            wlim, D, off, win, hp, ovf);
-        else
-          pair_group4x2<WCAP, false, NARROW>(ia, ib, va, vb, A.x, A.y, A.z, ga, gb, sx, sy, sz, tx,
-                                             ty, tz, tn, ng, bin, td.inv_bin, band, fb, nhist, B,
-                                             wlim, D, off, win, hp, ovf);
-        const size_t g = opaque_index(ga), h = opaque_index(gb);
-        if (va)
-          pair_block<double, WCAP, false>(A.x[g], A.y[g], A.z[g], tid, sx + ng, sy + ng, sz + ng,
-                                          cn - ng, s0 + ng, bin, td.inv_bin, td.band, nhist, B,
-                                          wlim, D, win, hp, ovf);
-        if (vb)
-          pair_block<double, WCAP, false>(A.x[h], A.y[h], A.z[h], tid, sx + ng, sy + ng, sz + ng,
-                                          cn - ng, s0 + ng, bin, td.inv_bin, td.band, nhist, B,
-                                          wlim, D, win, hp, ovf);
-      }
-      __syncthreads();
+    if (jA) {
+      const size_t g = opaque_index(ga);
+      if (va)
+        pair_block<double, WCAP, true>(A.x[g], A.y[g], A.z[g], tid, jx, jy, jz, J.cnt, 0u, bin,
+                                       td.inv_bin, td.band, nhist, B, wlim, D, win, hp, ovf);
+    } else if (jB) {
+      const size_t g = opaque_index(ga), h = opaque_index(gb);
+      if (va)
+        pair_block<double, WCAP, false>(A.x[g], A.y[g], A.z[g], tid, jx, jy, jz, J.cnt, 0u, bin,
+                                        td.inv_bin, td.band, nhist, B, wlim, D, win, hp, ovf);
+      if (vb)
+        pair_block<double, WCAP, true>(A.x[h], A.y[h], A.z[h], tid, jx, jy, jz, J.cnt, 0u, bin,
+                                       td.inv_bin, td.band, nhist, B, wlim, D, win, hp, ovf);
+    } else if (va || vb) {
+      double c[3];
+      const double band = tile_band(bj, td.band, td.inv_bin, nhist, c);
+      const FixBand fb = fix_band(band);
+      const IAtom ia = centred(xa, ya, za, c), ib = centred(xb, yb, zb, c);
+      const uint32_t ng = J.cnt & ~3u;
+      const double *rec = A.rec + 4 * (size_t)J.off;
+      if (safe)
+        pair_group4x2<WCAP, true, NARROW>(ia, ib, va, vb, A.x, A.y, A.z, ga, gb, J.off, rec, ng,
+                                          bin, td.inv_bin, band, fb, nhist, B, wlim, D, off, win,
+                                          hp, ovf);
+      else
+        pair_group4x2<WCAP, false, NARROW>(ia, ib, va, vb, A.x, A.y, A.z, ga, gb, J.off, rec, ng,
+                                           bin, td.inv_bin, band, fb, nhist, B, wlim, D, off,
+                                           win, hp, ovf);
+      const size_t g = opaque_index(ga), h = opaque_index(gb);
+      if (va)
+        pair_block<double, WCAP, false>(A.x[g], A.y[g], A.z[g], tid, jx + ng, jy + ng, jz + ng,
+                                        J.cnt - ng, ng, bin, td.inv_bin, td.band, nhist, B, wlim,
+                                        D, win, hp, ovf);
+      if (vb)
+        pair_block<double, WCAP, false>(A.x[h], A.y[h], A.z[h], tid, jx + ng, jy + ng, jz + ng,
+                                        J.cnt - ng, ng, bin, td.inv_bin, td.band, nhist, B, wlim,
+                                        D, win, hp, ovf);
     }
   }
+  __syncthreads();
   flush_window<WCAP>(win, B, min(B + (uint32_t)WCAP, nhist), D, hp);
 }
 
@@ -1252,6 +1236,17 @@ static void sort_structure_host(const T *xyz, size_t n, const unsigned *el_count
   }
 }
 
+// J records of the FP64 kernel (no-op for FP32 binning).
+template <class T>
+static void launch_records(hipStream_t st, const T *x, const T *y, const T *z, const Tile *tiles,
+                           const double *boxes, size_t ntiles, double *rec) {
+  if constexpr (sizeof(T) == 8) {
+    if (ntiles)
+      hipLaunchKernelGGL(hist_tile_records, dim3((unsigned)ntiles), dim3(256), 0, st, x, y, z,
+                         tiles, boxes, rec);
+  }
+}
+
 template <class T>
 static int launch_hist(xs_ctx *ctx, const Args<T> &A, size_t nitems, uint32_t wcap, bool prof) {
   if (!nitems) return XS_OK;
@@ -1453,13 +1448,16 @@ static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_co
   for (unsigned e = 0; e < n_el; ++e) maxne = std::max<uint32_t>(maxne, el_count[e]);
   size_t sortb = 0;
   const size_t eqb = equal_count ? eq_scratch_bytes<T>(std::max(maxne, 1u), &sortb) : 0;
+  const size_t nrec = sizeof(T) == 8 ? 4 * n : 0;  // FP64 J records
   XS_TRY(ensure_workspace(ctx, Carver::need({3 * n * sizeof(T), n * sizeof(uint32_t),
-                                             sizeof(unsigned long long), eqb})));
+                                             sizeof(unsigned long long), eqb,
+                                             nrec * sizeof(double)})));
   Carver cv(ctx->ws);
   T *sorted = cv.take<T>(3 * n);
   uint32_t *keys = cv.take<uint32_t>(n);
   unsigned long long *ovf_own = cv.take<unsigned long long>(1);
   void *eq_scratch = cv.take<char>(eqb);
+  double *rec = cv.take<double>(nrec);
   unsigned long long *ovf_d = ovf_user ? ovf_user : ovf_own;
   T *sxp = sorted, *syp = sorted + n, *szp = sorted + 2 * n;
 
@@ -1547,6 +1545,7 @@ static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_co
   if (!tiles.empty())
     hipLaunchKernelGGL((hist_tile_boxes<T>), dim3((unsigned)tiles.size()), dim3(256), 0, ctx->stream,
                        sxp, syp, szp, tiles_d, boxes);
+  launch_records<T>(ctx->stream, sxp, syp, szp, tiles_d, boxes, tiles.size(), rec);
   XS_HIP_CHECK(hipGetLastError());
   if (equal_count && std::isfinite(cap) && !tiles.empty()) {
     // equal-count tiles of a structure far from uniform density can be wider
@@ -1564,7 +1563,7 @@ static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_co
     }
   }
   XS_TRY(prof_end(ctx, "hist_prep", ev));
-  Args<T> A{sxp, syp, szp, tiles_d, boxes, types_d, items_d, hist_d, ovf_d};
+  Args<T> A{sxp, syp, szp, tiles_d, boxes, types_d, items_d, hist_d, ovf_d, rec};
   A.quot_f32 = quot;
   A.narrow = nhist <= kNarrowMax;
   return launch_hist<T>(ctx, A, items.size(), wcap, true);
@@ -1594,9 +1593,10 @@ static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_coun
 
   const size_t nt = std::max<size_t>(tiles.size(), 1), ni = std::max<size_t>(items.size(), 1);
   XS_TRY(ensure_pinned(ctx, 64));
+  const size_t nrec = sizeof(T) == 8 ? 4 * n : 0;  // FP64 J records
   XS_TRY(ensure_workspace(ctx, Carver::need({3 * n * sizeof(T), nt * sizeof(Tile), nt * 48,
                                              ntypes * sizeof(TypeDesc), ni * sizeof(Item),
-                                             hsize * 8, 8})));
+                                             hsize * 8, 8, nrec * sizeof(double)})));
   Carver cv(ctx->ws);
   T *sorted = cv.take<T>(3 * n);
   Tile *tiles_d = cv.take<Tile>(nt);
@@ -1605,6 +1605,7 @@ static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_coun
   Item *items_d = cv.take<Item>(ni);
   unsigned long long *hist_d = cv.take<unsigned long long>(hsize);
   unsigned long long *ovf_d = cv.take<unsigned long long>(1);
+  double *rec = cv.take<double>(nrec);
   // synchronous uploads from pageable host vectors
   XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   XS_HIP_CHECK(hipMemcpy(sorted, sx.data(), n * sizeof(T), hipMemcpyHostToDevice));
@@ -1620,7 +1621,10 @@ static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_coun
   const unsigned zb = (unsigned)std::min<size_t>((hsize + 255) / 256, 4096);
   hipLaunchKernelGGL(fill_u64, dim3(std::max(zb, 1u)), dim3(256), 0, ctx->stream, hist_d, hsize);
   hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, ctx->stream, ovf_d, (size_t)1);
-  Args<T> A{sorted, sorted + n, sorted + 2 * n, tiles_d, boxes_d, types_d, items_d, hist_d, ovf_d};
+  launch_records<T>(ctx->stream, sorted, sorted + n, sorted + 2 * n, tiles_d, boxes_d,
+                    tiles.size(), rec);
+  Args<T> A{sorted, sorted + n, sorted + 2 * n, tiles_d, boxes_d, types_d, items_d, hist_d, ovf_d,
+            rec};
   A.quot_f32 = quot;
   A.narrow = nhist <= kNarrowMax;
   XS_TRY(launch_hist<T>(ctx, A, items.size(), wcap_for(nhist), true));
@@ -1642,7 +1646,7 @@ namespace hist {
 static void free_plan(xs_batch *b) {
   if (!b) return;
   if (b->ctx) b->ctx->plans.erase(b);
-  for (void *p : {b->coords, b->tiles, b->boxes, b->types, b->items, (void *)b->ovf})
+  for (void *p : {b->coords, b->tiles, b->boxes, b->types, b->items, (void *)b->ovf, b->recs})
     if (p) (void)hipFree(p);
   release_batch_run(b->run);
   delete b;
@@ -1674,7 +1678,8 @@ static int plan_finish(xs_ctx *ctx, xs_batch *b, const std::vector<Tile> &tiles,
       hipMalloc(&b->boxes, std::max<size_t>(boxes.size(), 6) * 8) == hipSuccess &&
       hipMalloc(&b->types, std::max<size_t>(types.size(), 1) * sizeof(TypeDesc)) == hipSuccess &&
       hipMalloc(&b->items, std::max<size_t>(small.size(), 1) * sizeof(Item)) == hipSuccess &&
-      hipMalloc((void **)&b->ovf, std::max<size_t>(b->nstruct, 1) * 8) == hipSuccess;
+      hipMalloc((void **)&b->ovf, std::max<size_t>(b->nstruct, 1) * 8) == hipSuccess &&
+      (sizeof(T) == 4 || hipMalloc(&b->recs, std::max<size_t>(nc, 1) * 32) == hipSuccess);
   if (!ok) {
     (void)hipGetLastError();
     free_plan(b);
@@ -1692,6 +1697,12 @@ static int plan_finish(xs_ctx *ctx, xs_batch *b, const std::vector<Tile> &tiles,
     e = hipMemcpy(b->types, types.data(), types.size() * sizeof(TypeDesc), hipMemcpyHostToDevice);
   if (e == hipSuccess && !small.empty())
     e = hipMemcpy(b->items, small.data(), small.size() * sizeof(Item), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    launch_records<T>(ctx->stream, c, c + nc, c + 2 * nc, static_cast<const Tile *>(b->tiles),
+                      static_cast<const double *>(b->boxes), tiles.size(),
+                      static_cast<double *>(b->recs));
+    e = hipGetLastError();
+  }
   if (e != hipSuccess) {
     free_plan(b);
     return fail(XS_ERR_HIP, std::string("xs_batch_plan: upload failed: ") + hipGetErrorString(e));
@@ -2290,7 +2301,8 @@ static int plan_hist(xs_batch *b, unsigned long long *hist_d, bool prof) {
             static_cast<const TypeDesc *>(b->types),
             items,
             hist_d,
-            b->ovf};
+            b->ovf,
+            static_cast<const double *>(b->recs)};
   A.quot_f32 = b->quot_f32;
   A.narrow = std::all_of(b->nhist.begin(), b->nhist.end(), [](unsigned h) { return h <= kNarrowMax; });
   XS_TRY(launch_hist<T>(ctx, A, b->items8192, 8192, prof));

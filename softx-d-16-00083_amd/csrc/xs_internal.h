@@ -82,6 +82,7 @@ struct xs_batch {
   size_t natoms = 0;      // per axis in the sorted coordinate arrays
   void *coords = nullptr;  // sorted SoA (float or double)
   void *tiles = nullptr, *boxes = nullptr, *types = nullptr, *items = nullptr;
+  void *recs = nullptr;  // FP64 binning: J records, 4 doubles per atom
   unsigned long long *ovf = nullptr;  // one overflow counter per structure
   xs::BatchRunState run;
 };
