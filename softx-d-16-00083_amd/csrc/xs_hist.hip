@@ -1029,8 +1029,10 @@ __global__ __launch_bounds__(kTile) void hist_kernel_f64(Args<double> A) {
                                         J.cnt - ng, ng, bin, td.inv_bin, td.band, nhist, B, wlim,
                                         D, win, hp, ovf);
     }
+    // the work group's waves stay on one J tile, so that its records are
+    // read through the scalar cache once rather than once per wave
+    __syncthreads();
   }
-  __syncthreads();
   flush_window<WCAP>(win, B, min(B + (uint32_t)WCAP, nhist), D, hp);
 }
 
