@@ -59,6 +59,7 @@ namespace hist {
 
 constexpr int kStage = 256;      // J atoms staged in LDS per step
 constexpr int kBoxBlocks = 512;  // partial bounding boxes
+constexpr int kChunk = 256;      // FP64 binning: J atoms between barriers (8 KB of records)
 constexpr double kFill = 0.94 * kTile;  // target atoms in the fullest cell
 constexpr uint32_t kMaxCellsPerAxis = 256;
 
@@ -1004,21 +1005,31 @@ __global__ __launch_bounds__(kTile) void hist_kernel_f64(Args<double> A) {
       if (vb)
         pair_block<double, WCAP, true>(A.x[h], A.y[h], A.z[h], tid, jx, jy, jz, J.cnt, 0u, bin,
                                        td.inv_bin, td.band, nhist, B, wlim, D, win, hp, ovf);
-    } else if (va || vb) {
+    } else {
       double c[3];
       const double band = tile_band(bj, td.band, td.inv_bin, nhist, c);
       const FixBand fb = fix_band(band);
       const IAtom ia = centred(xa, ya, za, c), ib = centred(xb, yb, zb, c);
       const uint32_t ng = J.cnt & ~3u;
-      const double *rec = A.rec + 4 * (size_t)J.off;
-      if (safe)
-        pair_group4x2<WCAP, true, NARROW>(ia, ib, va, vb, A.x, A.y, A.z, ga, gb, J.off, rec, ng,
-                                          bin, td.inv_bin, band, fb, nhist, B, wlim, D, off, win,
-                                          hp, ovf);
-      else
-        pair_group4x2<WCAP, false, NARROW>(ia, ib, va, vb, A.x, A.y, A.z, ga, gb, J.off, rec, ng,
-                                           bin, td.inv_bin, band, fb, nhist, B, wlim, D, off,
-                                           win, hp, ovf);
+      // chunks of kChunk J atoms with a barrier between them: the work
+      // group's waves read the same records, which the scalar cache then
+      // serves once rather than once per wave
+      for (uint32_t c0 = 0; c0 < ng; c0 += kChunk) {
+        const uint32_t cn = min((uint32_t)kChunk, ng - c0);
+        const size_t jg = (size_t)J.off + c0;
+        const double *rec = A.rec + 4 * jg;
+        if (va || vb) {
+          if (safe)
+            pair_group4x2<WCAP, true, NARROW>(ia, ib, va, vb, A.x, A.y, A.z, ga, gb, jg, rec, cn,
+                                              bin, td.inv_bin, band, fb, nhist, B, wlim, D, off,
+                                              win, hp, ovf);
+          else
+            pair_group4x2<WCAP, false, NARROW>(ia, ib, va, vb, A.x, A.y, A.z, ga, gb, jg, rec, cn,
+                                               bin, td.inv_bin, band, fb, nhist, B, wlim, D, off,
+                                               win, hp, ovf);
+        }
+        if (c0 + kChunk < ng) __syncthreads();
+      }
       const size_t g = opaque_index(ga), h = opaque_index(gb);
       if (va)
         pair_block<double, WCAP, false>(A.x[g], A.y[g], A.z[g], tid, jx + ng, jy + ng, jz + ng,
@@ -1029,8 +1040,6 @@ __global__ __launch_bounds__(kTile) void hist_kernel_f64(Args<double> A) {
                                         J.cnt - ng, ng, bin, td.inv_bin, td.band, nhist, B, wlim,
                                         D, win, hp, ovf);
     }
-    // the work group's waves stay on one J tile, so that its records are
-    // read through the scalar cache once rather than once per wave
     __syncthreads();
   }
   flush_window<WCAP>(win, B, min(B + (uint32_t)WCAP, nhist), D, hp);
