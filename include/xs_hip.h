@@ -115,7 +115,8 @@ int xs_hist(xs_ctx *ctx, const void *xyz, size_t n_atoms,
 
 /* H3: histogram pair sums S[p*nq+iq] = sum_k h_p[k] sin(q r_k)/(q r_k + 1e-8),
  * r_k = (k+1/2) bin, empty bins skipped, S = 0 where q = 0.  FP64 terms,
- * sin advanced from bin to bin by a rotation, FP64 atomics across blocks.
+ * sin advanced from bin to bin by Reinsch's form of the three-term
+ * recurrence, FP64 atomics across blocks.
  * Replaces the bin loop of calcInt1DHist (CalcFunctions.cpp:844-871) and
  * calcIntHistKernelXray/Neutron (CalcFunctionsCUDA.cu:923-985).
  * XS_PAIRSUM_MFMA evaluates the same sums as one product S = H A^T on the
@@ -240,7 +241,10 @@ int xs_pairlist_pairsum(xs_ctx *ctx, const double *r, const unsigned *type, size
  * is host memory) and the structures are tiled on the GPU (radix sorts over
  * all atoms at once; fewer than 2^31 atoms in all); otherwise they are
  * tiled on the host's cores and uploaded.  Either way the call returns
- * when the plan is ready, and the coordinates are not needed afterwards. */
+ * when the plan is ready, and the coordinates are not needed afterwards.
+ * The plan holds on the device the sorted coordinates (24 or 12 bytes per
+ * atom), the tile tables and, for FP64 binning, 32 bytes of J records per
+ * atom (each atom relative to its tile's centre). */
 int xs_batch_plan(xs_ctx *ctx, unsigned nstruct, const unsigned *n_el,
                   const unsigned *el_count, const double *const *xyz,
                   double bin, const unsigned *nhist, unsigned flags,
