@@ -43,13 +43,13 @@ FP64_PEAK_TFLOPS = 78.6           # MI355X published FP64 vector peak (binning i
 VALU_ISSUE_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12   # non-packed lane-ops/s (39.3 T)
 OPS_PER_PAIR = 11                 # 3 sub, 3 mul, 2 add, sqrt, div, cvt
 # VALU issue cycles per 64 pairs of the histogram kernel, from rocprofv3 SQ
-# counters on this workload: 11.64 VALU wave-instructions per 64 pairs, 1.04
+# counters on this workload: 11.68 VALU wave-instructions per 64 pairs, 1.04
 # of them v_rsq_f64, which occupies the issue port four times as long as
-# v_fma_f64 (3 extra cycles); SQ_ACTIVE_INST_VALU reads 14.75.  A CU
+# v_fma_f64 (3 extra cycles); SQ_ACTIVE_INST_VALU reads 14.79.  A CU
 # issues one 64-lane VALU instruction per cycle (4 SIMDs x 16 lanes), so the
 # fraction of the CUs' issue cycles at the 2.4 GHz peak clock the kernel
 # spends issuing VALU work is SLOTS_PER_PAIR * pairs / 64 / (256 * 2.4e9 * t).
-SLOTS_PER_PAIR = 14.75
+SLOTS_PER_PAIR = 14.79
 CU_ISSUE_PER_S = 256 * 2.4e9
 # FP32 binning kernel (hist_kernel_f32<32768, true>) on the same workload:
 # SQ_ACTIVE_INST_VALU reads 23.86 wave-instructions per 64 pairs.  Its
@@ -72,7 +72,7 @@ AMP2D_OPS_PER_TERM = 7
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (KiB per dispatch), scaled by
 # the counters' reading for a 1 GiB device-to-device copy (FETCH_SIZE reads
 # 524,308 KiB for the 1,048,576 KiB copied, WRITE_SIZE 1,048,576 KiB).
-HIST_FETCH_KIB, HIST_WRITE_KIB = 6985296, 1910070
+HIST_FETCH_KIB, HIST_WRITE_KIB = 7746906, 9982160
 FETCH_SCALE, WRITE_SCALE = 1048576 / 524308, 1.0
 HIST_TRAFFIC_BYTES = int((HIST_FETCH_KIB * FETCH_SCALE + HIST_WRITE_KIB * WRITE_SCALE) * 1024)
 
@@ -984,8 +984,9 @@ def main():
                          SLOTS_PER_PAIR * pairs / 64 / (CU_ISSUE_PER_S * hist_ms * 1e-3), 4),
                      "traffic_note": "HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE, copy-calibrated); "
                                      "J records read per work item through the scalar cache, their "
-                                     "L2 misses served from MALL/HBM, and histogram window flushes: "
-                                     "~16 GB at ~70 GB/s, 0.9% of HBM bandwidth"},
+                                     "L2 misses served from MALL/HBM, and histogram window flushes "
+                                     "(device-scope atomics, 64 B each): ~26 GB at ~120 GB/s, 1.5% "
+                                     "of HBM bandwidth"},
         "kernels_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in kern.items()},
         "f32_binning": {"ms_per_step": round(ms32, 3),
                         "gpairs_per_s": round(pairs / (ms32 * 1e-3) / 1e9, 3),

@@ -1527,7 +1527,7 @@ static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_co
   std::vector<TypeDesc> types;
   add_types(n_el, ebase, en, nhist, bin, 0ull, 0u, types);
   std::vector<Item> items;
-  add_items(types, 0, (uint32_t)types.size(), items, sizeof(T) == 8);
+  add_items(types, 0, (uint32_t)types.size(), items, sizeof(T) == 8, kSingleSplit);
   order_items(items, types, tiles, sizeof(T) == 8);
 
   // 4. tables (exact sizes) in the table buffer; the stream is idle here
@@ -1599,7 +1599,7 @@ static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_coun
   std::vector<TypeDesc> types;
   add_types(n_el, ebase, en, nhist, bin, 0ull, 0u, types);
   std::vector<Item> items;
-  add_items(types, 0, (uint32_t)types.size(), items, sizeof(T) == 8);
+  add_items(types, 0, (uint32_t)types.size(), items, sizeof(T) == 8, kSingleSplit);
   order_items(items, types, tiles, sizeof(T) == 8);
 
   const size_t nt = std::max<size_t>(tiles.size(), 1), ni = std::max<size_t>(items.size(), 1);

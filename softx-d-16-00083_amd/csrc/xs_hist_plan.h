@@ -16,6 +16,10 @@ namespace hist {
 constexpr int kTile = 1024;      // atoms per tile == threads per work group
 constexpr int kSortCap = 2048;   // J tiles sorted per work item
 constexpr int kMaxEl = 16;
+// J-run split of a single structure's work items (add_items `split`): the
+// items of one structure are nearly equal, so the last round of work groups
+// leaves CUs idle unless there are many; a batch plan has items enough.
+constexpr uint32_t kSingleSplit = 8;
 
 struct Tile {
   uint32_t off, cnt;  // atom range in the sorted arrays
@@ -67,9 +71,10 @@ static_assert((uint64_t)max_jrun(false) * kTile * kTile < (1ull << 32), "item pa
 // n = units), so each unordered unit pair is visited once and items have
 // equal length; the J run is the tiles of those units.  Cross-element pairs:
 // each I unit against the J tiles in parts.  Paired items are split twice as
-// finely, so that there are as many items as unpaired ones.
+// finely, so that there are as many items as unpaired ones; `split` divides
+// every J run further.
 inline void add_items(const std::vector<TypeDesc> &types, uint32_t first, uint32_t last,
-                      std::vector<Item> &items, bool paired) {
+                      std::vector<Item> &items, bool paired, uint32_t split = 1) {
   for (uint32_t t = first; t < last; ++t) {
     const TypeDesc &d = types[t];
     if (!d.a_n || !d.b_n) continue;
@@ -85,13 +90,14 @@ inline void add_items(const std::vector<TypeDesc> &types, uint32_t first, uint32
           // the last unit holds one tile when nt is odd
           if ((nt & 1) && (nu - 1 + nu - u) % nu <= h) --total;
         }
-        const uint32_t chunk = std::min<uint32_t>(paired ? (total + 3) / 4 : total, max_jrun(paired));
+        const uint32_t parts = (paired ? 4 : 1) * split;
+        const uint32_t chunk = std::min<uint32_t>((total + parts - 1) / parts, max_jrun(paired));
         for (uint32_t c = 0; c < total; c += chunk)
           items.push_back(Item{t, u, (start + c) % nt, std::min<uint32_t>(chunk, total - c)});
       }
     } else {
       const uint32_t nu = paired ? (d.a_n + 1) / 2 : d.a_n;
-      const uint32_t parts = std::min<uint32_t>(d.b_n, paired ? 8u : 2u);
+      const uint32_t parts = std::min<uint32_t>(d.b_n, (paired ? 8u : 2u) * split);
       const uint32_t per = std::min<uint32_t>((d.b_n + parts - 1) / parts, max_jrun(paired));
       for (uint32_t u = 0; u < nu; ++u)
         for (uint32_t j0 = 0; j0 < d.b_n; j0 += per)

@@ -59,10 +59,10 @@ static void check_bound(uint32_t a_n, uint32_t b_n, bool same, bool paired) {
 
 // Every (I tile, J tile) visit of the items, counted per unordered pair
 // (like elements) or per ordered pair (cross elements).
-static void check_coverage(uint32_t a_n, uint32_t b_n, bool same, bool paired) {
+static void check_coverage(uint32_t a_n, uint32_t b_n, bool same, bool paired, uint32_t split) {
   const auto types = one_type(a_n, b_n, same);
   std::vector<Item> items;
-  add_items(types, 0, 1, items, paired);
+  add_items(types, 0, 1, items, paired, split);
   const uint32_t nb = same ? a_n : b_n;
   std::vector<uint32_t> seen((size_t)a_n * nb, 0u);
   auto visit = [&](uint32_t i, uint32_t j) {
@@ -88,17 +88,19 @@ static void check_coverage(uint32_t a_n, uint32_t b_n, bool same, bool paired) {
   for (uint32_t i = 0; i < a_n; ++i)
     for (uint32_t j = same ? i : 0; j < nb; ++j)
       if (seen[(size_t)i * nb + j] != 1) ++bad;
-  CHECK(bad == 0, "%zu tile pairs not visited exactly once (a_n %u b_n %u same %d paired %d)", bad,
-        a_n, b_n, (int)same, (int)paired);
+  CHECK(bad == 0, "%zu tile pairs not visited exactly once (a_n %u b_n %u same %d paired %d split %u)",
+        bad, a_n, b_n, (int)same, (int)paired, split);
 }
 
 int main() {
-  // coverage on small and ragged tile counts
-  for (uint32_t n : {1u, 2u, 3u, 4u, 5u, 7u, 8u, 64u, 65u, 513u, 1000u})
-    for (bool paired : {false, true}) {
-      check_coverage(n, 0, true, paired);
-      check_coverage(n, n / 3 + 1, false, paired);
-    }
+  // coverage on small and ragged tile counts, batch (1) and single-structure
+  // (kSingleSplit) splits of the J runs
+  for (uint32_t split : {1u, kSingleSplit})
+    for (uint32_t n : {1u, 2u, 3u, 4u, 5u, 7u, 8u, 64u, 65u, 513u, 1000u})
+      for (bool paired : {false, true}) {
+        check_coverage(n, 0, true, paired, split);
+        check_coverage(n, n / 3 + 1, false, paired, split);
+      }
   // the bound where the J runs would be longest: elements of 2 * 10^7 atoms
   // (20,001 tiles) and more
   for (bool paired : {false, true}) {
