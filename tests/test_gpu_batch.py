@@ -180,6 +180,33 @@ def test_batch_given_nhist_and_overflow(bctx, ensemble):
     b.close()
 
 
+@pytest.mark.parametrize("device", [False, True])
+def test_batch_grids_beyond_fixed_point_limit(bctx, ensemble, device):
+    """A plan whose largest grid exceeds 2^19 bins runs the FP64 kernel with
+    the fract form of the fast path for all its structures (the fixed-point
+    form takes grids of up to 2^19 bins): given bin counts of 2^19 + 7 and
+    1,000-3,000 over four structures, host and device plans, bit-exact."""
+    import torch
+    import xansons_hip as xh
+    sub = ensemble[:4]
+    nh = [(1 << 19) + 7, 1000, 2000, 3000]
+    if device:
+        dev = torch.device("cuda", 0)
+        keep = [torch.from_numpy(np.ascontiguousarray(e.xyz64())).to(dev) for e in sub]
+        b = xh.Batch(bctx, [(t.data_ptr(), e.el_count) for t, e in zip(keep, sub)], BIN, nhist=nh,
+                     f64=True, device=True)
+    else:
+        b = xh.Batch(bctx, [(e.xyz64(), e.el_count) for e in sub], BIN, nhist=nh, f64=True)
+    hist, _, _ = b.run()
+    ov = b.overflow()
+    for s, (e, (h, _, _)) in enumerate(zip(sub, split(b, sub, hist, None, None, 0))):
+        ho, ovo = O.hist_f64(e.xyz64(), e.el_count, BIN, nh[s])
+        print(f"struct {s}: nhist={nh[s]} mismatches={int(np.count_nonzero(h != ho))}", flush=True)
+        assert np.array_equal(h, ho), s
+        assert int(ov[s]) == ovo
+    b.close()
+
+
 def test_batch_graph_replays_match_eager(bctx, ensemble):
     import torch
     import xansons_hip as xh
