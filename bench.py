@@ -52,7 +52,7 @@ OPS_PER_PAIR = 11                 # 3 sub, 3 mul, 2 add, sqrt, div, cvt
 SLOTS_PER_PAIR = 14.79
 CU_ISSUE_PER_S = 256 * 2.4e9
 # FP32 binning kernel (hist_kernel_f32<32768, true>) on the same workload:
-# SQ_ACTIVE_INST_VALU reads 23.86 wave-instructions per 64 pairs.  Its
+# SQ_ACTIVE_INST_VALU reads 23.89 wave-instructions per 64 pairs.  Its
 # roofline fraction is the algorithmic one of the FP64 line: 11 ops per pair
 # x pairs / t against MI355X's FP32 vector peak (157.3 TFLOP/s).  Beside it,
 # the issue rate it reaches in wave-instructions per CU-cycle at 2.4 GHz:
@@ -60,7 +60,7 @@ CU_ISSUE_PER_S = 256 * 2.4e9
 # (measured), i.e. up to two FP32 wave-instructions per CU-cycle against one
 # for FP64, so each kernel's issue rate is read against the peak of its own
 # instruction type (2 for this one, 1 for the FP64 kernel).
-F32_ACTIVE_PER_64_PAIRS = 23.86
+F32_ACTIVE_PER_64_PAIRS = 23.89
 FP32_PEAK_TFLOPS = 157.3
 # Direct Debye and 2D units (one op per sin, cos, add, multiply): 4 ops per
 # pair-q term (sin, q r, x 1/(q r), accumulate) and 7 per complex-exponential
