@@ -290,6 +290,26 @@ def test_fp64_grid_at_the_fixed_point_limit(ctx, nhist):
     assert np.array_equal(devh, want)
 
 
+def test_fp64_distances_beyond_2p20_bins_on_a_small_grid(ctx):
+    """A given grid of 200,000 bins (the fixed-point fast path) with
+    distances of up to 1.9 million bins: estimates at or above 2^20 bins leave
+    the fixed-point range of the fast path; those pairs lie past the grid and
+    must all reach the overflow counter, the rest their bins.  Host and
+    device paths, bit-exact, overflow equal."""
+    rng = np.random.Generator(np.random.PCG64(4099))
+    x64 = np.ascontiguousarray(rng.uniform(-16.0, 16.0, (3, 1500)))
+    el, bin_, nh = [800, 700], 3e-5, 200000
+    assert O.hist_size(x64, bin_) > (1 << 20)
+    want, oo = O.hist_f64(x64, el, bin_, nh)
+    host, ov_h, devh, ov_d = _hist64_host_and_device(ctx, x64, el, bin_, nh)
+    print(f"overflow={oo} in-grid={int(want.sum())} host mismatched={int(np.sum(host != want))} "
+          f"device mismatched={int(np.sum(devh != want))} ovf {ov_h}/{ov_d}", flush=True)
+    assert oo > 0
+    assert ov_h == oo and ov_d == oo
+    assert np.array_equal(host, want)
+    assert np.array_equal(devh, want)
+
+
 def test_fp64_coincident_atoms_of_two_elements(ctx):
     """Two elements on the same sites (a mixed site written out as two atoms):
     the cross-type pairs lie at distance 0, where the tile-centred squared
