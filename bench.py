@@ -77,6 +77,39 @@ FETCH_SCALE, WRITE_SCALE = 1048576 / 524308, 1.0
 HIST_TRAFFIC_BYTES = int((HIST_FETCH_KIB * FETCH_SCALE + HIST_WRITE_KIB * WRITE_SCALE) * 1024)
 
 
+def rel_errors(got, ref, floor=1e-3):
+    """Error of a result against its FP64 evaluation: the largest pointwise
+    relative error over the points where |ref| >= floor * max |ref|, and the
+    L2-relative error ||got - ref|| / ||ref|| over all points."""
+    got, ref = np.ravel(got).astype(np.float64), np.ravel(ref).astype(np.float64)
+    m = np.abs(ref) >= floor * np.abs(ref).max()
+    return {"max_rel": float((np.abs(got - ref)[m] / np.abs(ref[m])).max()),
+            "l2_rel": float(np.linalg.norm(got - ref) / np.linalg.norm(ref))}
+
+
+def fp64_pairsums(h, bin_, q, chunk=32):
+    """Histogram pair sums in FP64 numpy, S_p(q) = sum_k h_p[k] sin(q r_k) /
+    (q r_k + 1e-8) with r_k = (k + 1/2) bin (CalcFunctions.cpp:856-862)."""
+    h = np.atleast_2d(h)
+    S = np.zeros((h.shape[0], q.size))
+    for p in range(h.shape[0]):
+        k = np.nonzero(h[p])[0]
+        r = (k + 0.5) * bin_
+        w = h[p, k].astype(np.float64)
+        for i in range(0, q.size, chunk):
+            qr = np.outer(q[i:i + chunk], r)
+            S[p, i:i + chunk] = (np.sin(qr) / (qr + 1e-8)) @ w
+    return S
+
+
+def intensity_two_elements(S, el_count, sl):
+    """I(q) of a two-element structure from its three pair sums (types 00, 01,
+    11) and scattering lengths: (sum_e n_e sl_e^2 + 2 sum_ab sl_a sl_b S_ab) / N."""
+    n = sum(el_count)
+    return (el_count[0] * sl[0] ** 2 + el_count[1] * sl[1] ** 2
+            + 2 * (sl[0] * sl[0] * S[0] + sl[0] * sl[1] * S[1] + sl[1] * sl[1] * S[2])) / n
+
+
 def gpu_step_setup(ctx, ens, nh, q, f64):
     import torch
     dev = torch.device("cuda", 0)
@@ -96,7 +129,7 @@ def gpu_step_setup(ctx, ens, nh, q, f64):
         ctx.hist_pairsum_dev(hist.data_ptr(), P, nh, BIN, qd.data_ptr(), q.size, S.data_ptr())
         ctx.pdf_dev(hist.data_ptr(), ens.el_count, nh, BIN, "rPDF", rho0, sl, pdf.data_ptr())
 
-    return step, hist, ovf
+    return step, hist, ovf, S
 
 
 def time_steps(ctx, step, steps, warmup):
@@ -294,7 +327,7 @@ def ensemble_leg(ctx, count, steps, warmup, use_graph=True):
     # sum_e N_e sl_e^2 + 2 sum_{a<=b} sl_a sl_b S_ab, over N), pointwise
     # where I >= 1e-3 max I
     S_h, Sm_h = S.cpu().numpy(), Sm.cpu().numpy()
-    im_rel, e0 = 0.0, 0
+    im_rel, im_l2, e0 = 0.0, 0.0, 0
     for s_i, p in enumerate(parts):
         sle, ec = sl[e0:e0 + p.n_el], p.el_count
         e0 += p.n_el
@@ -305,9 +338,8 @@ def ensemble_leg(ctx, count, steps, warmup, use_graph=True):
         def inten(SS):
             rows = SS[t0_ * nq:(t0_ + len(w)) * nq].reshape(len(w), nq)
             return (self_ + 2.0 * (np.asarray(w)[:, None] * rows).sum(axis=0)) / p.n
-        Ie, Ig = inten(S_h), inten(Sm_h)
-        m = np.abs(Ie) >= 1e-3 * np.abs(Ie).max()
-        im_rel = max(im_rel, float(np.max(np.abs(Ig[m] - Ie[m]) / np.abs(Ie[m]))))
+        e_ = rel_errors(inten(Sm_h), inten(S_h))
+        im_rel, im_l2 = max(im_rel, e_["max_rel"]), max(im_l2, e_["l2_rel"])
 
     # structure by structure, same kernels, outputs in the batch layout
     hist2 = torch.empty_like(hist)
@@ -403,6 +435,7 @@ def ensemble_leg(ctx, count, steps, warmup, use_graph=True):
             "pairsum_ms": pairsum_mfma_ms,
             "S_max_rel_diff_vs_fp64_pairsums": sm_rel,
             "I_max_rel_diff_vs_fp64_pairsums": im_rel,
+            "I_l2_rel_diff_vs_fp64_pairsums": im_l2,
             "note": "the same plan and runs with XS_PAIRSUM_MFMA: pair sums of all pair types as "
                     "int8 matrix-core products (types grouped by count slices and bin count); "
                     "histograms and PDFs unchanged"},
@@ -482,21 +515,22 @@ def pairsum_matrix_leg(ctx, steps):
     def s_err(S):
         return float((np.abs(S[:cols] - Se).max(axis=1) / np.abs(Se).max(axis=1)).max())
 
-    def i_err(S):
-        worst = 0.0
+    def i_errs(S):
+        """Worst pointwise and worst L2-relative error of I(q) over the particles."""
+        worst = {"max_rel": 0.0, "l2_rel": 0.0}
         for p_i in range(cols // 3):
             ec = parts[p_i].el_count
             sl = (3.63, 9.577)
-            n = sum(ec)
-
-            def inten(SS):
-                return (ec[0] * sl[0] ** 2 + ec[1] * sl[1] ** 2
-                        + 2 * (sl[0] * sl[0] * SS[0] + sl[0] * sl[1] * SS[1] + sl[1] * sl[1] * SS[2])) / n
-            Ie = inten(Se[3 * p_i:3 * p_i + 3])
-            Ig = inten(S[3 * p_i:3 * p_i + 3])
-            m = np.abs(Ie) >= 1e-3 * np.abs(Ie).max()
-            worst = max(worst, float((np.abs(Ig - Ie)[m] / np.abs(Ie[m])).max()))
+            e = rel_errors(intensity_two_elements(S[3 * p_i:3 * p_i + 3], ec, sl),
+                           intensity_two_elements(Se[3 * p_i:3 * p_i + 3], ec, sl))
+            worst = {k: max(worst[k], e[k]) for k in worst}
         return worst
+
+    def i_err(S):
+        return i_errs(S)["max_rel"]
+
+    def i_l2(S):
+        return i_errs(S)["l2_rel"]
 
     cmax = int(H.max())
     nb = 1
@@ -529,8 +563,11 @@ def pairsum_matrix_leg(ctx, steps):
                                   "plain_fp32": s_err(Sfh)},
         "max_rel_err_I_vs_fp64": {"fp64_vector": i_err(Svh), "mfma": i_err(Smh),
                                   "plain_fp32": i_err(Sfh)},
+        "l2_rel_err_I_vs_fp64": {"fp64_vector": i_l2(Svh), "mfma": i_l2(Smh),
+                                 "plain_fp32": i_l2(Sfh)},
         "error_sample": f"first {cols // 3} particles ({cols} histograms), all {nq} q; "
-                        "S: max |dS| / max |S| per histogram; I: pointwise where I >= 1e-3 max I",
+                        "S: max |dS| / max |S| per histogram; I: pointwise where I >= 1e-3 max I, "
+                        "and ||dI|| / ||I|| (worst particle)",
     }
 
 
@@ -614,6 +651,7 @@ def amp2d_leg(ctx, nrot=256, nrot_list=16):
                        fd.data_ptr(), Id.data_ptr())
     ctx.synchronize()
     err = float(np.max(np.abs(Id.cpu().numpy()[idx] - Ie) / Ie))
+    err_l2 = rel_errors(Id.cpu().numpy()[idx], Ie)["l2_rel"]
     terms = ens.n * nqp * nrot
     terms_list = ens.n * nqp * nrot_list
     achieved = AMP2D_OPS_PER_TERM * terms / (kms * 1e-3) / 1e12
@@ -629,7 +667,7 @@ def amp2d_leg(ctx, nrot=256, nrot_list=16):
         "list_kernel_ms": round(kms_list, 2),
         "list_gterms_per_s": round(terms_list / (kms_list * 1e-3) / 1e9, 1),
         "checks": {"friedel_max_rel": fried, "grid_vs_list_max_rel": vs_list,
-                   "fp64_sample_max_rel": err,
+                   "fp64_sample_max_rel": err, "fp64_sample_l2_rel": err_l2,
                    "fp64_sample": "64 q points x 2 rotations, FP64 numpy sum",
                    "grid_vs_list": f"first {nrot_list} rotations, all q points"},
     }
@@ -679,6 +717,7 @@ def debye_leg(ctx):
             Se = np.array([np.sum(np.sin(qq * r) / (qq * r + 1e-8)) for qq in q])
             Sg = S.cpu().numpy()
             rec["max_rel_err_vs_fp64"] = float(np.max(np.abs(Sg - Se)) / np.max(np.abs(Se)))
+            rec["l2_rel_err_vs_fp64"] = rel_errors(Sg, Se)["l2_rel"]
         out[name] = rec
     return out
 
@@ -944,14 +983,24 @@ def main():
     ctx = xh.Context(0)
     ctx.set_stream(torch.cuda.current_stream().cuda_stream)
     # headline: FP64 binning, the reference CPU path's expression (bit-exact counts)
-    step, hist, ovf = gpu_step_setup(ctx, ens, nh, q, f64=True)
+    step, hist, ovf, S = gpu_step_setup(ctx, ens, nh, q, f64=True)
     ms, kern = time_steps(ctx, step, args.steps, args.warmup)
     h = hist.view(3, nh).cpu().numpy()
     totals_ok = [int(v) for v in h.sum(axis=1)] == [na * (na - 1) // 2, na * nb, nb * (nb - 1) // 2]
+    # I(q) against its FP64 evaluation from the same histograms, every 16th q
+    qs = slice(0, NQ, 16)
+    Sg = S.view(3, NQ).cpu().numpy()[:, qs]
+    Se = fp64_pairsums(h, BIN, q[qs])
+    sl = (3.63, 9.577)
+    iq_err = rel_errors(intensity_two_elements(Sg, ens.el_count, sl),
+                        intensity_two_elements(Se, ens.el_count, sl))
+    s_err = [rel_errors(Sg[p], Se[p], floor=0.0) for p in range(3)]
     del step
-    step32, hist32, _ = gpu_step_setup(ctx, ens, nh, q, f64=False)
+    step32, hist32, _, S32 = gpu_step_setup(ctx, ens, nh, q, f64=False)
     ms32, kern32 = time_steps(ctx, step32, max(1, args.steps // 2), 1)
-    del step32, hist32
+    iq32_err = rel_errors(intensity_two_elements(S32.view(3, NQ).cpu().numpy()[:, qs], ens.el_count, sl),
+                          intensity_two_elements(Se, ens.el_count, sl))
+    del step32, hist32, S32
 
     hist_ms = kern["hist"][0] / max(kern["hist"][1], 1)
     hist32_ms = kern32["hist"][0] / max(kern32["hist"][1], 1)
@@ -1001,8 +1050,14 @@ def main():
                         "valu_active_per_64_pairs": F32_ACTIVE_PER_64_PAIRS,
                         "valu_wave_instr_per_cu_cycle": round(
                             F32_ACTIVE_PER_64_PAIRS * pairs / 64 / (CU_ISSUE_PER_S * hist32_ms * 1e-3), 4),
-                        "valu_issue_peak_wave_instr_per_cu_cycle": 2},
-        "checks": {"pair_totals_exact": totals_ok, "overflow": int(ovf.item())},
+                        "valu_issue_peak_wave_instr_per_cu_cycle": 2,
+                        "I_vs_fp64": {k: float(f"{v:.3e}") for k, v in iq32_err.items()}},
+        "checks": {"pair_totals_exact": totals_ok, "overflow": int(ovf.item()),
+                   "I_vs_fp64": {k: float(f"{v:.3e}") for k, v in iq_err.items()},
+                   "S_vs_fp64_per_type": [{k: float(f"{v:.3e}") for k, v in e.items()} for e in s_err],
+                   "error_sample": f"{NQ // 16} q (every 16th); FP64 numpy pair sums of the same "
+                                   "histograms; max_rel: pointwise where |ref| >= 1e-3 max |ref| "
+                                   "(all points for S), l2_rel: ||got - ref|| / ||ref||"},
     }
     progress(f"headline done: {line['value']} Gpairs/s")
     if not args.no_matrix:
