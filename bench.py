@@ -150,6 +150,17 @@ def time_steps(ctx, step, steps, warmup):
     return (t1 - t0) * 1e3 / steps, kern
 
 
+def host_cpu():
+    """The host's CPU model and the CPU counts the baseline ran beside."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(m=36, a=5.64, sigma=0.08):
     """Time the reference's own CPU histogram (XaNSoNS_OMP built from the
     reference sources, its calcHist stage timer) on a bounded sample of the
@@ -162,6 +173,7 @@ def cpu_baseline(m=36, a=5.64, sigma=0.08):
     n = 8 * m ** 3
     pairs = n * (n - 1) // 2
     threads = max(1, min(16, os.cpu_count() or 1))
+    host = host_cpu()
     sample = (f"rock-salt cube of {m}^3 conventional cells ({n} atoms, {pairs:.3e} pairs; the "
               f"workload's crystal, a = {a} A, displacement sd {sigma} A), bin {BIN} A, "
               "histogram stage only")
@@ -181,12 +193,13 @@ def cpu_baseline(m=36, a=5.64, sigma=0.08):
                         '  <Block fractional="Yes">\n    <Atoms>\n' + atoms + '    </Atoms>\n'
                         f'    <CellVectors a="{a} 0 0" b="0 {a} 0" c="0 0 {a}" N="{m} {m} {m}"/>\n'
                         '  </Block>\n</Config>\n')
+            env = dict(os.environ, OMP_PROC_BIND="close")
             out = subprocess.run([exe, "start.xml", str(threads)], cwd=tmp, capture_output=True,
-                                 text=True, timeout=600, check=True).stdout
+                                 text=True, timeout=600, check=True, env=env).stdout
             t = float(re.search(r"Histogram calculation time: ([0-9.eE+-]+) s", out).group(1))
             return dict(value=pairs / t / 1e9, unit="Gpairs/s", cores=threads, kind="reference",
-                        sample=sample + "; reference XaNSoNS_OMP (calcHist, FP64), its own stage timer",
-                        seconds=t)
+                        sample=sample + "; reference XaNSoNS_OMP (calcHist, FP64), its own stage timer, "
+                        "OMP_PROC_BIND=close", seconds=t, host=host)
         finally:
             shutil.rmtree(tmp, ignore_errors=True)
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -203,7 +216,8 @@ def cpu_baseline(m=36, a=5.64, sigma=0.08):
     O.hist_f64(np.ascontiguousarray(xyz), [n // 2, n // 2], BIN, nh, threads)
     t = time.perf_counter() - t0
     return dict(value=pairs / t / 1e9, unit="Gpairs/s", cores=threads, kind="port",
-                sample=sample + "; C restatement of calcHist (FP64, zig-zag rows)", seconds=t)
+                sample=sample + "; C restatement of calcHist (FP64, zig-zag rows)", seconds=t,
+                host=host)
 
 
 def copy_probe(nbytes=1 << 30):

@@ -2,6 +2,9 @@
 by tests/golden/make_golden.py).  Printed reference values carry 7
 significant digits, so agreement is checked at rtol 1e-6; histogram counts
 recovered from the printed RDFs must match exactly."""
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -143,3 +146,19 @@ def test_2d_xray_polar():
                           5, 1.0, polar=True, ff=g["ff"])
     close(I2, g["I2D_xray_polar"])
     close(I1, g["I1D_xray_polar"])
+
+
+def test_cpu_calibration_record():
+    """The oracle's C restatement against the reference's own OpenMP
+    executable, side by side on the same inputs at 8 threads in the build
+    container (tests/golden/make_cpu_calibration.py): times within 15% of
+    each other, outputs equal to the reference's printed 7 digits."""
+    with open(os.path.join(os.path.dirname(__file__), "golden", "cpu_calibration.json")) as f:
+        cal = json.load(f)
+    assert cal["threads"] == 8
+    for case in ("debye_fcc1985", "hist_rocksalt64000"):
+        assert 0.85 <= cal[case]["port_over_reference"] <= 1.15, case
+    assert cal["debye_fcc1985"]["I_max_rel_diff"] < 1e-6
+    h = cal["hist_rocksalt64000"]
+    assert h["pair_total_port"] == h["pairs"]
+    assert h["rdf_max_rel_diff"] < 1e-6
