@@ -61,6 +61,30 @@ def test_ragged_sizes(ctx, counts):
     assert_same(r64, f"ragged {counts} f64")
 
 
+def test_sixteen_elements_and_seventeen_rejected(ctx):
+    """The most elements a structure may have (16, 136 pair types), with
+    empty and one-atom elements among them, both binnings bit-exact; a 17th
+    element is rejected with XS_ERR_RANGE by the single-structure call and
+    by a batch plan, before anything runs."""
+    import xansons_hip as xh
+    counts = [37, 0, 64, 1, 129, 5, 300, 2, 0, 77, 600, 3, 9, 200, 1, 50]
+    ens = st.random_cluster(counts, box=16.0, min_dist=0.3, seed=1616)
+    r32, r64 = both_modes(ctx, ens, 0.01)
+    assert r64[0].shape[0] == 136
+    assert_same(r32, "16 elements f32")
+    assert_same(r64, "16 elements f64")
+    el17 = counts + [4]
+    x17 = np.ascontiguousarray(np.concatenate([ens.xyz64(), np.full((3, 4), 9.0) + np.arange(4)], axis=1))
+    for f64 in (False, True):
+        x = x17 if f64 else x17.astype(np.float32)
+        with pytest.raises(xh.XSError) as e:
+            ctx.hist(x, el17, 0.01, 4000, f64=f64)
+        assert e.value.code == -5
+        with pytest.raises(xh.XSError) as e:
+            xh.Batch(ctx, [(ens.xyz64(), counts), (x17, el17)], 0.01, f64=f64)
+        assert e.value.code == -5
+
+
 def test_sliding_window_large_nhist(ctx):
     """nhist far above the LDS window: the window slides and flushes."""
     ens = st.rocksalt_cube(ncell=18, vacancy=0.02, sigma=0.08, seed_vac=31, seed_jit=32)
