@@ -1010,6 +1010,16 @@ def main():
                         intensity_two_elements(Se, ens.el_count, sl))
     s_err = [rel_errors(Sg[p], Se[p], floor=0.0) for p in range(3)]
     del step
+    # the same histogram from host buffers (the C ABI copies the coordinates
+    # in and the counts out): one warm-up call, then two timed
+    x_host = np.ascontiguousarray(ens.xyz64())
+    ctx.hist(x_host, ens.el_count, BIN, nh, f64=True)
+    t0 = time.perf_counter()
+    for _ in range(2):
+        h_host, _ = ctx.hist(x_host, ens.el_count, BIN, nh, f64=True)
+    host_ms = (time.perf_counter() - t0) * 1e3 / 2
+    host_equal = bool(np.array_equal(h_host, h))
+    del x_host, h_host
     step32, hist32, _, S32 = gpu_step_setup(ctx, ens, nh, q, f64=False)
     ms32, kern32 = time_steps(ctx, step32, max(1, args.steps // 2), 1)
     iq32_err = rel_errors(intensity_two_elements(S32.view(3, NQ).cpu().numpy()[:, qs], ens.el_count, sl),
@@ -1051,6 +1061,11 @@ def main():
                                      "(device-scope atomics, 64 B each): ~26 GB at ~120 GB/s, 1.5% "
                                      "of HBM bandwidth"},
         "kernels_ms_per_step": {k: round(v[0] / max(v[1], 1), 4) for k, v in kern.items()},
+        "host_buffers": {"ms": round(host_ms, 3), "gpairs_per_s": round(pairs / (host_ms * 1e-3) / 1e9, 3),
+                         "counts_equal": host_equal,
+                         "note": "xs_hist with host pointers (FP64 binning), PCIe copies of the "
+                                 "23.5 MB of coordinates in and the 2.3 MB of counts out included; "
+                                 "histogram only, not the pair sums and PDF of a step"},
         "f32_binning": {"ms_per_step": round(ms32, 3),
                         "gpairs_per_s": round(pairs / (ms32 * 1e-3) / 1e9, 3),
                         "hist_kernel_ms": round(hist32_ms, 3),

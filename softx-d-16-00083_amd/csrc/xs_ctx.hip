@@ -50,6 +50,10 @@ int ensure_matrix(xs_ctx *ctx, size_t bytes) {
   return ensure_device(ctx, &ctx->mat, &ctx->mat_size, bytes, size_t(1) << 20, "matrix");
 }
 
+int ensure_io(xs_ctx *ctx, size_t bytes) {
+  return ensure_device(ctx, &ctx->io, &ctx->io_size, bytes, size_t(1) << 20, "host-copy");
+}
+
 int ensure_pinned(xs_ctx *ctx, size_t bytes) {
   if (bytes <= ctx->pin_size) return XS_OK;
   size_t want = align_up(bytes + bytes / 4, size_t(1) << 16);
@@ -174,6 +178,7 @@ int xs_free(xs_ctx *ctx) {
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->aux) (void)hipFree(ctx->aux);
   if (ctx->mat) (void)hipFree(ctx->mat);
+  if (ctx->io) (void)hipFree(ctx->io);
   if (ctx->quot_check) (void)hipFree(ctx->quot_check);
   if (ctx->pin) (void)hipHostFree(ctx->pin);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);

@@ -1580,65 +1580,29 @@ static int run_device(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_co
   return launch_hist<T>(ctx, A, items.size(), wcap, true);
 }
 
+// Host coordinates: one synchronous copy to the device (a buffer of the
+// context apart from the workspace), the device path for everything else
+// (bounding box, equal-count tiles, J records, kernel), and the counts back
+// over the same stream.  The host only checks the span first, so a bad input
+// is rejected before anything is copied.
 template <class T>
 static int run_host(xs_ctx *ctx, const T *xyz, size_t n, const unsigned *el_count, unsigned n_el,
                     double bin, unsigned nhist, uint64_t *hist_out, uint64_t *ovf_out, bool quot) {
-  const unsigned ntypes = ntypes_of(n_el);
-  const size_t hsize = (size_t)ntypes * nhist;
+  const size_t hsize = (size_t)ntypes_of(n_el) * nhist;
   {
     double bbox[6];
     host_bbox<T>(xyz, n, bbox);
     XS_TRY(check_span(bbox, bin));
   }
-  std::vector<T> sx(n), sy(n), sz(n);
-  std::vector<Tile> tiles;
-  std::vector<double> boxes;
-  uint32_t ebase[kMaxEl], en[kMaxEl];
-  sort_structure_host<T>(xyz, n, el_count, n_el, bin, nhist, sx.data(), sy.data(), sz.data(), 0u,
-                         tiles, boxes, ebase, en);
-  std::vector<TypeDesc> types;
-  add_types(n_el, ebase, en, nhist, bin, 0ull, 0u, types);
-  std::vector<Item> items;
-  add_items(types, 0, (uint32_t)types.size(), items, sizeof(T) == 8, kSingleSplit);
-  order_items(items, types, tiles, sizeof(T) == 8);
-
-  const size_t nt = std::max<size_t>(tiles.size(), 1), ni = std::max<size_t>(items.size(), 1);
-  XS_TRY(ensure_pinned(ctx, 64));
-  const size_t nrec = sizeof(T) == 8 ? 4 * n : 0;  // FP64 J records
-  XS_TRY(ensure_workspace(ctx, Carver::need({3 * n * sizeof(T), nt * sizeof(Tile), nt * 48,
-                                             ntypes * sizeof(TypeDesc), ni * sizeof(Item),
-                                             hsize * 8, 8, nrec * sizeof(double)})));
-  Carver cv(ctx->ws);
-  T *sorted = cv.take<T>(3 * n);
-  Tile *tiles_d = cv.take<Tile>(nt);
-  double *boxes_d = cv.take<double>(nt * 6);
-  TypeDesc *types_d = cv.take<TypeDesc>(ntypes);
-  Item *items_d = cv.take<Item>(ni);
+  XS_TRY(ensure_io(ctx, Carver::need({3 * n * sizeof(T), hsize * 8, 8})));
+  Carver cv(ctx->io);
+  T *xyz_d = cv.take<T>(3 * n);
   unsigned long long *hist_d = cv.take<unsigned long long>(hsize);
   unsigned long long *ovf_d = cv.take<unsigned long long>(1);
-  double *rec = cv.take<double>(nrec);
-  // synchronous uploads from pageable host vectors
+  // the stream may still read the buffer from an earlier call
   XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-  XS_HIP_CHECK(hipMemcpy(sorted, sx.data(), n * sizeof(T), hipMemcpyHostToDevice));
-  XS_HIP_CHECK(hipMemcpy(sorted + n, sy.data(), n * sizeof(T), hipMemcpyHostToDevice));
-  XS_HIP_CHECK(hipMemcpy(sorted + 2 * n, sz.data(), n * sizeof(T), hipMemcpyHostToDevice));
-  if (!tiles.empty()) {
-    XS_HIP_CHECK(hipMemcpy(tiles_d, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
-    XS_HIP_CHECK(hipMemcpy(boxes_d, boxes.data(), boxes.size() * 8, hipMemcpyHostToDevice));
-  }
-  XS_HIP_CHECK(hipMemcpy(types_d, types.data(), types.size() * sizeof(TypeDesc), hipMemcpyHostToDevice));
-  if (!items.empty())
-    XS_HIP_CHECK(hipMemcpy(items_d, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice));
-  const unsigned zb = (unsigned)std::min<size_t>((hsize + 255) / 256, 4096);
-  hipLaunchKernelGGL(fill_u64, dim3(std::max(zb, 1u)), dim3(256), 0, ctx->stream, hist_d, hsize);
-  hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, ctx->stream, ovf_d, (size_t)1);
-  launch_records<T>(ctx->stream, sorted, sorted + n, sorted + 2 * n, tiles_d, boxes_d,
-                    tiles.size(), rec);
-  Args<T> A{sorted, sorted + n, sorted + 2 * n, tiles_d, boxes_d, types_d, items_d, hist_d, ovf_d,
-            rec};
-  A.quot_f32 = quot;
-  A.narrow = nhist <= kNarrowMax;
-  XS_TRY(launch_hist<T>(ctx, A, items.size(), wcap_for(nhist), true));
+  XS_HIP_CHECK(hipMemcpy(xyz_d, xyz, 3 * n * sizeof(T), hipMemcpyHostToDevice));
+  XS_TRY(run_device<T>(ctx, xyz_d, n, el_count, n_el, bin, nhist, hist_d, ovf_d, quot));
   XS_HIP_CHECK(hipMemcpyAsync(hist_out, hist_d, hsize * 8, hipMemcpyDeviceToHost, ctx->stream));
   XS_HIP_CHECK(hipMemcpyAsync(ctx->pin, ovf_d, 8, hipMemcpyDeviceToHost, ctx->stream));
   XS_HIP_CHECK(hipStreamSynchronize(ctx->stream));

@@ -113,6 +113,10 @@ struct xs_ctx {
   // Device buffer of the matrix-core pair sums (operand slices), grown on demand.
   void *mat = nullptr;
   size_t mat_size = 0;
+  // Device copies of host inputs and outputs (coordinates in, counts out) of
+  // calls that take host pointers, apart from the workspace, grown on demand.
+  void *io = nullptr;
+  size_t io_size = 0;
   // Pinned host staging for small uploads (item lists, parameter tables).
   void *pin = nullptr;
   size_t pin_size = 0;
@@ -147,6 +151,8 @@ int ensure_workspace(xs_ctx *ctx, size_t bytes);
 int ensure_aux(xs_ctx *ctx, size_t bytes);
 // Ensure ctx->mat holds at least `bytes` (synchronises the stream if it grows).
 int ensure_matrix(xs_ctx *ctx, size_t bytes);
+// Ensure ctx->io holds at least `bytes` (synchronises the stream if it grows).
+int ensure_io(xs_ctx *ctx, size_t bytes);
 // Ensure ctx->pin holds at least `bytes` (synchronises the stream if it grows).
 int ensure_pinned(xs_ctx *ctx, size_t bytes);
 

@@ -175,3 +175,49 @@ def test_rearrangement_equals_reference(tmp, case):
         want = np.loadtxt(text.splitlines(), ndmin=2)[:, 1]
         got = col(tmp, fn)
         assert rel_err(got, want)[0] < 2e-6, fn
+
+
+OMP_EXE = os.path.join(REPO, "oracle", "_ref", "XaNSoNS_OMP")
+
+
+@needs_exe
+@pytest.mark.skipif(not os.path.exists(OMP_EXE), reason="reference CPU executable not built")
+def test_histogram_stage_against_reference_cpu(tmp):
+    """The reference's own driver with the drop-in and the reference CPU
+    executable (16 threads) on one input: a rock-salt cube of 36^3
+    conventional cells (373,248 atoms, 7.0e10 pairs) built by the
+    reference's ensemble builder from an 8-atom cell with Uiso = 0 (so both
+    programs place the same atoms), RDF at 0.005 A.  The printed RDFs agree
+    to their 7 digits; both programs' own stage timers are printed.  The
+    drop-in's histogram stage includes what a first call costs: creating the
+    GPU context, the host copy of the coordinates, the copies over PCIe and
+    the tiling."""
+    fcc = [(0, 0, 0), (0.5, 0.5, 0), (0.5, 0, 0.5), (0, 0.5, 0.5)]
+    atoms = "".join(f'      <Atom name="{nm}" r="{f[0] + dx} {f[1]} {f[2]}"/>\n'
+                    for nm, dx in (("Na", 0.0), ("Cl", 0.5)) for f in fcc)
+    with open(os.path.join(tmp, "sl.txt"), "w") as f:
+        f.write("Na 3.63\nCl 9.577\n")
+    xml = ('<?xml version="1.0"?>\n<Config>\n  <Calculation name="H" source="neutron" '
+           'scenario="PDFonly" FFfilename="sl.txt" hist_bin="0.005">\n'
+           '    <PDF type="RDF" density="0.05"/>\n  </Calculation>\n'
+           '  <Block fractional="Yes">\n    <Atoms>\n' + atoms + '    </Atoms>\n'
+           '    <CellVectors a="5.64 0 0" b="0 5.64 0" c="0 0 5.64" N="36 36 36"/>\n'
+           '  </Block>\n</Config>\n')
+    pat = r"(?:Histogram|Total) calculation time: ([0-9.eE+-]+) s"
+    out = run(tmp, xml)
+    got = col(tmp, "H_neut_RDF.txt")
+    t_drop = [float(v) for v in re.findall(pat, out)]
+    os.remove(os.path.join(tmp, "H_neut_RDF.txt"))
+    threads = max(1, min(16, os.cpu_count() or 1))
+    ref = subprocess.run([OMP_EXE, "start.xml", str(threads)], cwd=tmp, capture_output=True, text=True,
+                         timeout=600, env=dict(os.environ, OMP_PROC_BIND="close"))
+    assert ref.returncode == 0, ref.stdout[-2000:] + ref.stderr[-2000:]
+    want = col(tmp, "H_neut_RDF.txt")
+    t_ref = [float(v) for v in re.findall(pat, ref.stdout)]
+    print(f"373,248 atoms, 6.966e+10 pairs; histogram / total stage seconds: drop-in {t_drop}, "
+          f"reference ({threads} threads) {t_ref}", flush=True)
+    assert got.shape == want.shape
+    m = want != 0
+    assert np.array_equal(got == 0, want == 0)
+    assert np.max(np.abs(got[m] - want[m]) / np.abs(want[m])) < 1e-6
+    assert t_drop and t_ref
