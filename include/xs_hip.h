@@ -107,6 +107,9 @@ int xs_profile_read(xs_ctx *ctx, const char *kernel, double *total_ms,
  * significands has passed (once per bin and context; it synchronises), with
  * FP32 operations only, which is faster; XS_F32_DOUBLE_QUOTIENT keeps the
  * double product.  Both give the IEEE quotient, so the counts are the same.
+ * Host arrays pass through a device buffer that the context keeps until
+ * xs_free (3 * n_atoms coordinates and ntypes * nhist counts); the work
+ * between the copies is that of a device-pointer call.
  * Replaces calcHist (CalcFunctions.cpp:757-809) and calcHistCuda
  * (CalcFunctionsCUDA.cu:855-920). */
 int xs_hist(xs_ctx *ctx, const void *xyz, size_t n_atoms,
