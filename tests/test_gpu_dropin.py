@@ -189,9 +189,10 @@ def test_histogram_stage_against_reference_cpu(tmp):
     reference's ensemble builder from an 8-atom cell with Uiso = 0 (so both
     programs place the same atoms), RDF at 0.005 A.  The printed RDFs agree
     to their 7 digits; both programs' own stage timers are printed.  The
-    drop-in's histogram stage includes what a first call costs: creating the
-    GPU context, the host copy of the coordinates, the copies over PCIe and
-    the tiling."""
+    drop-in's histogram stage is its process's first histogram call: the
+    host copy of the coordinates, the copies over PCIe, the tiling and the
+    first launches of the histogram kernels (the GPU context already exists:
+    the drop-in's ensemble builder made it)."""
     fcc = [(0, 0, 0), (0.5, 0.5, 0), (0.5, 0, 0.5), (0, 0.5, 0.5)]
     atoms = "".join(f'      <Atom name="{nm}" r="{f[0] + dx} {f[1]} {f[2]}"/>\n'
                     for nm, dx in (("Na", 0.0), ("Cl", 0.5)) for f in fcc)
